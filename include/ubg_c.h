@@ -1,0 +1,200 @@
+/* ubg_c.h -- C ABI of the MI355X-native ubootgl fluid step (libubg.so).
+ *
+ * The boundary replaces the reference's in-process C++ classes
+ *   Simulation        simulation.hpp:18-141
+ *   MG (pressure solver, there is no class named PressureSolver)
+ *                     pressure_solver.hpp:13-76
+ *   rbgs / calculateResidualField   pressure_solver.hpp:3-11
+ * with opaque handles that own device-resident fields.  Every entry point
+ * returns an int status (UBG_OK = 0); no C++ exception crosses the ABI and the
+ * message of the last failure on the calling thread is ubg_last_error().
+ *
+ * Fields are float32 row-major arrays, index y*width + x, exactly the layout of
+ * Single2DGrid (db2dgrid.hpp:12-50).  Staggered sizes: vx is (W-1) x H, vy is
+ * W x (H-1), everything else W x H.  Flags are 0/1 (any other value is
+ * rejected with UBG_E_FLAG_VALUE); on the device they are stored as one byte
+ * per cell.
+ *
+ * Threading: one handle is owned by one host thread at a time; calls on a
+ * handle are ordered on the handle's own HIP stream.  Host-pointer entry
+ * points copy synchronously; compute entry points are asynchronous until
+ * ubg_*_sync() or a host copy.
+ */
+#ifndef UBG_C_H
+#define UBG_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UBG_API __attribute__((visibility("default")))
+
+/* ---- status codes ------------------------------------------------------ */
+#define UBG_OK 0
+#define UBG_E_INVALID 1          /* bad argument or handle                  */
+#define UBG_E_DOMAIN_TOO_SMALL 2 /* W or H < 8: the reference's MG indexes   */
+                                 /* past its level arrays there             */
+#define UBG_E_FLAG_VALUE 3       /* a flag value other than 0 or 1          */
+#define UBG_E_HIP 4              /* HIP runtime failure                     */
+#define UBG_E_OOM 5              /* device allocation failed                */
+#define UBG_E_NONFINITE 6        /* NaN/Inf detected in a norm              */
+#define UBG_E_BUDGET 7           /* request exceeds the memory budget       */
+#define UBG_E_NOT_CONVERGED 8    /* iteration limit reached                 */
+#define UBG_E_STATE 9            /* call not valid in the current state     */
+#define UBG_E_NO_DEVICE 10       /* no HIP device visible                   */
+
+/* ---- field, stage and operation ids ------------------------------------ */
+enum {
+  UBG_FIELD_VX = 0,      /* DoubleBuffered2DGrid vx, front  (simulation.hpp:128) */
+  UBG_FIELD_VX_BACK,     /* vx back buffer: live state (db2dgrid.hpp:52-109)     */
+  UBG_FIELD_VY,
+  UBG_FIELD_VY_BACK,
+  UBG_FIELD_VX_ACCUM,    /* simulation.hpp:129 */
+  UBG_FIELD_VY_ACCUM,
+  UBG_FIELD_VX_CURRENT,  /* simulation.hpp:130 */
+  UBG_FIELD_VY_CURRENT,
+  UBG_FIELD_P,           /* simulation.hpp:132 */
+  UBG_FIELD_F,
+  UBG_FIELD_FLAG,
+  UBG_FIELD_R,
+  UBG_FIELD_COUNT
+};
+
+enum {
+  UBG_STAGE_ACCUM = 0,        /* applyAccumulatedVelocity  simulation.cpp:376-396 */
+  UBG_STAGE_DIFFUSE,          /* diffuse                   simulation.cpp:104-162 */
+  UBG_STAGE_ADVECT,           /* advect                    simulation.cpp:241-354 */
+  UBG_STAGE_SET_VBCS,         /* setVBCs                   simulation.cpp:80-102  */
+  UBG_STAGE_PROJECT,          /* project                   simulation.cpp:164-208 */
+  UBG_STAGE_SET_PBC,          /* setPBC                    simulation.cpp:36-45   */
+  UBG_STAGE_SAVE_CURRENT,     /* saveCurrentVelocityFields simulation.cpp:16-19   */
+  UBG_STAGE_DIVERGENCE,       /* project's divergence + sinks simulation.cpp:166-187 */
+  UBG_STAGE_CORRECT_VELOCITY, /* project's correction      simulation.cpp:196-207 */
+  UBG_STAGE_MG_SOLVE,         /* one mg.solve(p,f,flag,h,true) simulation.cpp:189 */
+  UBG_STAGE_COUNT
+};
+
+enum {
+  UBG_MGOP_RBGS = 0,          /* rbgs, one red+black sweep   pressure_solver.cpp:49-89   */
+  UBG_MGOP_ZERO_GRADIENT,     /* setZeroGradientBC           pressure_solver.cpp:183-192 */
+  UBG_MGOP_RESIDUAL,          /* calculateResidualField -> r pressure_solver.cpp:91-116  */
+  UBG_MGOP_RESTRICT_RESIDUAL, /* residual + restrict into level+1's f, level+1's p = 0
+                                 pressure_solver.cpp:217-227 */
+  UBG_MGOP_PROLONG_CORRECT,   /* prolongate level+1's p, add to level's p
+                                 pressure_solver.cpp:231-235 */
+  UBG_MGOP_COUNT
+};
+
+enum { UBG_LEVEL_P = 0, UBG_LEVEL_F = 1, UBG_LEVEL_R = 2 };
+
+/* Solver parameters; ubg_params_default() gives the reference's values. */
+typedef struct ubg_params {
+  int nu_pre;           /* pre-smoothing sweeps, 3   (pressure_solver.cpp:211) */
+  int nu_post;          /* post-smoothing sweeps, 3  (pressure_solver.cpp:243) */
+  int nu_coarse;        /* sweeps on level L-2, 5    (pressure_solver.cpp:204) */
+  int vcycles_per_step; /* mg.solve calls per step, 2 (simulation.cpp:189-190) */
+  float alpha;          /* smoother relaxation, 1.0  (pressure_solver.cpp:204) */
+  int use_graph;        /* replay the step as a captured hipGraph, 1 */
+  int save_current;     /* copy vx/vy to *_current at step end, 1 */
+} ubg_params;
+
+typedef struct ubg_mg ubg_mg;
+typedef struct ubg_sim ubg_sim;
+
+/* ---- library ------------------------------------------------------------ */
+UBG_API int ubg_version(void);
+UBG_API const char *ubg_last_error(void);
+UBG_API void ubg_params_default(ubg_params *out);
+UBG_API int ubg_device_count(int *count);
+/* Level sizes of the MG hierarchy for a W x H grid: halve (integer) while
+ * both sizes exceed 3 (pressure_solver.hpp:16-31).  Returns the count. */
+UBG_API int ubg_mg_level_sizes(int width, int height, int *widths, int *heights, int cap);
+
+/* ---- MG: pressure_solver.hpp:13-76 -------------------------------------- */
+/* MG(w,h): hierarchy with all-fluid coarse flags (pressure_solver.hpp:16-31) */
+UBG_API int ubg_mg_create(int width, int height, ubg_mg **out);
+UBG_API int ubg_mg_destroy(ubg_mg *mg);
+UBG_API int ubg_mg_set_params(ubg_mg *mg, const ubg_params *prm);
+UBG_API int ubg_mg_num_levels(const ubg_mg *mg, int *levels);
+/* updateFields(flag) (pressure_solver.hpp:34-57): level-0 flag and the coarse
+ * classification of every level, on the device. */
+UBG_API int ubg_mg_update_fields(ubg_mg *mg, const float *flag);
+/* Level fields (p = ecs/p, f = rcs/f, r = residual scratch) as host copies. */
+UBG_API int ubg_mg_set_level_field(ubg_mg *mg, int level, int which, const float *src);
+UBG_API int ubg_mg_get_level_field(ubg_mg *mg, int level, int which, float *dst);
+UBG_API int ubg_mg_get_level_flag(ubg_mg *mg, int level, float *dst);
+/* One stage of the V-cycle on the resident level fields (stage-wise parity). */
+UBG_API int ubg_mg_level_op(ubg_mg *mg, int op, int level, float h, float alpha);
+/* ncycles x solve(p, f, flag, h, zeroGradientBC) on the resident level-0
+ * fields (pressure_solver.hpp:59-62, pressure_solver.cpp:201-248). */
+UBG_API int ubg_mg_solve(ubg_mg *mg, float h, int zero_gradient_bc, int ncycles);
+/* MG::solve(p, f, flag, h, zeroGradientBC) with host grids: uploads p, f and
+ * (if non-NULL) the level-0 flag, runs one V-cycle, downloads p.  Coarse
+ * flags come from the last update_fields, as in the reference. */
+UBG_API int ubg_mg_solve_host(ubg_mg *mg, float *p, const float *f, const float *flag,
+                              float h, int zero_gradient_bc);
+/* rbgs(p, f, flag, h, alpha) (pressure_solver.hpp:5-6), n sweeps, level 0. */
+UBG_API int ubg_mg_rbgs(ubg_mg *mg, float h, float alpha, int nsweeps);
+/* calculateResidualField (pressure_solver.hpp:9-10) on level 0: writes r
+ * (if dst != NULL) and the L2 norm accumulated in double. */
+UBG_API int ubg_mg_residual(ubg_mg *mg, float h, float *r_dst, double *norm);
+UBG_API int ubg_mg_sync(ubg_mg *mg);
+
+/* ---- Simulation: simulation.hpp:18-141 ---------------------------------- */
+/* Simulation(flag, pwidth, mu) (simulation.hpp:32-67): BCs S/N no-slip,
+ * W inflow, E zero-pressure outflow; vx(0,y)=1, vy(x,0)=0 in both buffers;
+ * MG(flag); h = pwidth/(W-1). */
+UBG_API int ubg_sim_create(const float *flag, int width, int height, float pwidth,
+                           float mu, ubg_sim **out);
+UBG_API int ubg_sim_destroy(ubg_sim *sim);
+UBG_API int ubg_sim_set_params(ubg_sim *sim, const ubg_params *prm);
+UBG_API int ubg_sim_get_params(const ubg_sim *sim, ubg_params *out);
+UBG_API int ubg_sim_dims(const ubg_sim *sim, int *width, int *height, float *h,
+                         float *pwidth, float *mu);
+UBG_API int ubg_sim_field_shape(const ubg_sim *sim, int field, int *width, int *height);
+/* Host copies of any field.  Setting UBG_FIELD_FLAG re-runs the coarse
+ * classification, as the game does with memcpy + mg.updateFields
+ * (ubootgl_app.cpp:111-112). */
+UBG_API int ubg_sim_set_field(ubg_sim *sim, int field, const float *src);
+UBG_API int ubg_sim_get_field(ubg_sim *sim, int field, float *dst);
+/* Device pointer of a field for zero-copy interop (valid until the next
+ * step; vx/vy front and back swap roles during a step). */
+UBG_API int ubg_sim_field_device_ptr(ubg_sim *sim, int field, void **dptr);
+/* step(timestep) (simulation.cpp:356-374) */
+UBG_API int ubg_sim_step(ubg_sim *sim, float dt);
+/* nsteps x step(dt) (+ tracer advection when tracers are attached) */
+UBG_API int ubg_sim_run(ubg_sim *sim, float dt, int nsteps);
+UBG_API int ubg_sim_stage(ubg_sim *sim, int stage, float dt);
+/* setGrids(c, val) (simulation.hpp:82-98) */
+UBG_API int ubg_sim_set_grids(ubg_sim *sim, int x, int y, float val);
+/* sinks (simulation.hpp:140, consumed by project simulation.cpp:173-187) */
+UBG_API int ubg_sim_add_sink(ubg_sim *sim, float x, float y, float z);
+UBG_API int ubg_sim_get_sinks(const ubg_sim *sim, float *xyz, int cap, int *count);
+/* the simulation's MG (Simulation::mg, simulation.hpp:133); borrowed */
+UBG_API int ubg_sim_get_mg(ubg_sim *sim, ubg_mg **mg);
+/* bilinearVel(c) (advect_floating_items.cpp:11-14) for n grid-space points
+ * c = (x0,y0,x1,y1,...); host arrays. */
+UBG_API int ubg_sim_bilinear_vel(ubg_sim *sim, const float *c, float *out, int n);
+/* Tracers: n points in physical coordinates (x,y) and ages, resident on the
+ * device.  Each ubg_sim_run step advects them with RK2 midpoint integration
+ * of bilinearVel, the freeze/age rule and Wang-hash respawn of
+ * advect_tracer_points.cs:42-82. */
+UBG_API int ubg_sim_set_tracers(ubg_sim *sim, const float *xy, const float *age, int n);
+UBG_API int ubg_sim_get_tracers(ubg_sim *sim, float *xy, float *age, int n);
+UBG_API int ubg_sim_advect_tracers(ubg_sim *sim, float dt, uint32_t seed);
+UBG_API int ubg_sim_sync(ubg_sim *sim);
+
+/* ---- measurement ---------------------------------------------------------- */
+/* Kernel timing with HIP events recorded on the handle's stream around every
+ * launch of one kernel class while enabled (ids: 0 = level-0 smoother sweep). */
+enum { UBG_TIMER_SMOOTH_L0 = 0, UBG_TIMER_COUNT };
+UBG_API int ubg_sim_timer_enable(ubg_sim *sim, int timer, int on);
+UBG_API int ubg_sim_timer_read(ubg_sim *sim, int timer, double *total_ms, long *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* UBG_C_H */

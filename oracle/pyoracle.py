@@ -1,0 +1,265 @@
+"""ctypes wrapper of the CPU oracle (oracle/ubg_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the ubootgl_amd package.  It restates
+the reference's CPU solver (simulation.cpp, pressure_solver.cpp,
+interpolators.hpp) so that the HIP path can be checked against it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libubg_oracle.so")
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+# field ids (same numbering as the product's UBG_FIELD_*)
+VX, VX_BACK, VY, VY_BACK, VX_ACCUM, VY_ACCUM, VX_CURRENT, VY_CURRENT, P, F, FLAG, R = range(12)
+FIELD_NAMES = ["vx", "vx_back", "vy", "vy_back", "vx_accum", "vy_accum",
+               "vx_current", "vy_current", "p", "f", "flag", "r"]
+# stage ids (same numbering as the product's UBG_STAGE_*)
+(ST_ACCUM, ST_DIFFUSE, ST_ADVECT, ST_SET_VBCS, ST_PROJECT, ST_SET_PBC,
+ ST_SAVE_CURRENT, ST_DIVERGENCE, ST_CORRECT_VELOCITY, ST_MG_SOLVE) = range(10)
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"oracle library not built: {LIB_PATH} (run make -C oracle)")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "oracle_set_threads": (C.c_int, [C.c_int]),
+        "oracle_mg_level_sizes": (C.c_int, [C.c_int, C.c_int, _i32p, _i32p, C.c_int]),
+        "oracle_coarsen_flag": (None, [_f32p, C.c_int, _f32p, C.c_int, C.c_int]),
+        "oracle_rbgs_f32": (None, [_f32p, _f32p, _f32p, C.c_int, C.c_int, C.c_float, C.c_float]),
+        "oracle_rbgs_f64": (None, [_f64p, _f64p, _f64p, C.c_int, C.c_int, C.c_double, C.c_double]),
+        "oracle_residual_f32": (C.c_double, [_f32p, _f32p, _f32p, _f32p, C.c_int, C.c_int, C.c_float]),
+        "oracle_residual_f64": (C.c_double, [_f64p, _f64p, _f64p, _f64p, C.c_int, C.c_int, C.c_double]),
+        "oracle_restrict_f32": (None, [_f32p, C.c_int, _f32p, C.c_int, C.c_int]),
+        "oracle_prolongate_f32": (None, [_f32p, C.c_int, C.c_int, _f32p, _f32p, C.c_int, _f32p]),
+        "oracle_correct_f32": (None, [_f32p, _f32p, C.c_int, C.c_int]),
+        "oracle_zero_gradient_f32": (None, [_f32p, C.c_int, C.c_int]),
+        "oracle_bicubic": (C.c_float, [_f32p, C.c_int, C.c_int, C.c_float, C.c_float]),
+        "oracle_bilinear": (C.c_float, [_f32p, C.c_int, C.c_int, C.c_float, C.c_float]),
+        "oracle_cubic_hermite": (C.c_float, [C.c_float] * 5),
+        "oracle_wang_hash": (C.c_uint32, [C.c_uint32]),
+        "oracle_lcg": (C.c_uint32, [C.c_uint32]),
+        "oracle_mg_create": (C.c_void_p, [C.c_int, C.c_int, C.c_int]),
+        "oracle_mg_destroy": (None, [C.c_void_p, C.c_int]),
+        "oracle_mg_set_params": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float]),
+        "oracle_mg_update_fields": (None, [C.c_void_p, _f32p]),
+        "oracle_mg64_update_fields": (None, [C.c_void_p, _f64p]),
+        "oracle_mg_get_flag": (None, [C.c_void_p, C.c_int, _f32p]),
+        "oracle_mg_solve": (None, [C.c_void_p, _f32p, _f32p, _f32p, C.c_float, C.c_int]),
+        "oracle_mg64_solve": (None, [C.c_void_p, _f64p, _f64p, _f64p, C.c_double, C.c_int]),
+        "oracle_sim_create": (C.c_void_p, [_f32p, C.c_int, C.c_int, C.c_float, C.c_float]),
+        "oracle_sim_destroy": (None, [C.c_void_p]),
+        "oracle_sim_set_params": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float]),
+        "oracle_sim_set_field": (C.c_int, [C.c_void_p, C.c_int, _f32p]),
+        "oracle_sim_get_field": (C.c_int, [C.c_void_p, C.c_int, _f32p]),
+        "oracle_sim_h": (C.c_float, [C.c_void_p]),
+        "oracle_sim_add_sink": (None, [C.c_void_p, C.c_float, C.c_float, C.c_float]),
+        "oracle_sim_num_sinks": (C.c_int, [C.c_void_p]),
+        "oracle_sim_step": (None, [C.c_void_p, C.c_float]),
+        "oracle_sim_stage": (C.c_int, [C.c_void_p, C.c_int, C.c_float]),
+        "oracle_sim_bilinear_vel": (None, [C.c_void_p, _f32p, _f32p, C.c_int]),
+        "oracle_flag_linear": (C.c_float, [_f32p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float]),
+        "oracle_sim_advect_tracers": (None, [C.c_void_p, _f32p, _f32p, C.c_int, C.c_float, C.c_uint32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def set_threads(n=0):
+    return lib().oracle_set_threads(int(n))
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def level_sizes(W, H):
+    ws = np.zeros(64, np.int32)
+    hs = np.zeros(64, np.int32)
+    n = lib().oracle_mg_level_sizes(W, H, ws, hs, 64)
+    return [(int(ws[i]), int(hs[i])) for i in range(n)]
+
+
+def rbgs(p, f, flag, h, alpha=1.0):
+    """pressure_solver.cpp:49-89 (two-pass semantics); p updated in place."""
+    H, W = p.shape
+    if p.dtype == np.float64:
+        lib().oracle_rbgs_f64(p, f64(f), f64(flag), W, H, h, alpha)
+    else:
+        lib().oracle_rbgs_f32(p, f32(f), f32(flag), W, H, h, alpha)
+    return p
+
+
+def residual(p, f, flag, h):
+    """pressure_solver.cpp:91-116; returns (r, ||r||_2)."""
+    H, W = p.shape
+    if p.dtype == np.float64:
+        r = np.zeros_like(p)
+        n = lib().oracle_residual_f64(p, f64(f), f64(flag), r, W, H, h)
+    else:
+        r = np.zeros((H, W), np.float32)
+        n = lib().oracle_residual_f32(f32(p), f32(f), f32(flag), r, W, H, h)
+    return r, n
+
+
+def restrict(r, Wc, Hc):
+    H, W = r.shape
+    rc = np.zeros((Hc, Wc), np.float32)
+    lib().oracle_restrict_f32(f32(r), W, rc, Wc, Hc)
+    return rc
+
+
+def prolongate(W, H, ec, flagc, flag):
+    Hc, Wc = ec.shape
+    e = np.zeros((H, W), np.float32)
+    lib().oracle_prolongate_f32(e, W, H, f32(ec), f32(flagc), Wc, f32(flag))
+    return e
+
+
+def correct(p, e):
+    H, W = p.shape
+    lib().oracle_correct_f32(p, f32(e), W, H)
+    return p
+
+
+def zero_gradient(p):
+    H, W = p.shape
+    lib().oracle_zero_gradient_f32(p, W, H)
+    return p
+
+
+def coarsen_flag(flag, Wc, Hc):
+    H, W = flag.shape
+    fc = np.zeros((Hc, Wc), np.float32)
+    lib().oracle_coarsen_flag(f32(flag), W, fc, Wc, Hc)
+    return fc
+
+
+def bicubic(g, cx, cy):
+    gh, gw = g.shape
+    return lib().oracle_bicubic(f32(g), gw, gh, cx, cy)
+
+
+def bilinear(g, cx, cy):
+    gh, gw = g.shape
+    return lib().oracle_bilinear(f32(g), gw, gh, cx, cy)
+
+
+class OracleMG:
+    """pressure_solver.hpp:13-76 (MG), fp32 or fp64 storage."""
+
+    def __init__(self, W, H, fp64=False, nu_pre=3, nu_post=3, nu_coarse=5, alpha=1.0):
+        self.W, self.H, self.fp64 = W, H, bool(fp64)
+        self.h_ = lib().oracle_mg_create(W, H, int(self.fp64))
+        lib().oracle_mg_set_params(self.h_, int(self.fp64), nu_pre, nu_post, nu_coarse, alpha)
+        self.sizes = level_sizes(W, H)
+
+    def update_fields(self, flag):
+        if self.fp64:
+            lib().oracle_mg64_update_fields(self.h_, f64(flag))
+        else:
+            lib().oracle_mg_update_fields(self.h_, f32(flag))
+
+    def flag_level(self, level):
+        assert not self.fp64
+        W, H = self.sizes[level]
+        out = np.zeros((H, W), np.float32)
+        lib().oracle_mg_get_flag(self.h_, level, out)
+        return out
+
+    def solve(self, p, f, flag, h, zero_gradient_bc=False):
+        if self.fp64:
+            lib().oracle_mg64_solve(self.h_, p, f64(f), f64(flag), h, int(zero_gradient_bc))
+        else:
+            lib().oracle_mg_solve(self.h_, p, f32(f), f32(flag), h, int(zero_gradient_bc))
+        return p
+
+    def __del__(self):
+        try:
+            lib().oracle_mg_destroy(self.h_, int(self.fp64))
+        except Exception:
+            pass
+
+
+class OracleSim:
+    """simulation.hpp:18-141 (Simulation) with the reference constructor."""
+
+    def __init__(self, flag, pwidth, mu, nu_pre=3, nu_post=3, nu_coarse=5, vcycles=2, alpha=1.0):
+        flag = f32(flag)
+        self.H, self.W = flag.shape
+        self.h_ = lib().oracle_sim_create(flag, self.W, self.H, pwidth, mu)
+        lib().oracle_sim_set_params(self.h_, nu_pre, nu_post, nu_coarse, vcycles, alpha)
+        self.pwidth, self.mu = pwidth, mu
+
+    @property
+    def h(self):
+        return lib().oracle_sim_h(self.h_)
+
+    def shape(self, fid):
+        W, H = self.W, self.H
+        if fid in (VX, VX_BACK, VX_ACCUM, VX_CURRENT):
+            return (H, W - 1)
+        if fid in (VY, VY_BACK, VY_ACCUM, VY_CURRENT):
+            return (H - 1, W)
+        return (H, W)
+
+    def get(self, fid):
+        out = np.zeros(self.shape(fid), np.float32)
+        assert lib().oracle_sim_get_field(self.h_, fid, out) == 0
+        return out
+
+    def set(self, fid, arr):
+        arr = f32(arr)
+        assert arr.shape == self.shape(fid)
+        assert lib().oracle_sim_set_field(self.h_, fid, arr) == 0
+
+    def step(self, dt):
+        lib().oracle_sim_step(self.h_, dt)
+
+    def stage(self, st, dt):
+        assert lib().oracle_sim_stage(self.h_, st, dt) == 0
+
+    def add_sink(self, x, y, z):
+        lib().oracle_sim_add_sink(self.h_, x, y, z)
+
+    def num_sinks(self):
+        return lib().oracle_sim_num_sinks(self.h_)
+
+    def bilinear_vel(self, c):
+        c = f32(c).reshape(-1, 2)
+        out = np.zeros_like(c)
+        lib().oracle_sim_bilinear_vel(self.h_, c, out, c.shape[0])
+        return out
+
+    def advect_tracers(self, xy, age, dt, seed):
+        """xy (n,2) float32 and age (n,) float32 are updated in place."""
+        assert xy.dtype == np.float32 and xy.flags.c_contiguous
+        assert age.dtype == np.float32 and age.flags.c_contiguous
+        lib().oracle_sim_advect_tracers(self.h_, xy.reshape(-1), age, age.shape[0], dt, seed)
+
+    def __del__(self):
+        try:
+            lib().oracle_sim_destroy(self.h_)
+        except Exception:
+            pass

@@ -1,0 +1,923 @@
+// ubg_oracle.cpp -- CPU restatement of ubootgl's fluid step.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the oracle: a clean-room, plain
+// C++17/OpenMP restatement of the reference algorithm (te42kyfo/ubootgl), used
+// to check the HIP implementation and, in bench.py, timed as the host CPU
+// baseline.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// leg may load it.  The product library (ubootgl_amd/libubg.so) never links or
+// calls it.
+//
+// Every function cites the reference file:line it follows.  Loop bounds,
+// evaluation order and the boundary-condition order are kept; the rbgs
+// smoother always uses the reference's two-pass red/black semantics
+// (pressure_solver.cpp:65-71), never its thread-count dependent wavefront
+// branch (pressure_solver.cpp:72-87).
+//
+// Pinning (tests/test_oracle.py):
+//   * MG::updateFields            closed-form integer rule, brute force
+//   * rbgs / smoothingKernel      closed form per point, fixed point of an exact
+//                                 discrete solution, colour-order independence
+//   * residual/restrict/prolong   closed forms, linearity, constant fields
+//   * MG V-cycle                  mgtest harmonic KAT (mgtest/mgtest.cpp:10-53),
+//                                 fp64 direct solve on tiny grids
+//   * bicubicSample               Catmull-Rom reproduces quadratics, node
+//                                 interpolation, clamp rule
+//   * advect                      zero velocity = identity, uniform shift
+//   * diffuse                     mu=0 => masked identity, closed form
+//   * project                     discrete identity div(v_new) = div(v) - L p
+//   * tracers                     bilinear exact on linear fields; Wang hash and
+//                                 the Numerical Recipes LCG as integer closed forms
+// Parity unpinned: the composition of advect's quirks under the reference's
+// own -Ofast build (the reference cannot be compiled here: it needs glm and
+// GLEW headers the image lacks), and glm::mix's exact rounding (x*(1-a)+y*a is
+// assumed) inside psampleFlagLinear.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstddef>
+#include <cstring>
+#include <vector>
+#include <omp.h>
+
+#define ORACLE_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Multigrid primitives, templated over the storage type (float = reference,
+// double = fp64 oracle for the mixed-precision extension).
+// ---------------------------------------------------------------------------
+
+// pressure_solver.cpp:10-24 (smoothingKernel)
+template <typename T>
+inline T smooth_point(const T *p, const T *f, const T *fl, int W, int x, int y,
+                      T h, T alpha) {
+  const size_t i = (size_t)y * W + x;
+  T sum = T(0);
+  sum += fl[i - 1] + fl[i + 1] + fl[i + W] + fl[i - W];
+  T val = T(0);
+  val += p[i - 1] * fl[i - 1];
+  val += p[i + 1] * fl[i + 1];
+  val += p[i - W] * fl[i - W];
+  val += p[i + W] * fl[i + W];
+  val += f[i] * h * h;
+  val = (sum == T(0)) ? T(0) : val / sum;
+  return fl[i] * (alpha * val + (T(1) - alpha) * p[i]);
+}
+
+// pressure_solver.cpp:35-47 (red = (x+y) odd) and :65-71 (two passes)
+template <typename T>
+void rbgs(T *p, const T *f, const T *fl, int W, int H, T h, T alpha) {
+  const bool par = H > 64;
+#pragma omp parallel for schedule(static) if (par)
+  for (int y = 1; y < H - 1; y++)
+    for (int x = 1 + (y % 2); x < W - 1; x += 2)
+      p[(size_t)y * W + x] = smooth_point(p, f, fl, W, x, y, h, alpha);
+#pragma omp parallel for schedule(static) if (par)
+  for (int y = 1; y < H - 1; y++)
+    for (int x = 1 + ((y + 1) % 2); x < W - 1; x += 2)
+      p[(size_t)y * W + x] = smooth_point(p, f, fl, W, x, y, h, alpha);
+}
+
+// pressure_solver.cpp:91-116 (calculateResidualField); the norm is summed in
+// double (the reference's float OpenMP reduction has no fixed order).
+template <typename T>
+double residual(const T *p, const T *f, const T *fl, T *r, int W, int H, T h) {
+  const T ihsq = T(1) / h / h;
+  double l2 = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : l2) if (H > 64)
+  for (int y = 1; y < H - 1; y++) {
+    for (int x = 1; x < W - 1; x++) {
+      const size_t i = (size_t)y * W + x;
+      const T pc = p[i];
+      T val = T(0);
+      val += p[i - 1] * fl[i - 1] + pc * (T(1) - fl[i - 1]);
+      val += p[i + 1] * fl[i + 1] + pc * (T(1) - fl[i + 1]);
+      val += p[i - W] * fl[i - W] + pc * (T(1) - fl[i - W]);
+      val += p[i + W] * fl[i + W] + pc * (T(1) - fl[i + W]);
+      val += T(-4) * pc;
+      val *= ihsq;
+      r[i] = (f[i] + val) * fl[i];
+      l2 += (double)r[i] * (double)r[i];
+    }
+  }
+  return std::sqrt(l2);
+}
+
+// pressure_solver.cpp:118-132 (restrict: full weighting at (2x,2y))
+template <typename T>
+void restrict_fw(const T *r, int W, T *rc, int Wc, int Hc) {
+#pragma omp parallel for schedule(static) if (Hc > 64)
+  for (int y = 1; y < Hc - 1; y++) {
+    for (int x = 1; x < Wc - 1; x++) {
+      auto R = [&](int xx, int yy) { return r[(size_t)yy * W + xx]; };
+      T v = T(0);
+      v = R(2 * x - 1, 2 * y - 1) * T(1) + R(2 * x, 2 * y - 1) * T(2) +
+          R(2 * x + 1, 2 * y - 1) * T(1);
+      v += R(2 * x - 1, 2 * y) * T(2) + R(2 * x, 2 * y) * T(4) +
+           R(2 * x + 1, 2 * y) * T(2);
+      v += R(2 * x - 1, 2 * y + 1) * T(1) + R(2 * x, 2 * y + 1) * T(2) +
+           R(2 * x + 1, 2 * y + 1) * T(1);
+      rc[(size_t)y * Wc + x] = v * (T(1) / T(16));
+    }
+  }
+}
+
+// The prolongation denominator adds the double literal 0.0001 to a sum of
+// flags and stores the result in the grid's type (pressure_solver.cpp:149).
+template <typename T> inline T eps_sum(T s) { return (T)((double)s + 0.0001); }
+
+// pressure_solver.cpp:134-172 (prolongate: e = 0, then four parity classes)
+template <typename T>
+void prolongate(T *e, int W, int H, const T *ec, const T *flc, int Wc,
+                const T *fl) {
+  std::fill(e, e + (size_t)W * H, T(0));
+  auto EC = [&](int x, int y) { return ec[(size_t)y * Wc + x]; };
+  auto FC = [&](int x, int y) { return flc[(size_t)y * Wc + x]; };
+  const bool par = H > 64;
+#pragma omp parallel for schedule(static) if (par)
+  for (int y = 2; y < H - 1; y += 2)
+    for (int x = 2; x < W - 1; x += 2)
+      e[(size_t)y * W + x] = EC(x / 2, y / 2) * fl[(size_t)y * W + x];
+#pragma omp parallel for schedule(static) if (par)
+  for (int y = 2; y < H - 1; y += 2)
+    for (int x = 1; x < W - 2; x += 2) {
+      T sum = eps_sum<T>(FC(x / 2, y / 2) + FC(x / 2 + 1, y / 2));
+      e[(size_t)y * W + x] =
+          fl[(size_t)y * W + x] * (EC(x / 2, y / 2) + EC(x / 2 + 1, y / 2)) /
+          sum;
+    }
+#pragma omp parallel for schedule(static) if (par)
+  for (int y = 1; y < H - 2; y += 2)
+    for (int x = 2; x < W - 1; x += 2) {
+      T sum = eps_sum<T>(FC(x / 2, y / 2) + FC(x / 2, y / 2 + 1));
+      e[(size_t)y * W + x] =
+          fl[(size_t)y * W + x] * (EC(x / 2, y / 2) + EC(x / 2, y / 2 + 1)) /
+          sum;
+    }
+#pragma omp parallel for schedule(static) if (par)
+  for (int y = 1; y < H - 2; y += 2)
+    for (int x = 1; x < W - 2; x += 2) {
+      T sum = eps_sum<T>(FC(x / 2, y / 2) + FC(x / 2 + 1, y / 2 + 1) +
+                         FC(x / 2 + 1, y / 2) + FC(x / 2, y / 2 + 1));
+      e[(size_t)y * W + x] =
+          fl[(size_t)y * W + x] *
+          (EC(x / 2, y / 2) + EC(x / 2 + 1, y / 2 + 1) + EC(x / 2 + 1, y / 2) +
+           EC(x / 2, y / 2 + 1)) /
+          sum;
+    }
+}
+
+// pressure_solver.cpp:174-181 (correct)
+template <typename T> void correct(T *p, const T *e, int W, int H) {
+#pragma omp parallel for schedule(static) if (H > 64)
+  for (int y = 1; y < H - 1; y++)
+    for (int x = 1; x < W - 1; x++) p[(size_t)y * W + x] += e[(size_t)y * W + x];
+}
+
+// pressure_solver.cpp:183-192 (setZeroGradientBC; corners untouched)
+template <typename T> void zero_gradient(T *p, int W, int H) {
+  for (int y = 1; y < H - 1; y++) {
+    p[(size_t)y * W] = p[(size_t)y * W + 1];
+    p[(size_t)y * W + W - 1] = p[(size_t)y * W + W - 2];
+  }
+  for (int x = 1; x < W - 1; x++) {
+    p[x] = p[(size_t)W + x];
+    p[(size_t)(H - 1) * W + x] = p[(size_t)(H - 2) * W + x];
+  }
+}
+
+// pressure_solver.hpp:16-31: level sizes halve (integer) while both > 3.
+void level_sizes(int W, int H, std::vector<int> &ws, std::vector<int> &hs) {
+  ws.clear();
+  hs.clear();
+  int cw = W, ch = H;
+  while (cw > 3 && ch > 3) {
+    ws.push_back(cw);
+    hs.push_back(ch);
+    cw /= 2;
+    ch /= 2;
+  }
+}
+
+// pressure_solver.hpp:34-57 (updateFields): coarse flag is 1 iff the
+// full-weighted fine-flag sum / 16 exceeds 0.2; the coarse boundary is 1.
+template <typename T>
+void coarsen_flag(const T *f, int W, T *fc, int Wc, int Hc) {
+  std::fill(fc, fc + (size_t)Wc * Hc, T(1));
+  for (int y = 1; y < Hc - 1; y++) {
+    for (int x = 1; x < Wc - 1; x++) {
+      auto F = [&](int xx, int yy) { return (float)f[(size_t)yy * W + xx]; };
+      float v = 0.0f;
+      v = F(2 * x - 1, 2 * y - 1) * 1 + F(2 * x, 2 * y - 1) * 2 +
+          F(2 * x + 1, 2 * y - 1) * 1;
+      v += F(2 * x - 1, 2 * y) * 2 + F(2 * x, 2 * y) * 4 +
+           F(2 * x + 1, 2 * y) * 2;
+      v += F(2 * x - 1, 2 * y + 1) * 1 + F(2 * x, 2 * y + 1) * 2 +
+           F(2 * x + 1, 2 * y + 1) * 1;
+      v *= 1.0f / 16.0f;
+      fc[(size_t)y * Wc + x] = (v > 0.2) ? T(1) : T(0);
+    }
+  }
+}
+
+struct MGParams {
+  int nu_pre = 3;    // pressure_solver.cpp:211
+  int nu_post = 3;   // pressure_solver.cpp:243
+  int nu_coarse = 5; // pressure_solver.cpp:204
+  float alpha = 1.0f;
+};
+
+// pressure_solver.hpp:13-76 + pressure_solver.cpp:201-248
+template <typename T> struct MG {
+  int W, H, levels;
+  std::vector<int> ws, hs;
+  std::vector<std::vector<T>> rs, rcs, ecs, es, flagcs;
+  MGParams prm;
+
+  MG(int W_, int H_) : W(W_), H(H_) {
+    level_sizes(W, H, ws, hs);
+    levels = (int)ws.size();
+    for (int l = 0; l < levels; l++) {
+      size_t n = (size_t)ws[l] * hs[l];
+      rs.emplace_back(n, T(0));
+      rcs.emplace_back(n, T(0));
+      ecs.emplace_back(n, T(0));
+      es.emplace_back(n, T(0));
+      flagcs.emplace_back(n, T(1));
+    }
+  }
+
+  void update_fields(const T *flag) {
+    std::copy(flag, flag + (size_t)W * H, flagcs[0].begin());
+    for (int l = 1; l < levels; l++)
+      coarsen_flag(flagcs[l - 1].data(), ws[l - 1], flagcs[l].data(), ws[l],
+                   hs[l]);
+  }
+
+  void solve_level(T *p, const T *f, const T *flag, T h, int level, bool zg) {
+    const int Wl = ws[level], Hl = hs[level];
+    const T alpha = (T)prm.alpha;
+    if (level == levels - 2) {
+      for (int i = 0; i < prm.nu_coarse; i++) rbgs(p, f, flag, Wl, Hl, h, alpha);
+      return;
+    }
+    for (int i = 0; i < prm.nu_pre; i++) {
+      rbgs(p, f, flag, Wl, Hl, h, alpha);
+      if (level == 0 && zg) zero_gradient(p, Wl, Hl);
+    }
+    T *r = rs[level].data();
+    std::fill(rs[level].begin(), rs[level].end(), T(0));
+    residual(p, f, flag, r, Wl, Hl, h);
+    const int Wc = ws[level + 1], Hc = hs[level + 1];
+    T *rc = rcs[level + 1].data();
+    std::fill(rcs[level + 1].begin(), rcs[level + 1].end(), T(0));
+    restrict_fw(r, Wl, rc, Wc, Hc);
+    T *ec = ecs[level + 1].data();
+    std::fill(ecs[level + 1].begin(), ecs[level + 1].end(), T(0));
+    const T *flagc = flagcs[level + 1].data();
+    // pressure_solver.cpp:229: h_c = h * (W_l - 1.0f) / (W_{l+1} - 1.0f)
+    T hc = h * (T)(Wl - 1.0f) / (T)(Wc - 1.0f);
+    solve_level(ec, rc, flagc, hc, level + 1, false);
+    T *e = es[level].data();
+    prolongate(e, Wl, Hl, ec, flagc, Wc, flag);
+    correct(p, e, Wl, Hl);
+    if (level == 0 && zg) zero_gradient(p, Wl, Hl);
+    for (int i = 0; i < prm.nu_post; i++) {
+      rbgs(p, f, flag, Wl, Hl, h, alpha);
+      if (level == 0 && zg) zero_gradient(p, Wl, Hl);
+    }
+  }
+
+  void solve(T *p, const T *f, const T *flag, T h, bool zg) {
+    solve_level(p, f, flag, h, 0, zg);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Interpolators (interpolators.hpp)
+// ---------------------------------------------------------------------------
+
+// interpolators.hpp:78-85 (CubicHermite, Catmull-Rom)
+inline float cubic_hermite(float t, float A, float B, float C, float D) {
+  float a = -A / 2.0f + (3.0f * B) / 2.0f - (3.0f * C) / 2.0f + D / 2.0f;
+  float b = A - (5.0f * B) / 2.0f + 2.0f * C - D / 2.0f;
+  float c = -A / 2.0f + C / 2.0f;
+  float d = B;
+  return a * t * t * t + b * t * t + c * t + d;
+}
+
+// interpolators.hpp:92-206 (bicubicSample, one lane): clamp to [3, n-3],
+// truncate, four vertical Catmull-Rom passes, then one horizontal pass.
+inline float bicubic(const float *g, int gw, int gh, float cx, float cy) {
+  cx = std::max(std::min(cx, gw - 3.0f), 3.0f);
+  cy = std::max(std::min(cy, gh - 3.0f), 3.0f);
+  int icx = (int)cx, icy = (int)cy;
+  float stx = cx - std::trunc(cx);
+  float sty = cy - std::trunc(cy);
+  const float *base = g + (std::ptrdiff_t)(icy - 1) * gw + (icx - 1);
+  float col[4];
+  for (int m = 0; m < 4; m++)
+    col[m] = cubic_hermite(sty, base[m], base[gw + m], base[2 * gw + m],
+                           base[3 * gw + m]);
+  return cubic_hermite(stx, col[0], col[1], col[2], col[3]);
+}
+
+// interpolators.hpp:11-27 (scalar bilinearSample with clamp to [0, n-1.1])
+inline float bilinear(const float *g, int gw, int gh, float cx, float cy) {
+  cx = std::min(std::max(cx, 0.0f), gw - 1.1f);
+  cy = std::min(std::max(cy, 0.0f), gh - 1.1f);
+  int ix = (int)cx, iy = (int)cy;
+  float sx = cx - std::floor(cx), sy = cy - std::floor(cy);
+  float v1 = g[(size_t)iy * gw + ix];
+  float v2 = g[(size_t)iy * gw + ix + 1];
+  float v3 = g[(size_t)(iy + 1) * gw + ix];
+  float v4 = g[(size_t)(iy + 1) * gw + ix + 1];
+  float vm1 = v1 + (v2 - v1) * sx;
+  float vm2 = v3 + (v4 - v3) * sx;
+  return vm1 + (vm2 - vm1) * sy;
+}
+
+// ---------------------------------------------------------------------------
+// Simulation (simulation.hpp / simulation.cpp)
+// ---------------------------------------------------------------------------
+
+enum BC { INFLOW = 0, OUTFLOW = 1, OUTFLOW_ZERO_PRESSURE = 2, NOSLIP = 3 };
+
+struct Sink {
+  float x, y, z;
+};
+
+struct Sim {
+  int W, H, Wx, Hy; // cells; vx is Wx x H, vy is W x Hy
+  float pwidth, mu, h, dt = 0.0f;
+  int bcW = INFLOW, bcE = OUTFLOW_ZERO_PRESSURE, bcN = NOSLIP, bcS = NOSLIP;
+  std::vector<float> vx[2], vy[2];
+  int fx = 0, fy = 0; // front index of vx and of vy (db2dgrid.hpp:52-64)
+  std::vector<float> vx_accum, vy_accum, vx_cur, vy_cur, p, f, flag, r;
+  MG<float> mg;
+  std::vector<Sink> sinks;
+  int vcycles = 2; // simulation.cpp:189-190
+
+  float *vxf() { return vx[fx].data(); }
+  float *vxb() { return vx[1 - fx].data(); }
+  float *vyf() { return vy[fy].data(); }
+  float *vyb() { return vy[1 - fy].data(); }
+
+  // simulation.hpp:32-67
+  Sim(const float *flagIn, int W_, int H_, float pw, float mu_)
+      : W(W_), H(H_), Wx(W_ - 1), Hy(H_ - 1), pwidth(pw), mu(mu_),
+        mg(W_, H_) {
+    for (int b = 0; b < 2; b++) {
+      vx[b].assign((size_t)Wx * H, 0.0f);
+      vy[b].assign((size_t)W * Hy, 0.0f);
+    }
+    vx_accum.assign((size_t)Wx * H, 0.0f);
+    vy_accum.assign((size_t)W * Hy, 0.0f);
+    vx_cur.assign((size_t)Wx * H, 0.0f);
+    vy_cur.assign((size_t)W * Hy, 0.0f);
+    p.assign((size_t)W * H, 0.0f);
+    f.assign((size_t)W * H, 0.0f);
+    r.assign((size_t)W * H, 0.0f);
+    flag.assign(flagIn, flagIn + (size_t)W * H);
+    for (int x = 0; x < W; x++) vy[0][x] = vy[1][x] = 0.0f;
+    for (int y = 0; y < H; y++) vx[0][(size_t)y * Wx] = vx[1][(size_t)y * Wx] = 1.0f;
+    mg.update_fields(flag.data());
+    h = pwidth / (W - 1.0f);
+  }
+
+  // simulation.cpp:23-34
+  float singlePBC(int bc, float a) { return bc == OUTFLOW_ZERO_PRESSURE ? -a : a; }
+  // simulation.cpp:36-45
+  void setPBC() {
+    float *P = p.data();
+    for (int y = 0; y < H; y++) {
+      P[(size_t)y * W] = singlePBC(bcW, P[(size_t)y * W + 1]);
+      P[(size_t)y * W + W - 1] = singlePBC(bcE, P[(size_t)y * W + W - 2]);
+    }
+    for (int x = 0; x < W; x++) {
+      P[x] = singlePBC(bcS, P[(size_t)W + x]);
+      P[(size_t)(H - 1) * W + x] = singlePBC(bcN, P[(size_t)(H - 2) * W + x]);
+    }
+  }
+  // simulation.cpp:50-78
+  static float VBCPar(int bc, float a, float b) {
+    if (bc == INFLOW) return b;
+    if (bc == OUTFLOW || bc == OUTFLOW_ZERO_PRESSURE) return std::fmax(a, 0.0f);
+    return 0.0f;
+  }
+  static float VBCPer(int bc, float a, float b) {
+    if (bc == INFLOW) return b;
+    if (bc == OUTFLOW || bc == OUTFLOW_ZERO_PRESSURE) return std::fmax(a, 0.0f);
+    return -a;
+  }
+  // simulation.cpp:80-102: sequential order defines the corners.
+  void setVBCs() {
+    float *xf = vxf(), *xb = vxb(), *yf = vyf(), *yb = vyb();
+    for (int y = 0; y < H; y++) {
+      size_t r0 = (size_t)y * Wx;
+      xf[r0] = xb[r0] = VBCPar(bcW, xf[r0 + 1], xf[r0]);
+      xf[r0 + Wx - 1] = xb[r0 + Wx - 1] = VBCPar(bcE, xf[r0 + Wx - 2], xf[r0 + Wx - 1]);
+    }
+    for (int x = 0; x < Wx; x++) {
+      size_t top = (size_t)(H - 1) * Wx + x, top1 = (size_t)(H - 2) * Wx + x;
+      xf[x] = xb[x] = VBCPer(bcS, xf[(size_t)Wx + x], xf[x]);
+      xf[top] = xb[top] = VBCPer(bcN, xf[top1], xf[top]);
+    }
+    for (int y = 0; y < Hy; y++) {
+      size_t r0 = (size_t)y * W;
+      yf[r0] = yb[r0] = VBCPer(bcW, yf[r0 + 1], yf[r0]);
+      yf[r0 + W - 1] = yb[r0 + W - 1] = VBCPer(bcE, yf[r0 + W - 2], yf[r0 + W - 1]);
+    }
+    for (int x = 0; x < W; x++) {
+      size_t top = (size_t)(Hy - 1) * W + x, top1 = (size_t)(Hy - 2) * W + x;
+      yf[x] = yb[x] = VBCPar(bcS, yf[(size_t)W + x], yf[x]);
+      yf[top] = yb[top] = VBCPar(bcN, yf[top1], yf[top]);
+    }
+  }
+
+  // simulation.cpp:376-396
+  void applyAccumulatedVelocity() {
+    float *xf = vxf(), *yf = vyf();
+#pragma omp parallel for schedule(static)
+    for (int y = 1; y < H - 1; y++)
+      for (int x = 1; x < Wx - 1; x++) {
+        size_t i = (size_t)y * Wx + x;
+        xf[i] += vx_accum[i];
+        vx_accum[i] = 0.0f;
+      }
+#pragma omp parallel for schedule(static)
+    for (int y = 1; y < Hy - 1; y++)
+      for (int x = 1; x < W - 1; x++) {
+        size_t i = (size_t)y * W + x;
+        yf[i] += vy_accum[i];
+        vy_accum[i] = 0.0f;
+      }
+  }
+
+  // simulation.cpp:104-162 (two Jacobi iterations for vx, then two for vy;
+  // each followed by swap and setVBCs)
+  void diffuse() {
+    const float a = dt * mu * (W - 1.0f) / pwidth;
+    const float *F = flag.data();
+    auto FL = [&](int x, int y) { return F[(size_t)y * W + x]; };
+    for (int it = 1; it < 3; it++) {
+      const float *s = vxf();
+      float *d = vxb();
+#pragma omp parallel for schedule(static)
+      for (int y = 1; y < H - 1; y++) {
+        for (int x = 1; x < Wx - 1; x++) {
+          auto V = [&](int xx, int yy) { return s[(size_t)yy * Wx + xx]; };
+          float val = 0;
+          val += V(x + 1, y) * FL(x + 1, y) * FL(x + 2, y);
+          val += V(x - 1, y) * FL(x, y) * FL(x - 1, y);
+          float fvn = FL(x, y + 1) * FL(x - 1, y + 1);
+          val += V(x, y + 1) * fvn + (1.0f - fvn) * -V(x, y);
+          float fvs = FL(x, y - 1) * FL(x - 1, y - 1);
+          val += V(x, y - 1) * fvs + (1.0f - fvs) * -V(x, y);
+          d[(size_t)y * Wx + x] =
+              FL(x, y) * FL(x + 1, y) * (V(x, y) + a * val) / (1.0f + 4.0f * a);
+        }
+      }
+      fx = 1 - fx; // vx.swap()
+      setVBCs();
+    }
+    for (int it = 1; it < 3; it++) {
+      const float *s = vyf();
+      float *d = vyb();
+#pragma omp parallel for schedule(static)
+      for (int y = 1; y < Hy - 1; y++) {
+        for (int x = 1; x < W - 1; x++) {
+          auto V = [&](int xx, int yy) { return s[(size_t)yy * W + xx]; };
+          float val = 0;
+          val += V(x, y - 1) * FL(x, y) * FL(x, y - 1);
+          val += V(x, y + 1) * FL(x, y + 1) * FL(x, y + 2);
+          float fve = FL(x + 1, y) * FL(x + 1, y + 1);
+          val += V(x + 1, y) * fve + (1.0f - fve) * -V(x, y);
+          float fvw = FL(x - 1, y) * FL(x - 1, y + 1);
+          val += V(x - 1, y) * fvw + (1.0f - fvw) * -V(x, y);
+          d[(size_t)y * W + x] =
+              FL(x, y) * FL(x, y + 1) * (V(x, y) + a * val) / (1.0f + 4.0f * a);
+        }
+      }
+      fy = 1 - fy; // vy.swap()
+      setVBCs();
+    }
+  }
+
+  // simulation.cpp:241-354: RK2 backtrace over 8-lane chunks.  A chunk is
+  // skipped unless some lane has flag(x-1)+flag(x)==2 (vx) or
+  // flag(x,y)+flag(x,y-1)==2 (vy); chunks start at x=1 and stop 8 lanes short
+  // of the grid width, so the tail columns keep the back buffer.  All loads use
+  // flat row-major offsets exactly as the 8-wide vector loads do.
+  void advect() {
+    const float ih = 1.0f / h;
+    const float hdt = 0.5f * dt * ih, fdt = dt * ih;
+    const float *F = flag.data();
+    const float *xf = vxf(), *yf = vyf();
+    float *xb = vxb(), *yb = vyb();
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int y = 1; y < H - 1; y++) {
+      for (int x0 = 1; x0 < Wx - 8; x0 += 8) {
+        bool any = false;
+        for (int k = 0; k < 8; k++) {
+          size_t fi = (size_t)y * W + x0 + k;
+          if (F[fi - 1] + F[fi] == 2.0f) any = true;
+        }
+        if (!any) continue;
+        for (int k = 0; k < 8; k++) {
+          const int x = x0 + k;
+          float posx = (float)x + 0.5f, posy = (float)y;
+          float vx1 = xf[(size_t)x + (size_t)y * Wx];
+          float vy1 = ((yf[x + (size_t)y * W] + yf[x + (size_t)(y - 1) * W]) +
+                       (yf[x + 1 + (size_t)y * W] + yf[x + 1 + (size_t)(y - 1) * W])) *
+                      0.25f;
+          float midx = posx - vx1 * hdt, midy = posy - vy1 * hdt;
+          float vx2 = bicubic(xf, Wx, H, midx - 0.5f, midy);
+          float vy2 = bicubic(yf, W, Hy, midx, midy - 0.5f);
+          float endx = posx - vx2 * fdt, endy = posy - vy2 * fdt;
+          float v = bicubic(xf, Wx, H, endx - 0.5f, endy);
+          v = v * F[(size_t)y * W + x] * F[(size_t)y * W + x + 1];
+          xb[(size_t)Wx * y + x] = v;
+        }
+      }
+      for (int x0 = 1; x0 < W - 8; x0 += 8) {
+        bool any = false;
+        for (int k = 0; k < 8; k++) {
+          size_t fi = (size_t)y * W + x0 + k;
+          if (F[fi] + F[fi - W] == 2.0f) any = true;
+        }
+        if (!any) continue;
+        for (int k = 0; k < 8; k++) {
+          const int x = x0 + k;
+          float posy = (float)y + 0.5f, posx = (float)x;
+          float vy1 = yf[(size_t)x + (size_t)y * W];
+          float vx1 = ((xf[x + (size_t)y * Wx] + xf[x + (size_t)(y - 1) * Wx]) +
+                       (xf[x + 1 + (size_t)y * Wx] + xf[x + 1 + (size_t)(y - 1) * Wx])) *
+                      0.25f;
+          float midx = posx - vx1 * hdt, midy = posy - vy1 * hdt;
+          float vx2 = bicubic(xf, Wx, H, midx - 0.5f, midy);
+          float vy2 = bicubic(yf, W, Hy, midx, midy - 0.5f);
+          float endx = posx - vx2 * fdt, endy = posy - vy2 * fdt;
+          float v = bicubic(yf, W, Hy, endx, endy - 0.5f);
+          v = v * F[(size_t)y * W + x] * F[(size_t)(y + 1) * W + x];
+          yb[(size_t)W * y + x] = v;
+        }
+      }
+    }
+    fx = 1 - fx; // vx.swap()
+    fy = 1 - fy; // vy.swap()
+  }
+
+  // simulation.cpp:166-171
+  void divergence() {
+    const float ih = 1.0f / h;
+    const float *xf = vxf(), *yf = vyf();
+#pragma omp parallel for schedule(static)
+    for (int y = 1; y < H - 1; y++)
+      for (int x = 1; x < W - 1; x++)
+        f[(size_t)y * W + x] =
+            -ih * (xf[(size_t)y * Wx + x] - xf[(size_t)y * Wx + x - 1] +
+                   yf[(size_t)y * W + x] - yf[(size_t)(y - 1) * W + x]);
+  }
+
+  // simulation.cpp:173-187
+  void apply_sinks() {
+    for (auto &s : sinks) {
+      float gx = s.x / h + 0.5f, gy = s.y / h + 0.5f;
+      if (gx <= 3 || gx > W - 3 || gy <= 3 || gy > H - 3) continue;
+      for (int yy = -1; yy <= 1; yy++)
+        for (int xx = -1; xx <= 1; xx++)
+          f[(size_t)(int)(gy + yy) * W + (int)(gx + xx)] = s.z;
+      s.z = (float)(s.z * std::pow(0.000001, (double)(dt * 50)));
+    }
+    sinks.erase(std::remove_if(sinks.begin(), sinks.end(),
+                               [](const Sink &t) { return t.z < 0.05f; }),
+                sinks.end());
+  }
+
+  // simulation.cpp:196-207
+  void correct_velocity() {
+    const float ih = 1.0f / h;
+    const float *F = flag.data();
+    float *xf = vxf(), *yf = vyf();
+#pragma omp parallel for schedule(static)
+    for (int y = 1; y < H - 1; y++)
+      for (int x = 1; x < W - 2; x++)
+        xf[(size_t)y * Wx + x] -= F[(size_t)y * W + x] * F[(size_t)y * W + x + 1] *
+                                  ih * (p[(size_t)y * W + x + 1] - p[(size_t)y * W + x]);
+#pragma omp parallel for schedule(static)
+    for (int y = 1; y < H - 2; y++)
+      for (int x = 1; x < W - 1; x++)
+        yf[(size_t)y * W + x] -= F[(size_t)y * W + x] * F[(size_t)(y + 1) * W + x] *
+                                 ih * (p[(size_t)(y + 1) * W + x] - p[(size_t)y * W + x]);
+  }
+
+  // simulation.cpp:164-208
+  void project() {
+    divergence();
+    apply_sinks();
+    for (int c = 0; c < vcycles; c++) mg.solve(p.data(), f.data(), flag.data(), h, true);
+    setPBC();
+    correct_velocity();
+  }
+
+  // simulation.cpp:16-19
+  void saveCurrent() {
+    std::copy(vx[fx].begin(), vx[fx].end(), vx_cur.begin());
+    std::copy(vy[fy].begin(), vy[fy].end(), vy_cur.begin());
+  }
+
+  // simulation.cpp:356-374
+  void step(float timestep) {
+    dt = timestep;
+    applyAccumulatedVelocity();
+    diffuse();
+    advect();
+    setVBCs();
+    project();
+    setVBCs();
+    saveCurrent();
+  }
+};
+
+// advect_tracer_points.cs:21-39: Thomas Wang's 32-bit hash and the Numerical
+// Recipes LCG.
+inline uint32_t wang_hash(uint32_t seed) {
+  seed = (seed ^ 61u) ^ (seed >> 16);
+  seed *= 9u;
+  seed = seed ^ (seed >> 4);
+  seed *= 0x27d4eb2du;
+  seed = seed ^ (seed >> 15);
+  return seed;
+}
+inline uint32_t lcg(uint32_t &s) {
+  s = 1664525u * s + 1013904223u;
+  return s;
+}
+inline float randf(uint32_t &s) { return (float)(lcg(s) % 100000u) / 100000.0f; }
+
+// simulation.cpp:398-411 (psampleFlagLinear); glm::mix(x,y,a) = x*(1-a)+y*a
+inline float flag_linear(const float *F, int W, int H, float pwidth, float px,
+                         float py) {
+  float cx = px / (pwidth / W) - 0.5f, cy = py / (pwidth / W) - 0.5f;
+  int ix = (int)cx, iy = (int)cy;
+  ix = std::max(0, std::min(W - 2, ix));
+  iy = std::max(0, std::min(H - 2, iy));
+  float sx = cx - std::floor(cx), sy = cy - std::floor(cy);
+  float p01 = F[(size_t)(iy + 1) * W + ix], p11 = F[(size_t)(iy + 1) * W + ix + 1];
+  float p00 = F[(size_t)iy * W + ix], p10 = F[(size_t)iy * W + ix + 1];
+  auto mix = [](float a, float b, float t) { return a * (1.0f - t) + b * t; };
+  return mix(mix(p00, p10, sx), mix(p01, p11, sx), sy);
+}
+
+} // namespace
+
+// ===========================================================================
+// C ABI of the oracle (loaded by ctypes from tests/ and bench.py only)
+// ===========================================================================
+
+ORACLE_API int oracle_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+}
+
+ORACLE_API int oracle_mg_level_sizes(int W, int H, int *ws, int *hs, int cap) {
+  std::vector<int> a, b;
+  level_sizes(W, H, a, b);
+  for (int i = 0; i < (int)a.size() && i < cap; i++) {
+    ws[i] = a[i];
+    hs[i] = b[i];
+  }
+  return (int)a.size();
+}
+
+ORACLE_API void oracle_coarsen_flag(const float *f, int W, float *fc, int Wc, int Hc) {
+  coarsen_flag<float>(f, W, fc, Wc, Hc);
+}
+ORACLE_API void oracle_rbgs_f32(float *p, const float *f, const float *fl, int W, int H,
+                                float h, float alpha) {
+  rbgs<float>(p, f, fl, W, H, h, alpha);
+}
+ORACLE_API void oracle_rbgs_f64(double *p, const double *f, const double *fl, int W,
+                                int H, double h, double alpha) {
+  rbgs<double>(p, f, fl, W, H, h, alpha);
+}
+ORACLE_API double oracle_residual_f32(const float *p, const float *f, const float *fl,
+                                      float *r, int W, int H, float h) {
+  return residual<float>(p, f, fl, r, W, H, h);
+}
+ORACLE_API double oracle_residual_f64(const double *p, const double *f, const double *fl,
+                                      double *r, int W, int H, double h) {
+  return residual<double>(p, f, fl, r, W, H, h);
+}
+ORACLE_API void oracle_restrict_f32(const float *r, int W, float *rc, int Wc, int Hc) {
+  restrict_fw<float>(r, W, rc, Wc, Hc);
+}
+ORACLE_API void oracle_prolongate_f32(float *e, int W, int H, const float *ec,
+                                      const float *flc, int Wc, const float *fl) {
+  prolongate<float>(e, W, H, ec, flc, Wc, fl);
+}
+ORACLE_API void oracle_correct_f32(float *p, const float *e, int W, int H) {
+  correct<float>(p, e, W, H);
+}
+ORACLE_API void oracle_zero_gradient_f32(float *p, int W, int H) {
+  zero_gradient<float>(p, W, H);
+}
+ORACLE_API float oracle_bicubic(const float *g, int gw, int gh, float cx, float cy) {
+  return bicubic(g, gw, gh, cx, cy);
+}
+ORACLE_API float oracle_bilinear(const float *g, int gw, int gh, float cx, float cy) {
+  return bilinear(g, gw, gh, cx, cy);
+}
+ORACLE_API float oracle_cubic_hermite(float t, float A, float B, float C, float D) {
+  return cubic_hermite(t, A, B, C, D);
+}
+ORACLE_API uint32_t oracle_wang_hash(uint32_t s) { return wang_hash(s); }
+ORACLE_API uint32_t oracle_lcg(uint32_t s) { return lcg(s); }
+
+// ---- MG handle (fp32 = reference, fp64 = the mixed-precision oracle) ----
+ORACLE_API void *oracle_mg_create(int W, int H, int fp64) {
+  if (fp64) return new MG<double>(W, H);
+  return new MG<float>(W, H);
+}
+ORACLE_API void oracle_mg_destroy(void *m, int fp64) {
+  if (fp64) delete (MG<double> *)m;
+  else delete (MG<float> *)m;
+}
+ORACLE_API void oracle_mg_set_params(void *m, int fp64, int nu_pre, int nu_post,
+                                     int nu_coarse, float alpha) {
+  MGParams p;
+  p.nu_pre = nu_pre;
+  p.nu_post = nu_post;
+  p.nu_coarse = nu_coarse;
+  p.alpha = alpha;
+  if (fp64) ((MG<double> *)m)->prm = p;
+  else ((MG<float> *)m)->prm = p;
+}
+ORACLE_API void oracle_mg_update_fields(void *m, const float *flag) {
+  ((MG<float> *)m)->update_fields(flag);
+}
+ORACLE_API void oracle_mg64_update_fields(void *m, const double *flag) {
+  ((MG<double> *)m)->update_fields(flag);
+}
+ORACLE_API void oracle_mg_get_flag(void *m, int level, float *out) {
+  auto *g = (MG<float> *)m;
+  std::copy(g->flagcs[level].begin(), g->flagcs[level].end(), out);
+}
+ORACLE_API void oracle_mg_solve(void *m, float *p, const float *f, const float *flag,
+                                float h, int zg) {
+  ((MG<float> *)m)->solve(p, f, flag, h, zg != 0);
+}
+ORACLE_API void oracle_mg64_solve(void *m, double *p, const double *f,
+                                  const double *flag, double h, int zg) {
+  ((MG<double> *)m)->solve(p, f, flag, h, zg != 0);
+}
+
+// ---- Simulation handle ----
+enum {
+  OF_VX = 0, OF_VX_BACK, OF_VY, OF_VY_BACK, OF_VX_ACCUM, OF_VY_ACCUM,
+  OF_VX_CURRENT, OF_VY_CURRENT, OF_P, OF_F, OF_FLAG, OF_R
+};
+
+static float *sim_field(Sim *s, int id) {
+  switch (id) {
+  case OF_VX: return s->vxf();
+  case OF_VX_BACK: return s->vxb();
+  case OF_VY: return s->vyf();
+  case OF_VY_BACK: return s->vyb();
+  case OF_VX_ACCUM: return s->vx_accum.data();
+  case OF_VY_ACCUM: return s->vy_accum.data();
+  case OF_VX_CURRENT: return s->vx_cur.data();
+  case OF_VY_CURRENT: return s->vy_cur.data();
+  case OF_P: return s->p.data();
+  case OF_F: return s->f.data();
+  case OF_FLAG: return s->flag.data();
+  case OF_R: return s->r.data();
+  }
+  return nullptr;
+}
+static size_t sim_field_size(Sim *s, int id) {
+  switch (id) {
+  case OF_VX: case OF_VX_BACK: case OF_VX_ACCUM: case OF_VX_CURRENT:
+    return (size_t)s->Wx * s->H;
+  case OF_VY: case OF_VY_BACK: case OF_VY_ACCUM: case OF_VY_CURRENT:
+    return (size_t)s->W * s->Hy;
+  default: return (size_t)s->W * s->H;
+  }
+}
+
+ORACLE_API void *oracle_sim_create(const float *flag, int W, int H, float pwidth, float mu) {
+  return new Sim(flag, W, H, pwidth, mu);
+}
+ORACLE_API void oracle_sim_destroy(void *s) { delete (Sim *)s; }
+ORACLE_API void oracle_sim_set_params(void *sp, int nu_pre, int nu_post, int nu_coarse,
+                                      int vcycles, float alpha) {
+  Sim *s = (Sim *)sp;
+  s->mg.prm.nu_pre = nu_pre;
+  s->mg.prm.nu_post = nu_post;
+  s->mg.prm.nu_coarse = nu_coarse;
+  s->mg.prm.alpha = alpha;
+  s->vcycles = vcycles;
+}
+ORACLE_API int oracle_sim_set_field(void *sp, int id, const float *src) {
+  Sim *s = (Sim *)sp;
+  float *d = sim_field(s, id);
+  if (!d) return -1;
+  std::memcpy(d, src, sim_field_size(s, id) * sizeof(float));
+  if (id == OF_FLAG) s->mg.update_fields(s->flag.data());
+  return 0;
+}
+ORACLE_API int oracle_sim_get_field(void *sp, int id, float *dst) {
+  Sim *s = (Sim *)sp;
+  float *d = sim_field(s, id);
+  if (!d) return -1;
+  std::memcpy(dst, d, sim_field_size(s, id) * sizeof(float));
+  return 0;
+}
+ORACLE_API float oracle_sim_h(void *sp) { return ((Sim *)sp)->h; }
+ORACLE_API void oracle_sim_add_sink(void *sp, float x, float y, float z) {
+  ((Sim *)sp)->sinks.push_back({x, y, z});
+}
+ORACLE_API int oracle_sim_num_sinks(void *sp) { return (int)((Sim *)sp)->sinks.size(); }
+ORACLE_API void oracle_sim_step(void *sp, float dt) { ((Sim *)sp)->step(dt); }
+
+enum { OS_ACCUM = 0, OS_DIFFUSE, OS_ADVECT, OS_SET_VBCS, OS_PROJECT, OS_SET_PBC,
+       OS_SAVE_CURRENT, OS_DIVERGENCE, OS_CORRECT_VELOCITY, OS_MG_SOLVE };
+ORACLE_API int oracle_sim_stage(void *sp, int stage, float dt) {
+  Sim *s = (Sim *)sp;
+  s->dt = dt;
+  switch (stage) {
+  case OS_ACCUM: s->applyAccumulatedVelocity(); break;
+  case OS_DIFFUSE: s->diffuse(); break;
+  case OS_ADVECT: s->advect(); break;
+  case OS_SET_VBCS: s->setVBCs(); break;
+  case OS_PROJECT: s->project(); break;
+  case OS_SET_PBC: s->setPBC(); break;
+  case OS_SAVE_CURRENT: s->saveCurrent(); break;
+  case OS_DIVERGENCE: s->divergence(); s->apply_sinks(); break;
+  case OS_CORRECT_VELOCITY: s->correct_velocity(); break;
+  case OS_MG_SOLVE: s->mg.solve(s->p.data(), s->f.data(), s->flag.data(), s->h, true); break;
+  default: return -1;
+  }
+  return 0;
+}
+
+// advect_floating_items.cpp:11-14 (bilinearVel), for n points in grid space.
+ORACLE_API void oracle_sim_bilinear_vel(void *sp, const float *c, float *out, int n) {
+  Sim *s = (Sim *)sp;
+  for (int i = 0; i < n; i++) {
+    out[2 * i] = bilinear(s->vxf(), s->Wx, s->H, c[2 * i] - 0.5f, c[2 * i + 1]);
+    out[2 * i + 1] = bilinear(s->vyf(), s->W, s->Hy, c[2 * i], c[2 * i + 1] - 0.5f);
+  }
+}
+
+ORACLE_API float oracle_flag_linear(const float *F, int W, int H, float pwidth, float px,
+                                    float py) {
+  return flag_linear(F, W, H, pwidth, px, py);
+}
+
+// One tracer step: RK2 midpoint with bilinearVel (advect_floating_items.cpp:11-14),
+// freeze/age rule and Wang/LCG respawn (advect_tracer_points.cs:42-82).  The
+// velocity is the front buffer after the step, positions are in physical units.
+ORACLE_API void oracle_sim_advect_tracers(void *sp, float *xy, float *age, int n, float dt,
+                                          uint32_t seed) {
+  Sim *s = (Sim *)sp;
+  const float h = s->h;
+  const float pdx = s->pwidth, pdy = s->pwidth * s->H / s->W;
+  const float *xf = s->vxf(), *yf = s->vyf();
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    float px = xy[2 * i], py = xy[2 * i + 1];
+    float cx = px / h, cy = py / h;
+    float v1x = bilinear(xf, s->Wx, s->H, cx - 0.5f, cy);
+    float v1y = bilinear(yf, s->W, s->Hy, cx, cy - 0.5f);
+    float mx = px + v1x * dt * 0.5f, my = py + v1y * dt * 0.5f;
+    float mcx = mx / h, mcy = my / h;
+    float v2x = bilinear(xf, s->Wx, s->H, mcx - 0.5f, mcy);
+    float v2y = bilinear(yf, s->W, s->Hy, mcx, mcy - 0.5f);
+    float nx = px + v2x * dt, ny = py + v2y * dt;
+    float a = age[i];
+    if (px < 0.0f || py < 0.0f || px > pdx || py > pdy ||
+        flag_linear(s->flag.data(), s->W, s->H, s->pwidth, px, py) < 0.6f) {
+      nx = px;
+      ny = py;
+      a += 0.1f;
+    }
+    if (a > 2.0f * 3.141f) {
+      uint32_t st = wang_hash((uint32_t)i + seed);
+      float rx = randf(st);
+      float ry = randf(st);
+      nx = rx * pdx;
+      ny = ry * pdy;
+      a = 0.0f;
+    } else {
+      a = a + 0.02f;
+    }
+    xy[2 * i] = nx;
+    xy[2 * i + 1] = ny;
+    age[i] = a;
+  }
+}

@@ -1,0 +1,1051 @@
+// capi.hip -- the C ABI of libubg.so (include/ubg_c.h): handle lifetime,
+// device memory, host copies, the MG V-cycle driver and Simulation::step.
+//
+// Host-side control flow follows the reference:
+//   MG::MG / updateFields / solve / solveLevel   pressure_solver.hpp:16-62,
+//                                                pressure_solver.cpp:201-248
+//   Simulation ctor / step / diffuse / advect /  simulation.hpp:32-67,
+//   project / setVBCs / setPBC / accumulate      simulation.cpp:16-208, 241-396
+// Every kernel is issued on the handle's stream; a whole step can be captured
+// once per (front-buffer parity, dt) and replayed as a hipGraph.
+#include "ubg_internal.hpp"
+#include "../../include/ubg_c.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+using namespace ubg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define UBG_HIP(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(e_ == hipErrorOutOfMemory ? UBG_E_OOM : UBG_E_HIP,                   \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+  } while (0)
+
+#define UBG_TRY(expr)                                                                  \
+  do {                                                                                 \
+    int rc_ = (expr);                                                                  \
+    if (rc_ != UBG_OK) return rc_;                                                     \
+  } while (0)
+
+int check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(UBG_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return UBG_OK;
+}
+
+size_t align_up(size_t n, size_t a = 256) { return (n + a - 1) / a * a; }
+
+// A simple bump allocator over one hipMalloc.
+struct Pool {
+  char *base = nullptr;
+  size_t size = 0, off = 0;
+  int init(size_t bytes) {
+    size = bytes;
+    UBG_HIP(hipMalloc((void **)&base, bytes));
+    UBG_HIP(hipMemset(base, 0, bytes));
+    return UBG_OK;
+  }
+  template <typename T> T *take(size_t n) {
+    T *p = (T *)(base + off);
+    off += align_up(n * sizeof(T));
+    return p;
+  }
+  void release() {
+    if (base) (void)hipFree(base);
+    base = nullptr;
+  }
+};
+
+void level_sizes(int W, int H, std::vector<int> &ws, std::vector<int> &hs) {
+  ws.clear();
+  hs.clear();
+  int cw = W, ch = H;
+  while (cw > 3 && ch > 3) { // pressure_solver.hpp:20-30
+    ws.push_back(cw);
+    hs.push_back(ch);
+    cw /= 2;
+    ch /= 2;
+  }
+}
+
+int validate_flag(const float *flag, size_t n, std::vector<uint8_t> &out) {
+  out.resize(n);
+  for (size_t i = 0; i < n; i++) {
+    float v = flag[i];
+    if (v == 0.0f) out[i] = 0;
+    else if (v == 1.0f) out[i] = 1;
+    else
+      return fail(UBG_E_FLAG_VALUE, "flag value at index " + std::to_string(i) + " is " +
+                                        std::to_string(v) + " (flags must be 0 or 1)");
+  }
+  return UBG_OK;
+}
+
+struct Timer {
+  bool on = false;
+  std::vector<hipEvent_t> ev; // pairs
+  size_t used = 0;
+  double total_ms = 0.0;
+  long launches = 0;
+};
+
+void default_params(ubg_params *p) {
+  p->nu_pre = 3;
+  p->nu_post = 3;
+  p->nu_coarse = 5;
+  p->vcycles_per_step = 2;
+  p->alpha = 1.0f;
+  p->use_graph = 1;
+  p->save_current = 1;
+}
+
+int check_params(const ubg_params *p) {
+  if (!p) return fail(UBG_E_INVALID, "null params");
+  if (p->nu_pre < 0 || p->nu_post < 0 || p->nu_coarse < 0 || p->vcycles_per_step < 0 ||
+      p->nu_pre > 64 || p->nu_post > 64 || p->nu_coarse > 256 || p->vcycles_per_step > 1024)
+    return fail(UBG_E_INVALID, "smoothing counts out of range");
+  if (!(p->alpha > 0.0f && p->alpha < 2.0f)) return fail(UBG_E_INVALID, "alpha must be in (0,2)");
+  return UBG_OK;
+}
+
+} // namespace
+
+// ===========================================================================
+// MG
+// ===========================================================================
+struct ubg_mg {
+  int W = 0, H = 0, levels = 0;
+  std::vector<Level> lv;
+  Pool pool;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  ubg_params prm{};
+  double *d_acc = nullptr;
+  Timer *timer = nullptr; // borrowed from the owning simulation
+  std::recursive_mutex mu;
+
+  // smoother sweep at level l (zero-gradient only on level 0 if requested)
+  void sweep(int l, float h, bool zg) {
+    const Level &L = lv[l];
+    bool timed = timer && timer->on && l == 0;
+    if (timed) {
+      if (timer->used + 2 > timer->ev.size()) {
+        size_t n = timer->ev.size();
+        timer->ev.resize(n + 512);
+        for (size_t i = n; i < timer->ev.size(); i++) (void)hipEventCreate(&timer->ev[i]);
+      }
+      (void)hipEventRecord(timer->ev[timer->used], stream);
+    }
+    launch_rbgs_fused(L, h, prm.alpha, zg, stream);
+    if (timed) {
+      (void)hipEventRecord(timer->ev[timer->used + 1], stream);
+      timer->used += 2;
+    }
+  }
+
+  // MG::solveLevel (pressure_solver.cpp:201-248)
+  void solve_level(int l, float h, bool zg) {
+    const bool z = zg && l == 0;
+    if (l == levels - 2) {
+      for (int i = 0; i < prm.nu_coarse; i++) sweep(l, h, false);
+      return;
+    }
+    if (l > 0 && coarse_fits_lds(lv.data(), l, levels)) {
+      launch_coarse_vcycle(lv.data(), l, levels, h, prm.nu_pre, prm.nu_post, prm.nu_coarse,
+                           prm.alpha, stream);
+      return;
+    }
+    for (int i = 0; i < prm.nu_pre; i++) sweep(l, h, z);
+    const Level &L = lv[l], &C = lv[l + 1];
+    launch_residual_restrict(L, C, h, stream);
+    float hc = h * (L.w - 1.0f) / (C.w - 1.0f); // pressure_solver.cpp:229
+    solve_level(l + 1, hc, false);
+    launch_prolong_correct(L, C, stream);
+    if (z) launch_zero_gradient(L.p, L.w, L.h, stream);
+    for (int i = 0; i < prm.nu_post; i++) sweep(l, h, z);
+  }
+
+  // coarse flags of levels 1..L-1 from the level-0 flag (updateFields)
+  void classify() {
+    launch_build_mask(lv[0].flag, lv[0].mask, lv[0].w, lv[0].h, stream);
+    for (int l = 1; l < levels; l++) {
+      launch_coarsen(lv[l - 1].flag, lv[l - 1].w, lv[l].flag, lv[l].w, lv[l].h, stream);
+      launch_build_mask(lv[l].flag, lv[l].mask, lv[l].w, lv[l].h, stream);
+    }
+  }
+};
+
+namespace {
+
+// Build an MG hierarchy.  If level-0 buffers are given they are borrowed
+// (the simulation's p, f, flag, mask); otherwise the MG owns them.
+int mg_create(int W, int H, hipStream_t stream, float *p0, float *f0, uint8_t *flag0,
+              uint8_t *mask0, ubg_mg **out) {
+  if (!out) return fail(UBG_E_INVALID, "null output pointer");
+  if (W < 8 || H < 8)
+    return fail(UBG_E_DOMAIN_TOO_SMALL, "MG needs at least 8 cells in each dimension");
+  auto *m = new ubg_mg();
+  m->W = W;
+  m->H = H;
+  default_params(&m->prm);
+  std::vector<int> ws, hs;
+  level_sizes(W, H, ws, hs);
+  m->levels = (int)ws.size();
+  size_t bytes = 256;
+  for (int l = 0; l < m->levels; l++) {
+    size_t n = (size_t)ws[l] * hs[l];
+    bool own = (l > 0) || !p0;
+    bytes += align_up(n * 4);              // r
+    if (own) bytes += 2 * align_up(n * 4) + 2 * align_up(n); // p, f, flag, mask
+  }
+  int rc = m->pool.init(bytes);
+  if (rc) {
+    delete m;
+    return rc;
+  }
+  m->d_acc = m->pool.take<double>(4);
+  m->lv.resize(m->levels);
+  for (int l = 0; l < m->levels; l++) {
+    Level &L = m->lv[l];
+    L.w = ws[l];
+    L.h = hs[l];
+    size_t n = (size_t)L.w * L.h;
+    L.r = m->pool.take<float>(n);
+    if (l == 0 && p0) {
+      L.p = p0;
+      L.f = f0;
+      L.flag = flag0;
+      L.mask = mask0;
+    } else {
+      L.p = m->pool.take<float>(n);
+      L.f = m->pool.take<float>(n);
+      L.flag = m->pool.take<uint8_t>(n);
+      L.mask = m->pool.take<uint8_t>(n);
+    }
+  }
+  if (stream) {
+    m->stream = stream;
+  } else {
+    hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      m->pool.release();
+      delete m;
+      return fail(UBG_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    m->own_stream = true;
+  }
+  // MG(w,h): every coarse flag starts as 1 (flagcs fill(1.0), hpp:26);
+  // level 0 too until updateFields.
+  for (int l = 0; l < m->levels; l++) {
+    if (l == 0 && p0) continue;
+    hipMemsetAsync(m->lv[l].flag, 1, (size_t)m->lv[l].w * m->lv[l].h, m->stream);
+    launch_build_mask(m->lv[l].flag, m->lv[l].mask, m->lv[l].w, m->lv[l].h, m->stream);
+  }
+  rc = check_launch();
+  if (rc) return rc;
+  UBG_HIP(hipStreamSynchronize(m->stream));
+  *out = m;
+  return UBG_OK;
+}
+
+int mg_upload_flag(ubg_mg *m, const float *flag) {
+  std::vector<uint8_t> f8;
+  UBG_TRY(validate_flag(flag, (size_t)m->W * m->H, f8));
+  UBG_HIP(hipMemcpyAsync(m->lv[0].flag, f8.data(), f8.size(), hipMemcpyHostToDevice, m->stream));
+  m->classify();
+  UBG_TRY(check_launch());
+  UBG_HIP(hipStreamSynchronize(m->stream));
+  return UBG_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+UBG_API int ubg_version(void) { return 100; }
+UBG_API const char *ubg_last_error(void) { return g_err.c_str(); }
+UBG_API void ubg_params_default(ubg_params *out) {
+  if (out) default_params(out);
+}
+UBG_API int ubg_device_count(int *count) {
+  if (!count) return fail(UBG_E_INVALID, "null count");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) n = 0;
+  *count = n;
+  return UBG_OK;
+}
+UBG_API int ubg_mg_level_sizes(int W, int H, int *widths, int *heights, int cap) {
+  std::vector<int> ws, hs;
+  level_sizes(W, H, ws, hs);
+  for (int i = 0; i < (int)ws.size() && i < cap; i++) {
+    if (widths) widths[i] = ws[i];
+    if (heights) heights[i] = hs[i];
+  }
+  return (int)ws.size();
+}
+
+UBG_API int ubg_mg_create(int width, int height, ubg_mg **out) {
+  int n = 0;
+  ubg_device_count(&n);
+  if (n <= 0) return fail(UBG_E_NO_DEVICE, "no HIP device visible");
+  return mg_create(width, height, nullptr, nullptr, nullptr, nullptr, nullptr, out);
+}
+
+UBG_API int ubg_mg_destroy(ubg_mg *m) {
+  if (!m) return UBG_OK;
+  (void)hipStreamSynchronize(m->stream);
+  if (m->own_stream) (void)hipStreamDestroy(m->stream);
+  m->pool.release();
+  delete m;
+  return UBG_OK;
+}
+
+UBG_API int ubg_mg_set_params(ubg_mg *m, const ubg_params *prm) {
+  if (!m) return fail(UBG_E_INVALID, "null mg");
+  UBG_TRY(check_params(prm));
+  std::lock_guard<std::recursive_mutex> g(m->mu);
+  m->prm = *prm;
+  return UBG_OK;
+}
+
+UBG_API int ubg_mg_num_levels(const ubg_mg *m, int *levels) {
+  if (!m || !levels) return fail(UBG_E_INVALID, "null argument");
+  *levels = m->levels;
+  return UBG_OK;
+}
+
+UBG_API int ubg_mg_update_fields(ubg_mg *m, const float *flag) {
+  if (!m || !flag) return fail(UBG_E_INVALID, "null argument");
+  std::lock_guard<std::recursive_mutex> g(m->mu);
+  return mg_upload_flag(m, flag);
+}
+
+static int level_ptr(ubg_mg *m, int level, int which, float **ptr) {
+  if (!m) return fail(UBG_E_INVALID, "null mg");
+  if (level < 0 || level >= m->levels) return fail(UBG_E_INVALID, "level out of range");
+  Level &L = m->lv[level];
+  if (which == UBG_LEVEL_P) *ptr = L.p;
+  else if (which == UBG_LEVEL_F) *ptr = L.f;
+  else if (which == UBG_LEVEL_R) *ptr = L.r;
+  else return fail(UBG_E_INVALID, "unknown level field");
+  return UBG_OK;
+}
+
+UBG_API int ubg_mg_set_level_field(ubg_mg *m, int level, int which, const float *src) {
+  float *d = nullptr;
+  UBG_TRY(level_ptr(m, level, which, &d));
+  if (!src) return fail(UBG_E_INVALID, "null source");
+  std::lock_guard<std::recursive_mutex> g(m->mu);
+  size_t n = (size_t)m->lv[level].w * m->lv[level].h;
+  UBG_HIP(hipMemcpyAsync(d, src, n * 4, hipMemcpyHostToDevice, m->stream));
+  UBG_HIP(hipStreamSynchronize(m->stream));
+  return UBG_OK;
+}
+
+UBG_API int ubg_mg_get_level_field(ubg_mg *m, int level, int which, float *dst) {
+  float *d = nullptr;
+  UBG_TRY(level_ptr(m, level, which, &d));
+  if (!dst) return fail(UBG_E_INVALID, "null destination");
+  std::lock_guard<std::recursive_mutex> g(m->mu);
+  size_t n = (size_t)m->lv[level].w * m->lv[level].h;
+  UBG_HIP(hipMemcpyAsync(dst, d, n * 4, hipMemcpyDeviceToHost, m->stream));
+  UBG_HIP(hipStreamSynchronize(m->stream));
+  return UBG_OK;
+}
+
+UBG_API int ubg_mg_get_level_flag(ubg_mg *m, int level, float *dst) {
+  if (!m || !dst) return fail(UBG_E_INVALID, "null argument");
+  if (level < 0 || level >= m->levels) return fail(UBG_E_INVALID, "level out of range");
+  std::lock_guard<std::recursive_mutex> g(m->mu);
+  const Level &L = m->lv[level];
+  size_t n = (size_t)L.w * L.h;
+  std::vector<uint8_t> tmp(n);
+  UBG_HIP(hipMemcpyAsync(tmp.data(), L.flag, n, hipMemcpyDeviceToHost, m->stream));
+  UBG_HIP(hipStreamSynchronize(m->stream));
+  for (size_t i = 0; i < n; i++) dst[i] = (float)tmp[i];
+  return UBG_OK;
+}
+
+UBG_API int ubg_mg_level_op(ubg_mg *m, int op, int level, float h, float alpha) {
+  if (!m) return fail(UBG_E_INVALID, "null mg");
+  if (level < 0 || level >= m->levels) return fail(UBG_E_INVALID, "level out of range");
+  std::lock_guard<std::recursive_mutex> g(m->mu);
+  const Level &L = m->lv[level];
+  switch (op) {
+  case UBG_MGOP_RBGS:
+    launch_rbgs_fused(L, h, alpha, false, m->stream);
+    break;
+  case UBG_MGOP_ZERO_GRADIENT:
+    launch_zero_gradient(L.p, L.w, L.h, m->stream);
+    break;
+  case UBG_MGOP_RESIDUAL:
+    launch_fill_f32(L.r, (size_t)L.w * L.h, 0.0f, m->stream);
+    launch_residual(L, h, m->stream);
+    break;
+  case UBG_MGOP_RESTRICT_RESIDUAL:
+    if (level + 1 >= m->levels) return fail(UBG_E_INVALID, "no coarser level");
+    launch_residual_restrict(L, m->lv[level + 1], h, m->stream);
+    break;
+  case UBG_MGOP_PROLONG_CORRECT:
+    if (level + 1 >= m->levels) return fail(UBG_E_INVALID, "no coarser level");
+    launch_prolong_correct(L, m->lv[level + 1], m->stream);
+    break;
+  default:
+    return fail(UBG_E_INVALID, "unknown MG operation");
+  }
+  UBG_TRY(check_launch());
+  UBG_HIP(hipStreamSynchronize(m->stream));
+  return UBG_OK;
+}
+
+UBG_API int ubg_mg_solve(ubg_mg *m, float h, int zg, int ncycles) {
+  if (!m) return fail(UBG_E_INVALID, "null mg");
+  if (ncycles < 0) return fail(UBG_E_INVALID, "negative cycle count");
+  std::lock_guard<std::recursive_mutex> g(m->mu);
+  for (int c = 0; c < ncycles; c++) m->solve_level(0, h, zg != 0);
+  UBG_TRY(check_launch());
+  return UBG_OK;
+}
+
+UBG_API int ubg_mg_solve_host(ubg_mg *m, float *p, const float *f, const float *flag, float h,
+                              int zg) {
+  if (!m || !p || !f) return fail(UBG_E_INVALID, "null argument");
+  std::lock_guard<std::recursive_mutex> g(m->mu);
+  const Level &L = m->lv[0];
+  size_t n = (size_t)L.w * L.h;
+  if (flag) {
+    std::vector<uint8_t> f8;
+    UBG_TRY(validate_flag(flag, n, f8));
+    UBG_HIP(hipMemcpyAsync(L.flag, f8.data(), n, hipMemcpyHostToDevice, m->stream));
+    launch_build_mask(L.flag, L.mask, L.w, L.h, m->stream);
+    UBG_HIP(hipStreamSynchronize(m->stream));
+  }
+  UBG_HIP(hipMemcpyAsync(L.p, p, n * 4, hipMemcpyHostToDevice, m->stream));
+  UBG_HIP(hipMemcpyAsync(L.f, f, n * 4, hipMemcpyHostToDevice, m->stream));
+  m->solve_level(0, h, zg != 0);
+  UBG_TRY(check_launch());
+  UBG_HIP(hipMemcpyAsync(p, L.p, n * 4, hipMemcpyDeviceToHost, m->stream));
+  UBG_HIP(hipStreamSynchronize(m->stream));
+  return UBG_OK;
+}
+
+UBG_API int ubg_mg_rbgs(ubg_mg *m, float h, float alpha, int nsweeps) {
+  if (!m) return fail(UBG_E_INVALID, "null mg");
+  std::lock_guard<std::recursive_mutex> g(m->mu);
+  for (int i = 0; i < nsweeps; i++) launch_rbgs_fused(m->lv[0], h, alpha, false, m->stream);
+  UBG_TRY(check_launch());
+  return UBG_OK;
+}
+
+UBG_API int ubg_mg_residual(ubg_mg *m, float h, float *r_dst, double *norm) {
+  if (!m) return fail(UBG_E_INVALID, "null mg");
+  std::lock_guard<std::recursive_mutex> g(m->mu);
+  const Level &L = m->lv[0];
+  size_t n = (size_t)L.w * L.h;
+  launch_fill_f32(L.r, n, 0.0f, m->stream);
+  launch_residual(L, h, m->stream);
+  launch_fill_f64(m->d_acc, 1, 0.0, m->stream);
+  launch_sumsq(L.r, n, m->d_acc, m->stream);
+  UBG_TRY(check_launch());
+  double acc = 0.0;
+  UBG_HIP(hipMemcpyAsync(&acc, m->d_acc, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+  if (r_dst) UBG_HIP(hipMemcpyAsync(r_dst, L.r, n * 4, hipMemcpyDeviceToHost, m->stream));
+  UBG_HIP(hipStreamSynchronize(m->stream));
+  if (norm) *norm = std::sqrt(acc);
+  if (!std::isfinite(acc)) return fail(UBG_E_NONFINITE, "residual norm is not finite");
+  return UBG_OK;
+}
+
+UBG_API int ubg_mg_sync(ubg_mg *m) {
+  if (!m) return fail(UBG_E_INVALID, "null mg");
+  UBG_HIP(hipStreamSynchronize(m->stream));
+  return UBG_OK;
+}
+
+} // extern "C"
+
+// ===========================================================================
+// Simulation
+// ===========================================================================
+struct SinkH {
+  float x, y, z;
+};
+
+struct ubg_sim {
+  int W = 0, H = 0;
+  float pwidth = 0, mu = 0, h = 0, dt = 0;
+  BCs bc;
+  float *vx[2] = {nullptr, nullptr}, *vy[2] = {nullptr, nullptr};
+  int fx = 0, fy = 0;
+  float *vx_acc = nullptr, *vy_acc = nullptr, *vx_cur = nullptr, *vy_cur = nullptr;
+  float *p = nullptr, *f = nullptr, *r = nullptr;
+  uint8_t *flag = nullptr, *mask = nullptr;
+  bool accum_dirty = false;
+  ubg_mg *mg = nullptr;
+  Pool pool;
+  hipStream_t stream = nullptr;
+  ubg_params prm{};
+  std::vector<SinkH> sinks;
+  int *d_sink_xy = nullptr;
+  float *d_sink_val = nullptr;
+  int sink_cap = 0;
+  float *tr_xy = nullptr, *tr_age = nullptr;
+  int ntr = 0, tr_cap = 0;
+  uint32_t tr_seed = 1;
+  Timer timer;
+  // captured step graphs, one per front-buffer parity
+  struct G {
+    bool valid = false;
+    hipGraphExec_t exec = nullptr;
+    float dt = 0;
+    int fx = 0, fy = 0;
+  } graphs[2];
+  std::recursive_mutex mu_;
+
+  VelView vel() const { return VelView{W, H, vx[fx], vx[1 - fx], vy[fy], vy[1 - fy]}; }
+
+  void set_vbcs() { launch_set_vbcs(vel(), bc, stream); }
+
+  // simulation.cpp:104-162
+  void diffuse() {
+    const float a = dt * mu * (W - 1.0f) / pwidth;
+    for (int it = 1; it < 3; it++) {
+      launch_diffuse_vx(vel(), flag, a, stream);
+      fx = 1 - fx;
+      set_vbcs();
+    }
+    for (int it = 1; it < 3; it++) {
+      launch_diffuse_vy(vel(), flag, a, stream);
+      fy = 1 - fy;
+      set_vbcs();
+    }
+  }
+
+  // simulation.cpp:241-354
+  void advect() {
+    const float ih = 1.0f / h;
+    launch_advect(vel(), flag, 0.5f * dt * ih, dt * ih, stream);
+    fx = 1 - fx;
+    fy = 1 - fy;
+  }
+
+  // sinks (simulation.cpp:173-187): stamps resolved on the host
+  int apply_sinks() {
+    if (sinks.empty()) return UBG_OK;
+    std::vector<int> xy;
+    std::vector<float> val;
+    for (auto &s : sinks) {
+      float gx = s.x / h + 0.5f, gy = s.y / h + 0.5f;
+      if (gx <= 3 || gx > W - 3 || gy <= 3 || gy > H - 3) continue;
+      xy.push_back((int)gx);
+      xy.push_back((int)gy);
+      val.push_back(s.z);
+      s.z = (float)(s.z * std::pow(0.000001, (double)(dt * 50)));
+    }
+    sinks.erase(std::remove_if(sinks.begin(), sinks.end(),
+                               [](const SinkH &t) { return t.z < 0.05f; }),
+                sinks.end());
+    int n = (int)val.size();
+    if (n == 0) return UBG_OK;
+    if (n > sink_cap) {
+      if (d_sink_xy) (void)hipFree(d_sink_xy);
+      if (d_sink_val) (void)hipFree(d_sink_val);
+      sink_cap = std::max(n, 64);
+      UBG_HIP(hipMalloc((void **)&d_sink_xy, sizeof(int) * 2 * sink_cap));
+      UBG_HIP(hipMalloc((void **)&d_sink_val, sizeof(float) * sink_cap));
+    }
+    UBG_HIP(hipMemcpyAsync(d_sink_xy, xy.data(), sizeof(int) * 2 * n, hipMemcpyHostToDevice, stream));
+    UBG_HIP(hipMemcpyAsync(d_sink_val, val.data(), sizeof(float) * n, hipMemcpyHostToDevice, stream));
+    launch_stamp_sinks(f, W, d_sink_xy, d_sink_val, n, stream);
+    // the host arrays must outlive the async copies
+    UBG_HIP(hipStreamSynchronize(stream));
+    return UBG_OK;
+  }
+
+  void divergence() { launch_divergence(vel(), f, 1.0f / h, stream); }
+
+  void correct_velocity() { launch_correct_velocity(vel(), p, flag, 1.0f / h, stream); }
+
+  void mg_solve() { mg->solve_level(0, h, true); }
+
+  // simulation.cpp:164-208
+  int project() {
+    divergence();
+    UBG_TRY(apply_sinks());
+    for (int c = 0; c < prm.vcycles_per_step; c++) mg_solve();
+    launch_set_pbc(p, W, H, bc, stream);
+    correct_velocity();
+    return UBG_OK;
+  }
+
+  void save_current() {
+    (void)hipMemcpyAsync(vx_cur, vx[fx], sizeof(float) * (W - 1) * H, hipMemcpyDeviceToDevice, stream);
+    (void)hipMemcpyAsync(vy_cur, vy[fy], sizeof(float) * W * (H - 1), hipMemcpyDeviceToDevice, stream);
+  }
+
+  // the part of step() after applyAccumulatedVelocity, without sinks
+  void step_body() {
+    diffuse();
+    advect();
+    set_vbcs();
+    divergence();
+    for (int c = 0; c < prm.vcycles_per_step; c++) mg_solve();
+    launch_set_pbc(p, W, H, bc, stream);
+    correct_velocity();
+    set_vbcs();
+    if (prm.save_current) save_current();
+  }
+
+  // simulation.cpp:356-374
+  int step(float timestep) {
+    dt = timestep;
+    if (accum_dirty) {
+      launch_accumulate(vel(), vx_acc, vy_acc, stream);
+      accum_dirty = false;
+    }
+    bool graphable = prm.use_graph && sinks.empty() && !timer.on;
+    if (graphable) {
+      int slot = fx; // fx and fy always flip together over a step
+      G &g = graphs[slot];
+      if (!g.valid || g.dt != dt || g.fx != fx || g.fy != fy) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+        g.valid = false;
+        int fx0 = fx, fy0 = fy;
+        hipGraph_t graph;
+        UBG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        step_body();
+        hipError_t e = hipStreamEndCapture(stream, &graph);
+        if (e != hipSuccess) return fail(UBG_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(UBG_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+        g.valid = true;
+        g.dt = dt;
+        g.fx = fx0;
+        g.fy = fy0;
+        // capture did not execute anything: restore the buffer roles
+        fx = fx0;
+        fy = fy0;
+      }
+      UBG_HIP(hipGraphLaunch(g.exec, stream));
+      fx = 1 - fx;
+      fy = 1 - fy;
+    } else {
+      diffuse();
+      advect();
+      set_vbcs();
+      UBG_TRY(project());
+      set_vbcs();
+      if (prm.save_current) save_current();
+    }
+    return check_launch();
+  }
+
+  void invalidate_graphs() {
+    for (auto &g : graphs) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      g.exec = nullptr;
+      g.valid = false;
+    }
+  }
+
+  float *field(int id) {
+    switch (id) {
+    case UBG_FIELD_VX: return vx[fx];
+    case UBG_FIELD_VX_BACK: return vx[1 - fx];
+    case UBG_FIELD_VY: return vy[fy];
+    case UBG_FIELD_VY_BACK: return vy[1 - fy];
+    case UBG_FIELD_VX_ACCUM: return vx_acc;
+    case UBG_FIELD_VY_ACCUM: return vy_acc;
+    case UBG_FIELD_VX_CURRENT: return vx_cur;
+    case UBG_FIELD_VY_CURRENT: return vy_cur;
+    case UBG_FIELD_P: return p;
+    case UBG_FIELD_F: return f;
+    case UBG_FIELD_R: return r;
+    }
+    return nullptr;
+  }
+  void shape(int id, int *w, int *hh) const {
+    switch (id) {
+    case UBG_FIELD_VX: case UBG_FIELD_VX_BACK: case UBG_FIELD_VX_ACCUM: case UBG_FIELD_VX_CURRENT:
+      *w = W - 1; *hh = H; return;
+    case UBG_FIELD_VY: case UBG_FIELD_VY_BACK: case UBG_FIELD_VY_ACCUM: case UBG_FIELD_VY_CURRENT:
+      *w = W; *hh = H - 1; return;
+    default:
+      *w = W; *hh = H;
+    }
+  }
+};
+
+extern "C" {
+
+UBG_API int ubg_sim_create(const float *flag, int width, int height, float pwidth, float mu,
+                           ubg_sim **out) {
+  if (!flag || !out) return fail(UBG_E_INVALID, "null argument");
+  if (width < 8 || height < 8)
+    return fail(UBG_E_DOMAIN_TOO_SMALL, "Simulation needs at least 8 cells in each dimension");
+  int ndev = 0;
+  ubg_device_count(&ndev);
+  if (ndev <= 0) return fail(UBG_E_NO_DEVICE, "no HIP device visible");
+  std::vector<uint8_t> f8;
+  UBG_TRY(validate_flag(flag, (size_t)width * height, f8));
+  auto *s = new ubg_sim();
+  s->W = width;
+  s->H = height;
+  s->pwidth = pwidth;
+  s->mu = mu;
+  default_params(&s->prm);
+  const size_t nc = (size_t)width * height, nx = (size_t)(width - 1) * height,
+               ny = (size_t)width * (height - 1);
+  size_t bytes = 256 + 4 * align_up(nx * 4) + 4 * align_up(ny * 4) + 3 * align_up(nc * 4) +
+                 2 * align_up(nc);
+  int rc = s->pool.init(bytes);
+  if (rc) {
+    delete s;
+    return rc;
+  }
+  for (int b = 0; b < 2; b++) {
+    s->vx[b] = s->pool.take<float>(nx);
+    s->vy[b] = s->pool.take<float>(ny);
+  }
+  s->vx_acc = s->pool.take<float>(nx);
+  s->vy_acc = s->pool.take<float>(ny);
+  s->vx_cur = s->pool.take<float>(nx);
+  s->vy_cur = s->pool.take<float>(ny);
+  s->p = s->pool.take<float>(nc);
+  s->f = s->pool.take<float>(nc);
+  s->r = s->pool.take<float>(nc);
+  s->flag = s->pool.take<uint8_t>(nc);
+  s->mask = s->pool.take<uint8_t>(nc);
+  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    s->pool.release();
+    delete s;
+    return fail(UBG_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  // simulation.hpp:55-60: vy(x,0) = 0 and vx(0,y) = 1 in both buffers
+  std::vector<float> col((size_t)(width - 1) * height, 0.0f);
+  for (int y = 0; y < height; y++) col[(size_t)y * (width - 1)] = 1.0f;
+  for (int b = 0; b < 2; b++)
+    (void)hipMemcpyAsync(s->vx[b], col.data(), col.size() * 4, hipMemcpyHostToDevice, s->stream);
+  (void)hipMemcpyAsync(s->flag, f8.data(), nc, hipMemcpyHostToDevice, s->stream);
+  (void)hipStreamSynchronize(s->stream);
+  rc = mg_create(width, height, s->stream, s->p, s->f, s->flag, s->mask, &s->mg);
+  if (rc) {
+    (void)hipStreamDestroy(s->stream);
+    s->pool.release();
+    delete s;
+    return rc;
+  }
+  s->mg->timer = &s->timer;
+  s->mg->classify(); // MG(flag) -> updateFields (simulation.hpp:62)
+  rc = check_launch();
+  if (rc) return rc;
+  s->h = pwidth / (width - 1.0f); // simulation.hpp:63
+  UBG_HIP(hipStreamSynchronize(s->stream));
+  *out = s;
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_destroy(ubg_sim *s) {
+  if (!s) return UBG_OK;
+  (void)hipStreamSynchronize(s->stream);
+  s->invalidate_graphs();
+  for (auto ev : s->timer.ev) (void)hipEventDestroy(ev);
+  ubg_mg_destroy(s->mg);
+  if (s->d_sink_xy) (void)hipFree(s->d_sink_xy);
+  if (s->d_sink_val) (void)hipFree(s->d_sink_val);
+  if (s->tr_xy) (void)hipFree(s->tr_xy);
+  if (s->tr_age) (void)hipFree(s->tr_age);
+  (void)hipStreamDestroy(s->stream);
+  s->pool.release();
+  delete s;
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_set_params(ubg_sim *s, const ubg_params *prm) {
+  if (!s) return fail(UBG_E_INVALID, "null sim");
+  UBG_TRY(check_params(prm));
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->prm = *prm;
+  s->mg->prm = *prm;
+  s->invalidate_graphs();
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_get_params(const ubg_sim *s, ubg_params *out) {
+  if (!s || !out) return fail(UBG_E_INVALID, "null argument");
+  *out = s->prm;
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_dims(const ubg_sim *s, int *w, int *h, float *hh, float *pw, float *mu) {
+  if (!s) return fail(UBG_E_INVALID, "null sim");
+  if (w) *w = s->W;
+  if (h) *h = s->H;
+  if (hh) *hh = s->h;
+  if (pw) *pw = s->pwidth;
+  if (mu) *mu = s->mu;
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_field_shape(const ubg_sim *s, int field, int *w, int *h) {
+  if (!s || !w || !h) return fail(UBG_E_INVALID, "null argument");
+  if (field < 0 || field >= UBG_FIELD_COUNT) return fail(UBG_E_INVALID, "unknown field");
+  s->shape(field, w, h);
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_set_field(ubg_sim *s, int field, const float *src) {
+  if (!s || !src) return fail(UBG_E_INVALID, "null argument");
+  if (field < 0 || field >= UBG_FIELD_COUNT) return fail(UBG_E_INVALID, "unknown field");
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  int w, h;
+  s->shape(field, &w, &h);
+  size_t n = (size_t)w * h;
+  if (field == UBG_FIELD_FLAG) {
+    UBG_TRY(mg_upload_flag(s->mg, src)); // memcpy + mg.updateFields
+    return UBG_OK;
+  }
+  UBG_HIP(hipMemcpyAsync(s->field(field), src, n * 4, hipMemcpyHostToDevice, s->stream));
+  UBG_HIP(hipStreamSynchronize(s->stream));
+  if (field == UBG_FIELD_VX_ACCUM || field == UBG_FIELD_VY_ACCUM) s->accum_dirty = true;
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_get_field(ubg_sim *s, int field, float *dst) {
+  if (!s || !dst) return fail(UBG_E_INVALID, "null argument");
+  if (field < 0 || field >= UBG_FIELD_COUNT) return fail(UBG_E_INVALID, "unknown field");
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  int w, h;
+  s->shape(field, &w, &h);
+  size_t n = (size_t)w * h;
+  if (field == UBG_FIELD_FLAG) {
+    std::vector<uint8_t> tmp(n);
+    UBG_HIP(hipMemcpyAsync(tmp.data(), s->flag, n, hipMemcpyDeviceToHost, s->stream));
+    UBG_HIP(hipStreamSynchronize(s->stream));
+    for (size_t i = 0; i < n; i++) dst[i] = (float)tmp[i];
+    return UBG_OK;
+  }
+  UBG_HIP(hipMemcpyAsync(dst, s->field(field), n * 4, hipMemcpyDeviceToHost, s->stream));
+  UBG_HIP(hipStreamSynchronize(s->stream));
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_field_device_ptr(ubg_sim *s, int field, void **dptr) {
+  if (!s || !dptr) return fail(UBG_E_INVALID, "null argument");
+  if (field < 0 || field >= UBG_FIELD_COUNT) return fail(UBG_E_INVALID, "unknown field");
+  *dptr = field == UBG_FIELD_FLAG ? (void *)s->flag : (void *)s->field(field);
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_step(ubg_sim *s, float dt) {
+  if (!s) return fail(UBG_E_INVALID, "null sim");
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  return s->step(dt);
+}
+
+UBG_API int ubg_sim_advect_tracers(ubg_sim *s, float dt, uint32_t seed) {
+  if (!s) return fail(UBG_E_INVALID, "null sim");
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  launch_tracers(s->vel(), s->flag, s->h, s->pwidth, dt, seed, s->tr_xy, s->tr_age, s->ntr,
+                 s->stream);
+  return check_launch();
+}
+
+UBG_API int ubg_sim_run(ubg_sim *s, float dt, int nsteps) {
+  if (!s) return fail(UBG_E_INVALID, "null sim");
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  for (int i = 0; i < nsteps; i++) {
+    UBG_TRY(s->step(dt));
+    if (s->ntr > 0) {
+      launch_tracers(s->vel(), s->flag, s->h, s->pwidth, dt, s->tr_seed++, s->tr_xy, s->tr_age,
+                     s->ntr, s->stream);
+    }
+  }
+  return check_launch();
+}
+
+UBG_API int ubg_sim_stage(ubg_sim *s, int stage, float dt) {
+  if (!s) return fail(UBG_E_INVALID, "null sim");
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->dt = dt;
+  switch (stage) {
+  case UBG_STAGE_ACCUM:
+    launch_accumulate(s->vel(), s->vx_acc, s->vy_acc, s->stream);
+    s->accum_dirty = false;
+    break;
+  case UBG_STAGE_DIFFUSE: s->diffuse(); break;
+  case UBG_STAGE_ADVECT: s->advect(); break;
+  case UBG_STAGE_SET_VBCS: s->set_vbcs(); break;
+  case UBG_STAGE_PROJECT: UBG_TRY(s->project()); break;
+  case UBG_STAGE_SET_PBC: launch_set_pbc(s->p, s->W, s->H, s->bc, s->stream); break;
+  case UBG_STAGE_SAVE_CURRENT: s->save_current(); break;
+  case UBG_STAGE_DIVERGENCE:
+    s->divergence();
+    UBG_TRY(s->apply_sinks());
+    break;
+  case UBG_STAGE_CORRECT_VELOCITY: s->correct_velocity(); break;
+  case UBG_STAGE_MG_SOLVE: s->mg_solve(); break;
+  default: return fail(UBG_E_INVALID, "unknown stage");
+  }
+  UBG_TRY(check_launch());
+  UBG_HIP(hipStreamSynchronize(s->stream));
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_set_grids(ubg_sim *s, int x, int y, float val) {
+  if (!s) return fail(UBG_E_INVALID, "null sim");
+  if (val != 0.0f && val != 1.0f) return fail(UBG_E_FLAG_VALUE, "flag value must be 0 or 1");
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  // simulation.hpp:82-98: out-of-range cells are ignored
+  if (!(x >= 0 && y >= 0 && x < s->W && y < s->H)) return UBG_OK;
+  const int W = s->W;
+  uint8_t v8 = val != 0.0f ? 1 : 0;
+  UBG_HIP(hipMemcpyAsync(s->flag + (size_t)y * W + x, &v8, 1, hipMemcpyHostToDevice, s->stream));
+  if (val == 0.0f) {
+    const float z = 0.0f;
+    if (x < W - 1) UBG_HIP(hipMemcpyAsync(s->vx[s->fx] + (size_t)y * (W - 1) + x, &z, 4, hipMemcpyHostToDevice, s->stream));
+    if (x > 0) UBG_HIP(hipMemcpyAsync(s->vx[s->fx] + (size_t)y * (W - 1) + x - 1, &z, 4, hipMemcpyHostToDevice, s->stream));
+    if (y < s->H - 1) UBG_HIP(hipMemcpyAsync(s->vy[s->fy] + (size_t)y * W + x, &z, 4, hipMemcpyHostToDevice, s->stream));
+    if (y > 0) UBG_HIP(hipMemcpyAsync(s->vy[s->fy] + (size_t)(y - 1) * W + x, &z, 4, hipMemcpyHostToDevice, s->stream));
+    UBG_HIP(hipMemcpyAsync(s->p + (size_t)y * W + x, &z, 4, hipMemcpyHostToDevice, s->stream));
+  }
+  // the level-0 neighbour mask around the cell
+  launch_build_mask(s->flag, s->mask, s->W, s->H, s->stream);
+  UBG_TRY(check_launch());
+  UBG_HIP(hipStreamSynchronize(s->stream));
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_add_sink(ubg_sim *s, float x, float y, float z) {
+  if (!s) return fail(UBG_E_INVALID, "null sim");
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->sinks.push_back({x, y, z});
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_get_sinks(const ubg_sim *s, float *xyz, int cap, int *count) {
+  if (!s || !count) return fail(UBG_E_INVALID, "null argument");
+  *count = (int)s->sinks.size();
+  for (int i = 0; i < (int)s->sinks.size() && i < cap && xyz; i++) {
+    xyz[3 * i] = s->sinks[i].x;
+    xyz[3 * i + 1] = s->sinks[i].y;
+    xyz[3 * i + 2] = s->sinks[i].z;
+  }
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_get_mg(ubg_sim *s, ubg_mg **mg) {
+  if (!s || !mg) return fail(UBG_E_INVALID, "null argument");
+  *mg = s->mg;
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_bilinear_vel(ubg_sim *s, const float *c, float *out, int n) {
+  if (!s || !c || !out || n < 0) return fail(UBG_E_INVALID, "bad argument");
+  if (n == 0) return UBG_OK;
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  float *d = nullptr;
+  UBG_HIP(hipMalloc((void **)&d, sizeof(float) * 4 * n));
+  hipMemcpyAsync(d, c, sizeof(float) * 2 * n, hipMemcpyHostToDevice, s->stream);
+  launch_bilinear_vel(s->vel(), d, d + 2 * n, n, s->stream);
+  int rc = check_launch();
+  hipMemcpyAsync(out, d + 2 * n, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, s->stream);
+  hipError_t e = hipStreamSynchronize(s->stream);
+  (void)hipFree(d);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(UBG_E_HIP, hipGetErrorString(e));
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_set_tracers(ubg_sim *s, const float *xy, const float *age, int n) {
+  if (!s || n < 0 || (n > 0 && !xy)) return fail(UBG_E_INVALID, "bad argument");
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  if (n > s->tr_cap) {
+    if (s->tr_xy) (void)hipFree(s->tr_xy);
+    if (s->tr_age) (void)hipFree(s->tr_age);
+    s->tr_xy = s->tr_age = nullptr;
+    s->tr_cap = 0;
+    UBG_HIP(hipMalloc((void **)&s->tr_xy, sizeof(float) * 2 * (size_t)n));
+    UBG_HIP(hipMalloc((void **)&s->tr_age, sizeof(float) * (size_t)n));
+    s->tr_cap = n;
+  }
+  s->ntr = n;
+  if (n == 0) return UBG_OK;
+  UBG_HIP(hipMemcpyAsync(s->tr_xy, xy, sizeof(float) * 2 * (size_t)n, hipMemcpyHostToDevice, s->stream));
+  if (age) {
+    UBG_HIP(hipMemcpyAsync(s->tr_age, age, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, s->stream));
+  } else {
+    launch_fill_f32(s->tr_age, (size_t)n, 0.0f, s->stream);
+  }
+  UBG_TRY(check_launch());
+  UBG_HIP(hipStreamSynchronize(s->stream));
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_get_tracers(ubg_sim *s, float *xy, float *age, int n) {
+  if (!s || n < 0 || n > s->ntr) return fail(UBG_E_INVALID, "bad argument");
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  if (n == 0) return UBG_OK;
+  if (xy) UBG_HIP(hipMemcpyAsync(xy, s->tr_xy, sizeof(float) * 2 * (size_t)n, hipMemcpyDeviceToHost, s->stream));
+  if (age) UBG_HIP(hipMemcpyAsync(age, s->tr_age, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, s->stream));
+  UBG_HIP(hipStreamSynchronize(s->stream));
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_sync(ubg_sim *s) {
+  if (!s) return fail(UBG_E_INVALID, "null sim");
+  UBG_HIP(hipStreamSynchronize(s->stream));
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_timer_enable(ubg_sim *s, int timer, int on) {
+  if (!s || timer != UBG_TIMER_SMOOTH_L0) return fail(UBG_E_INVALID, "bad argument");
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  UBG_HIP(hipStreamSynchronize(s->stream));
+  s->timer.on = on != 0;
+  s->timer.used = 0;
+  s->timer.total_ms = 0.0;
+  s->timer.launches = 0;
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_timer_read(ubg_sim *s, int timer, double *total_ms, long *launches) {
+  if (!s || timer != UBG_TIMER_SMOOTH_L0) return fail(UBG_E_INVALID, "bad argument");
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  UBG_HIP(hipStreamSynchronize(s->stream));
+  Timer &t = s->timer;
+  for (size_t i = 0; i + 1 < t.used; i += 2) {
+    float ms = 0.0f;
+    UBG_HIP(hipEventElapsedTime(&ms, t.ev[i], t.ev[i + 1]));
+    t.total_ms += ms;
+    t.launches++;
+  }
+  t.used = 0;
+  if (total_ms) *total_ms = t.total_ms;
+  if (launches) *launches = t.launches;
+  return UBG_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,422 @@
+// sim_kernels.hip -- fluid-step kernels for gfx950 (simulation.cpp of the
+// reference): boundary conditions, accumulation, masked Jacobi diffusion,
+// semi-Lagrangian RK2 advection with Catmull-Rom sampling, divergence,
+// velocity correction, bilinear velocity sampling and tracer advection.
+// Memory-bound gathers and stencils; one thread per face / cell / point.
+#include "ubg_internal.hpp"
+
+namespace ubg {
+
+namespace {
+
+constexpr int TPB = 256;
+inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// simulation.cpp:50-78
+__device__ __forceinline__ float vbc_par(int bc, float a, float b) {
+  if (bc == BC_INFLOW) return b;
+  if (bc == BC_OUTFLOW || bc == BC_OUTFLOW_ZERO_PRESSURE) return fmaxf(a, 0.0f);
+  return 0.0f;
+}
+__device__ __forceinline__ float vbc_per(int bc, float a, float b) {
+  if (bc == BC_INFLOW) return b;
+  if (bc == BC_OUTFLOW || bc == BC_OUTFLOW_ZERO_PRESSURE) return fmaxf(a, 0.0f);
+  return -a;
+}
+
+// setVBCs (simulation.cpp:80-102).  The reference runs four sequential loops:
+// vx W/E columns, vx S/N rows, vy W/E columns, vy S/N rows.  Only the row
+// loops read values the column loops wrote (the corners' neighbours), so one
+// workgroup runs both column loops, a barrier, then both row loops, which
+// reproduces the sequential result exactly.
+__global__ void __launch_bounds__(1024) k_set_vbcs(float *__restrict__ xf, float *__restrict__ xb,
+                                                    float *__restrict__ yf, float *__restrict__ yb,
+                                                    int W, int H, int bw, int be, int bn, int bs) {
+  const int Wx = W - 1, Hy = H - 1;
+  for (int y = threadIdx.x; y < H; y += blockDim.x) {
+    size_t r0 = (size_t)y * Wx;
+    float w0 = vbc_par(bw, xf[r0 + 1], xf[r0]);
+    float e0 = vbc_par(be, xf[r0 + Wx - 2], xf[r0 + Wx - 1]);
+    xf[r0] = xb[r0] = w0;
+    xf[r0 + Wx - 1] = xb[r0 + Wx - 1] = e0;
+  }
+  for (int y = threadIdx.x; y < Hy; y += blockDim.x) {
+    size_t r0 = (size_t)y * W;
+    float w0 = vbc_per(bw, yf[r0 + 1], yf[r0]);
+    float e0 = vbc_per(be, yf[r0 + W - 2], yf[r0 + W - 1]);
+    yf[r0] = yb[r0] = w0;
+    yf[r0 + W - 1] = yb[r0 + W - 1] = e0;
+  }
+  __syncthreads();
+  for (int x = threadIdx.x; x < Wx; x += blockDim.x) {
+    size_t top = (size_t)(H - 1) * Wx + x, top1 = (size_t)(H - 2) * Wx + x;
+    float s0 = vbc_per(bs, xf[(size_t)Wx + x], xf[x]);
+    float n0 = vbc_per(bn, xf[top1], xf[top]);
+    xf[x] = xb[x] = s0;
+    xf[top] = xb[top] = n0;
+  }
+  for (int x = threadIdx.x; x < W; x += blockDim.x) {
+    size_t top = (size_t)(Hy - 1) * W + x, top1 = (size_t)(Hy - 2) * W + x;
+    float s0 = vbc_par(bs, yf[(size_t)W + x], yf[x]);
+    float n0 = vbc_par(bn, yf[top1], yf[top]);
+    yf[x] = yb[x] = s0;
+    yf[top] = yb[top] = n0;
+  }
+}
+
+// setPBC (simulation.cpp:36-45): W/E columns over all rows, then S/N rows over
+// all columns; only OUTFLOW_ZERO_PRESSURE negates.
+__device__ __forceinline__ float pbc(int bc, float a) {
+  return bc == BC_OUTFLOW_ZERO_PRESSURE ? -a : a;
+}
+__global__ void __launch_bounds__(1024) k_set_pbc(float *__restrict__ p, int W, int H, int bw,
+                                                   int be, int bn, int bs) {
+  for (int y = threadIdx.x; y < H; y += blockDim.x) {
+    size_t r0 = (size_t)y * W;
+    float w0 = pbc(bw, p[r0 + 1]);
+    float e0 = pbc(be, p[r0 + W - 2]);
+    p[r0] = w0;
+    p[r0 + W - 1] = e0;
+  }
+  __syncthreads();
+  for (int x = threadIdx.x; x < W; x += blockDim.x) {
+    float s0 = pbc(bs, p[(size_t)W + x]);
+    float n0 = pbc(bn, p[(size_t)(H - 2) * W + x]);
+    p[x] = s0;
+    p[(size_t)(H - 1) * W + x] = n0;
+  }
+}
+
+// applyAccumulatedVelocity (simulation.cpp:376-396), interior faces only.
+__global__ void k_accumulate(float *__restrict__ xf, float *__restrict__ xa,
+                             float *__restrict__ yf, float *__restrict__ ya, int W, int H) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int y = blockIdx.y + 1;
+  const int Wx = W - 1, Hy = H - 1;
+  if (x < Wx - 1 && y < H - 1) {
+    size_t i = (size_t)y * Wx + x;
+    xf[i] += xa[i];
+    xa[i] = 0.0f;
+  }
+  if (x < W - 1 && y < Hy - 1) {
+    size_t i = (size_t)y * W + x;
+    yf[i] += ya[i];
+    ya[i] = 0.0f;
+  }
+}
+
+__device__ __forceinline__ float FL(const uint8_t *__restrict__ fl, int W, int x, int y) {
+  return (float)fl[(size_t)y * W + x];
+}
+
+// diffuse, vx part (simulation.cpp:114-129): one Jacobi iteration of the
+// masked implicit diffusion; the y-neighbour masks use column x-1.
+__global__ void k_diffuse_vx(const float *__restrict__ s, float *__restrict__ d,
+                             const uint8_t *__restrict__ fl, int W, int H, float a) {
+  const int Wx = W - 1;
+  int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int y = blockIdx.y + 1;
+  if (x >= Wx - 1) return;
+  size_t i = (size_t)y * Wx + x;
+  float vc = s[i];
+  float val = 0.0f;
+  val += s[i + 1] * FL(fl, W, x + 1, y) * FL(fl, W, x + 2, y);
+  val += s[i - 1] * FL(fl, W, x, y) * FL(fl, W, x - 1, y);
+  float fvn = FL(fl, W, x, y + 1) * FL(fl, W, x - 1, y + 1);
+  val += s[i + Wx] * fvn + (1.0f - fvn) * -vc;
+  float fvs = FL(fl, W, x, y - 1) * FL(fl, W, x - 1, y - 1);
+  val += s[i - Wx] * fvs + (1.0f - fvs) * -vc;
+  d[i] = FL(fl, W, x, y) * FL(fl, W, x + 1, y) * (vc + a * val) / (1.0f + 4.0f * a);
+}
+
+// diffuse, vy part (simulation.cpp:140-155)
+__global__ void k_diffuse_vy(const float *__restrict__ s, float *__restrict__ d,
+                             const uint8_t *__restrict__ fl, int W, int H, float a) {
+  const int Hy = H - 1;
+  int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int y = blockIdx.y + 1;
+  if (x >= W - 1 || y >= Hy - 1) return;
+  size_t i = (size_t)y * W + x;
+  float vc = s[i];
+  float val = 0.0f;
+  val += s[i - W] * FL(fl, W, x, y) * FL(fl, W, x, y - 1);
+  val += s[i + W] * FL(fl, W, x, y + 1) * FL(fl, W, x, y + 2);
+  float fve = FL(fl, W, x + 1, y) * FL(fl, W, x + 1, y + 1);
+  val += s[i + 1] * fve + (1.0f - fve) * -vc;
+  float fvw = FL(fl, W, x - 1, y) * FL(fl, W, x - 1, y + 1);
+  val += s[i - 1] * fvw + (1.0f - fvw) * -vc;
+  d[i] = FL(fl, W, x, y) * FL(fl, W, x, y + 1) * (vc + a * val) / (1.0f + 4.0f * a);
+}
+
+// CubicHermite (interpolators.hpp:78-85), Catmull-Rom
+__device__ __forceinline__ float cubic_hermite(float t, float A, float B, float C, float D) {
+  float a = -A / 2.0f + (3.0f * B) / 2.0f - (3.0f * C) / 2.0f + D / 2.0f;
+  float b = A - (5.0f * B) / 2.0f + 2.0f * C - D / 2.0f;
+  float c = -A / 2.0f + C / 2.0f;
+  return a * t * t * t + b * t * t + c * t + B;
+}
+
+// bicubicSample for one lane (interpolators.hpp:92-206): clamp to [3, n-3],
+// truncate, Catmull-Rom down each of four columns, then across.
+__device__ __forceinline__ float bicubic(const float *__restrict__ g, int gw, int gh, float cx,
+                                         float cy) {
+  cx = fmaxf(fminf(cx, gw - 3.0f), 3.0f);
+  cy = fmaxf(fminf(cy, gh - 3.0f), 3.0f);
+  int icx = (int)cx, icy = (int)cy;
+  float stx = cx - truncf(cx), sty = cy - truncf(cy);
+  const float *b0 = g + (ptrdiff_t)(icy - 1) * gw + (icx - 1);
+  const float *b1 = b0 + gw, *b2 = b1 + gw, *b3 = b2 + gw;
+  float c0 = cubic_hermite(sty, b0[0], b1[0], b2[0], b3[0]);
+  float c1 = cubic_hermite(sty, b0[1], b1[1], b2[1], b3[1]);
+  float c2 = cubic_hermite(sty, b0[2], b1[2], b2[2], b3[2]);
+  float c3 = cubic_hermite(sty, b0[3], b1[3], b2[3], b3[3]);
+  return cubic_hermite(stx, c0, c1, c2, c3);
+}
+
+// advect (simulation.cpp:241-354).  Thread = one lane of one 8-lane chunk.
+// blockIdx.z selects the vx (0) or vy (1) sweep.  A chunk starting at
+// x0 = 1 + 8k is processed only if some lane has both flags of the skip test
+// set; the skip test and write mask differ as in the reference.  Lanes never
+// reached (tail columns, skipped chunks) keep the back buffer.
+__global__ void k_advect(const float *__restrict__ xf, const float *__restrict__ yf,
+                         float *__restrict__ xb, float *__restrict__ yb,
+                         const uint8_t *__restrict__ fl, int W, int H, int nck_x, int nck_y,
+                         float hdt, float fdt) {
+  const int Wx = W - 1, Hy = H - 1;
+  const int lane = blockIdx.x * blockDim.x + threadIdx.x; // 0-based: x = lane + 1
+  const int y = blockIdx.y + 1;
+  const bool is_vy = blockIdx.z != 0;
+  if (lane >= 8 * (is_vy ? nck_y : nck_x)) return;
+  const int x = lane + 1;
+  const int x0 = 1 + (lane & ~7);
+  const uint8_t *frow = fl + (size_t)y * W;
+  bool any = false;
+  if (!is_vy) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) any |= (frow[x0 + k - 1] & frow[x0 + k]) != 0;
+  } else {
+    const uint8_t *fdn = frow - W;
+#pragma unroll
+    for (int k = 0; k < 8; k++) any |= (frow[x0 + k] & fdn[x0 + k]) != 0;
+  }
+  if (!any) return;
+  if (!is_vy) {
+    float posx = (float)x + 0.5f, posy = (float)y;
+    float vx1 = xf[(size_t)x + (size_t)y * Wx];
+    float vy1 = ((yf[x + (size_t)y * W] + yf[x + (size_t)(y - 1) * W]) +
+                 (yf[x + 1 + (size_t)y * W] + yf[x + 1 + (size_t)(y - 1) * W])) * 0.25f;
+    float midx = posx - vx1 * hdt, midy = posy - vy1 * hdt;
+    float vx2 = bicubic(xf, Wx, H, midx - 0.5f, midy);
+    float vy2 = bicubic(yf, W, Hy, midx, midy - 0.5f);
+    float endx = posx - vx2 * fdt, endy = posy - vy2 * fdt;
+    float v = bicubic(xf, Wx, H, endx - 0.5f, endy);
+    v = v * (float)frow[x] * (float)frow[x + 1];
+    xb[(size_t)Wx * y + x] = v;
+  } else {
+    float posy = (float)y + 0.5f, posx = (float)x;
+    float vy1 = yf[(size_t)x + (size_t)y * W];
+    float vx1 = ((xf[x + (size_t)y * Wx] + xf[x + (size_t)(y - 1) * Wx]) +
+                 (xf[x + 1 + (size_t)y * Wx] + xf[x + 1 + (size_t)(y - 1) * Wx])) * 0.25f;
+    float midx = posx - vx1 * hdt, midy = posy - vy1 * hdt;
+    float vx2 = bicubic(xf, Wx, H, midx - 0.5f, midy);
+    float vy2 = bicubic(yf, W, Hy, midx, midy - 0.5f);
+    float endx = posx - vx2 * fdt, endy = posy - vy2 * fdt;
+    float v = bicubic(yf, W, Hy, endx, endy - 0.5f);
+    v = v * (float)frow[x] * (float)frow[x + W];
+    yb[(size_t)W * y + x] = v;
+  }
+}
+
+// project's divergence (simulation.cpp:166-171), interior, not flag-masked.
+__global__ void k_divergence(const float *__restrict__ xf, const float *__restrict__ yf,
+                             float *__restrict__ f, int W, int H, float ih) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int y = blockIdx.y + 1;
+  if (x >= W - 1) return;
+  const int Wx = W - 1;
+  size_t ix = (size_t)y * Wx + x;
+  size_t iy = (size_t)y * W + x;
+  f[iy] = -ih * (xf[ix] - xf[ix - 1] + yf[iy] - yf[iy - W]);
+}
+
+// 3x3 sink stamps (simulation.cpp:173-182); positions resolved on the host.
+__global__ void k_stamp_sinks(float *__restrict__ f, int W, const int *__restrict__ xy,
+                              const float *__restrict__ val, int n) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 9 * n) return;
+  int s = t / 9, k = t % 9;
+  int gx = xy[2 * s] + (k % 3) - 1, gy = xy[2 * s + 1] + (k / 3) - 1;
+  f[(size_t)gy * W + gx] = val[s];
+}
+
+// project's velocity correction (simulation.cpp:196-207)
+__global__ void k_correct_velocity(float *__restrict__ xf, float *__restrict__ yf,
+                                   const float *__restrict__ p, const uint8_t *__restrict__ fl,
+                                   int W, int H, float ih) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int y = blockIdx.y + 1;
+  if (x >= W - 1) return;
+  const int Wx = W - 1;
+  size_t c = (size_t)y * W + x;
+  float Fc = (float)fl[c];
+  float pc = p[c];
+  if (x < W - 2)
+    xf[(size_t)y * Wx + x] -= Fc * (float)fl[c + 1] * ih * (p[c + 1] - pc);
+  if (y < H - 2)
+    yf[(size_t)y * W + x] -= Fc * (float)fl[c + W] * ih * (p[c + W] - pc);
+}
+
+// scalar bilinearSample (interpolators.hpp:11-27), clamp to [0, n-1.1]
+__device__ __forceinline__ float bilinear(const float *__restrict__ g, int gw, int gh, float cx,
+                                          float cy) {
+  cx = fminf(fmaxf(cx, 0.0f), gw - 1.1f);
+  cy = fminf(fmaxf(cy, 0.0f), gh - 1.1f);
+  int ix = (int)cx, iy = (int)cy;
+  float sx = cx - floorf(cx), sy = cy - floorf(cy);
+  const float *r0 = g + (size_t)iy * gw + ix;
+  float v1 = r0[0], v2 = r0[1], v3 = r0[gw], v4 = r0[gw + 1];
+  float vm1 = v1 + (v2 - v1) * sx;
+  float vm2 = v3 + (v4 - v3) * sx;
+  return vm1 + (vm2 - vm1) * sy;
+}
+
+// Simulation::bilinearVel (advect_floating_items.cpp:11-14)
+__device__ __forceinline__ float2 bilinear_vel(const float *__restrict__ xf,
+                                               const float *__restrict__ yf, int W, int H,
+                                               float cx, float cy) {
+  return make_float2(bilinear(xf, W - 1, H, cx - 0.5f, cy), bilinear(yf, W, H - 1, cx, cy - 0.5f));
+}
+
+__global__ void k_bilinear_vel(const float *__restrict__ xf, const float *__restrict__ yf, int W,
+                               int H, const float *__restrict__ c, float *__restrict__ out, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float2 v = bilinear_vel(xf, yf, W, H, c[2 * i], c[2 * i + 1]);
+  out[2 * i] = v.x;
+  out[2 * i + 1] = v.y;
+}
+
+// advect_tracer_points.cs:21-39
+__device__ __forceinline__ uint32_t wang_hash(uint32_t seed) {
+  seed = (seed ^ 61u) ^ (seed >> 16);
+  seed *= 9u;
+  seed = seed ^ (seed >> 4);
+  seed *= 0x27d4eb2du;
+  seed = seed ^ (seed >> 15);
+  return seed;
+}
+__device__ __forceinline__ float randf(uint32_t &s) {
+  s = 1664525u * s + 1013904223u;
+  return (float)(s % 100000u) / 100000.0f;
+}
+
+// psampleFlagLinear (simulation.cpp:398-411); mix(x,y,a) = x*(1-a) + y*a
+__device__ __forceinline__ float flag_linear(const uint8_t *__restrict__ F, int W, int H,
+                                             float pwidth, float px, float py) {
+  float cx = px / (pwidth / W) - 0.5f, cy = py / (pwidth / W) - 0.5f;
+  int ix = (int)cx, iy = (int)cy;
+  ix = max(0, min(W - 2, ix));
+  iy = max(0, min(H - 2, iy));
+  float sx = cx - floorf(cx), sy = cy - floorf(cy);
+  const uint8_t *r0 = F + (size_t)iy * W + ix;
+  float p00 = r0[0], p10 = r0[1], p01 = r0[W], p11 = r0[W + 1];
+  float m0 = p00 * (1.0f - sx) + p10 * sx;
+  float m1 = p01 * (1.0f - sx) + p11 * sx;
+  return m0 * (1.0f - sy) + m1 * sy;
+}
+
+// One tracer step (advect_tracer_points.cs:42-82 with bilinearVel sampling):
+// RK2 midpoint; a point outside [0,pdim] or in a solid (linear flag < 0.6)
+// freezes and ages 0.1; past age 2*3.141 it respawns from the Wang hash and
+// the LCG, otherwise it ages 0.02.
+__global__ void k_tracers(const float *__restrict__ xf, const float *__restrict__ yf,
+                          const uint8_t *__restrict__ F, int W, int H, float h, float pwidth,
+                          float dt, uint32_t seed, float2 *__restrict__ xy,
+                          float *__restrict__ age, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float pdx = pwidth, pdy = pwidth * H / W;
+  float2 P = xy[i];
+  float2 v1 = bilinear_vel(xf, yf, W, H, P.x / h, P.y / h);
+  float mx = P.x + v1.x * dt * 0.5f, my = P.y + v1.y * dt * 0.5f;
+  float2 v2 = bilinear_vel(xf, yf, W, H, mx / h, my / h);
+  float nx = P.x + v2.x * dt, ny = P.y + v2.y * dt;
+  float a = age[i];
+  if (P.x < 0.0f || P.y < 0.0f || P.x > pdx || P.y > pdy ||
+      flag_linear(F, W, H, pwidth, P.x, P.y) < 0.6f) {
+    nx = P.x;
+    ny = P.y;
+    a += 0.1f;
+  }
+  if (a > 2.0f * 3.141f) {
+    uint32_t st = wang_hash((uint32_t)i + seed);
+    float rx = randf(st);
+    float ry = randf(st);
+    nx = rx * pdx;
+    ny = ry * pdy;
+    a = 0.0f;
+  } else {
+    a = a + 0.02f;
+  }
+  xy[i] = make_float2(nx, ny);
+  age[i] = a;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+void launch_set_vbcs(const VelView &v, const BCs &bc, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_vbcs, dim3(1), dim3(1024), 0, s, v.vxf, v.vxb, v.vyf, v.vyb, v.W, v.H,
+                     bc.west, bc.east, bc.north, bc.south);
+}
+void launch_set_pbc(float *p, int W, int H, const BCs &bc, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_pbc, dim3(1), dim3(1024), 0, s, p, W, H, bc.west, bc.east, bc.north,
+                     bc.south);
+}
+void launch_accumulate(const VelView &v, float *xa, float *ya, hipStream_t s) {
+  hipLaunchKernelGGL(k_accumulate, dim3(ceil_div(v.W - 2, TPB), v.H - 2), dim3(TPB), 0, s, v.vxf,
+                     xa, v.vyf, ya, v.W, v.H);
+}
+void launch_diffuse_vx(const VelView &v, const uint8_t *flag, float a, hipStream_t s) {
+  hipLaunchKernelGGL(k_diffuse_vx, dim3(ceil_div(v.W - 3, TPB), v.H - 2), dim3(TPB), 0, s, v.vxf,
+                     v.vxb, flag, v.W, v.H, a);
+}
+void launch_diffuse_vy(const VelView &v, const uint8_t *flag, float a, hipStream_t s) {
+  hipLaunchKernelGGL(k_diffuse_vy, dim3(ceil_div(v.W - 2, TPB), v.H - 3), dim3(TPB), 0, s, v.vyf,
+                     v.vyb, flag, v.W, v.H, a);
+}
+void launch_advect(const VelView &v, const uint8_t *flag, float hdt, float fdt, hipStream_t s) {
+  const int Wx = v.W - 1;
+  int nck_x = Wx > 9 ? (Wx - 2) / 8 : 0;
+  int nck_y = v.W > 9 ? (v.W - 2) / 8 : 0;
+  int lanes = 8 * (nck_x > nck_y ? nck_x : nck_y);
+  if (lanes == 0 || v.H < 3) return;
+  hipLaunchKernelGGL(k_advect, dim3(ceil_div(lanes, TPB), v.H - 2, 2), dim3(TPB), 0, s, v.vxf,
+                     v.vyf, v.vxb, v.vyb, flag, v.W, v.H, nck_x, nck_y, hdt, fdt);
+}
+void launch_divergence(const VelView &v, float *f, float ih, hipStream_t s) {
+  hipLaunchKernelGGL(k_divergence, dim3(ceil_div(v.W - 2, TPB), v.H - 2), dim3(TPB), 0, s, v.vxf,
+                     v.vyf, f, v.W, v.H, ih);
+}
+void launch_stamp_sinks(float *f, int W, const int *xy, const float *val, int n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_stamp_sinks, dim3(ceil_div(9 * n, TPB)), dim3(TPB), 0, s, f, W, xy, val, n);
+}
+void launch_correct_velocity(const VelView &v, const float *p, const uint8_t *flag, float ih,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_correct_velocity, dim3(ceil_div(v.W - 2, TPB), v.H - 2), dim3(TPB), 0, s,
+                     v.vxf, v.vyf, p, flag, v.W, v.H, ih);
+}
+void launch_bilinear_vel(const VelView &v, const float *c, float *out, int n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_bilinear_vel, dim3(ceil_div(n, TPB)), dim3(TPB), 0, s, v.vxf, v.vyf, v.W,
+                     v.H, c, out, n);
+}
+void launch_tracers(const VelView &v, const uint8_t *flag, float h, float pwidth, float dt,
+                    uint32_t seed, float *xy, float *age, int n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_tracers, dim3(ceil_div(n, TPB)), dim3(TPB), 0, s, v.vxf, v.vyf, flag, v.W,
+                     v.H, h, pwidth, dt, seed, (float2 *)xy, age, n);
+}
+
+} // namespace ubg

@@ -1,0 +1,86 @@
+// ubg_internal.hpp -- device data layout and kernel launchers of libubg.so.
+//
+// HBM layout (one GPU):
+//   * every field is a dense row-major fp32 array, index y*w + x, exactly the
+//     host layout of Single2DGrid (db2dgrid.hpp:12-50), so host copies are
+//     plain memcpys and flat-offset quirks of the reference carry over;
+//   * flags are one byte per cell: `flag` holds the 0/1 value, `mask` holds the
+//     cell's own flag in bit 0 and its four neighbours' flags in bits 1-4, so
+//     the smoother, the residual and the prolongation read one byte per cell;
+//   * an MG hierarchy is one allocation carved into levels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ubg {
+
+enum : uint8_t {
+  MK_SELF = 1u, // F(x, y)
+  MK_W = 2u,    // F(x-1, y)
+  MK_E = 4u,    // F(x+1, y)
+  MK_S = 8u,    // F(x, y-1)
+  MK_N = 16u,   // F(x, y+1)
+};
+
+// Boundary-condition kinds of Simulation::BC (simulation.hpp:69)
+enum BCKind : int { BC_INFLOW = 0, BC_OUTFLOW = 1, BC_OUTFLOW_ZERO_PRESSURE = 2, BC_NOSLIP = 3 };
+
+struct Level {
+  int w = 0, h = 0;
+  float *p = nullptr;      // solution: level 0 pressure, coarse levels ecs
+  float *f = nullptr;      // right-hand side: level 0 f, coarse levels rcs
+  float *r = nullptr;      // residual scratch (stage-wise residual only)
+  uint8_t *flag = nullptr; // 0/1 flag (level 0 input, coarse flagcs)
+  uint8_t *mask = nullptr; // own + neighbour flag bits
+};
+
+struct BCs {
+  int west = BC_INFLOW, east = BC_OUTFLOW_ZERO_PRESSURE, north = BC_NOSLIP, south = BC_NOSLIP;
+};
+
+// Staggered velocity state of one domain: vx is (W-1) x H, vy is W x (H-1).
+struct VelView {
+  int W, H;
+  float *vxf, *vxb, *vyf, *vyb;
+};
+
+// ---- generic -------------------------------------------------------------
+void launch_fill_f32(float *dst, size_t n, float v, hipStream_t s);
+void launch_fill_f64(double *dst, size_t n, double v, hipStream_t s);
+void launch_f32_from_u8(const uint8_t *src, float *dst, size_t n, hipStream_t s);
+void launch_sumsq(const float *src, size_t n, double *acc, hipStream_t s);
+
+// ---- multigrid (pressure_solver.hpp / pressure_solver.cpp) ---------------
+void launch_build_mask(const uint8_t *flag, uint8_t *mask, int w, int h, hipStream_t s);
+void launch_coarsen(const uint8_t *ff, int wf, uint8_t *fc, int wc, int hc, hipStream_t s);
+// one red+black sweep, two colour passes
+void launch_rbgs_sweep(const Level &L, float h, float alpha, hipStream_t s);
+// one red+black sweep fused in one launch (LDS tile with a two-cell halo);
+// zero_gradient applies setZeroGradientBC to the result.
+void launch_rbgs_fused(const Level &L, float h, float alpha, bool zero_gradient, hipStream_t s);
+void launch_zero_gradient(float *p, int w, int h, hipStream_t s);
+void launch_residual(const Level &L, float h, hipStream_t s);
+void launch_residual_restrict(const Level &F, const Level &C, float h, hipStream_t s);
+void launch_prolong_correct(const Level &F, const Level &C, hipStream_t s);
+// single-workgroup V-cycle of every level from `first` down, all in LDS
+bool coarse_fits_lds(const Level *lv, int first, int levels);
+void launch_coarse_vcycle(const Level *lv, int first, int levels, float h, int nu_pre,
+                          int nu_post, int nu_coarse, float alpha, hipStream_t s);
+
+// ---- fluid step (simulation.cpp) -----------------------------------------
+void launch_set_vbcs(const VelView &v, const BCs &bc, hipStream_t s);
+void launch_set_pbc(float *p, int W, int H, const BCs &bc, hipStream_t s);
+void launch_accumulate(const VelView &v, float *vx_acc, float *vy_acc, hipStream_t s);
+void launch_diffuse_vx(const VelView &v, const uint8_t *flag, float a, hipStream_t s);
+void launch_diffuse_vy(const VelView &v, const uint8_t *flag, float a, hipStream_t s);
+void launch_advect(const VelView &v, const uint8_t *flag, float hdt, float fdt, hipStream_t s);
+void launch_divergence(const VelView &v, float *f, float ih, hipStream_t s);
+void launch_stamp_sinks(float *f, int W, const int *xy, const float *val, int n, hipStream_t s);
+void launch_correct_velocity(const VelView &v, const float *p, const uint8_t *flag, float ih,
+                             hipStream_t s);
+void launch_bilinear_vel(const VelView &v, const float *c, float *out, int n, hipStream_t s);
+void launch_tracers(const VelView &v, const uint8_t *flag, float h, float pwidth, float dt,
+                    uint32_t seed, float *xy, float *age, int n, hipStream_t s);
+
+} // namespace ubg

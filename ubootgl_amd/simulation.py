@@ -1,0 +1,280 @@
+"""Python mirror of the reference's Simulation / MG interface over libubg.so.
+
+Names and argument meaning follow simulation.hpp:18-141 and
+pressure_solver.hpp:3-76; fields live on the GPU and are exchanged with host
+numpy arrays explicitly (`get(name)` / `set(name, array)`), which is the one
+behavioural difference from the reference's public std::vector members.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check, fptr, lib
+
+_FIELD_IDS = {n: i for i, n in enumerate(L.FIELD_NAMES)}
+
+
+def _fid(name):
+    if isinstance(name, int):
+        return name
+    return _FIELD_IDS[name]
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def level_sizes(W, H):
+    """MG level sizes: halve while both sizes exceed 3 (pressure_solver.hpp:16-31)."""
+    ws = (C.c_int * 64)()
+    hs = (C.c_int * 64)()
+    n = lib().ubg_mg_level_sizes(W, H, ws, hs, 64)
+    return [(ws[i], hs[i]) for i in range(n)]
+
+
+class Params:
+    """Solver parameters; defaults are the reference's (3/3/5 sweeps, 2 V-cycles, alpha 1)."""
+
+    def __init__(self, **kw):
+        self._p = L.default_params()
+        for k, v in kw.items():
+            if not hasattr(self._p, k):
+                raise AttributeError(k)
+            setattr(self._p, k, v)
+
+    def __getattr__(self, k):
+        return getattr(self._p, k)
+
+
+class MG:
+    """pressure_solver.hpp:13-76.  MG(width, height) or MG(flag) (which also
+    runs updateFields); solve() takes host grids like the reference."""
+
+    def __init__(self, width_or_flag, height=None, _handle=None, _owner=None):
+        self._owner = _owner
+        if _handle is not None:
+            self._h = _handle
+            self._borrowed = True
+        else:
+            self._borrowed = False
+            if height is None:
+                flag = _f32(width_or_flag)
+                H, W = flag.shape
+            else:
+                W, H, flag = int(width_or_flag), int(height), None
+            h = C.c_void_p()
+            check(lib().ubg_mg_create(W, H, C.byref(h)))
+            self._h = h
+            if flag is not None:
+                self.updateFields(flag)
+        n = C.c_int()
+        check(lib().ubg_mg_num_levels(self._h, C.byref(n)))
+        self.levels = n.value
+        self.sizes = None
+        if _handle is None:
+            self.set_sizes(W, H)
+
+    def _level_shape(self, level):
+        if self.sizes is None:
+            raise RuntimeError("sizes unknown")
+        W, H = self.sizes[level]
+        return (H, W)
+
+    def set_sizes(self, W, H):
+        self.sizes = level_sizes(W, H)
+
+    def set_params(self, params):
+        check(lib().ubg_mg_set_params(self._h, C.byref(params._p)))
+
+    def updateFields(self, flag):
+        flag = _f32(flag)
+        if self.sizes is None:
+            self.set_sizes(flag.shape[1], flag.shape[0])
+        check(lib().ubg_mg_update_fields(self._h, fptr(flag)))
+
+    def solve(self, p, f, flag, h, zeroGradientBC=False):
+        """MG::solve on host grids; p (float32, C-contiguous) is updated in place."""
+        assert p.dtype == np.float32 and p.flags.c_contiguous
+        if self.sizes is None:
+            self.set_sizes(p.shape[1], p.shape[0])
+        fl = None if flag is None else _f32(flag)
+        check(lib().ubg_mg_solve_host(self._h, fptr(p), fptr(_f32(f)), fptr(fl), h,
+                                      int(bool(zeroGradientBC))))
+        return p
+
+    # resident-field interface -------------------------------------------------
+    def set_level(self, level, which, arr):
+        check(lib().ubg_mg_set_level_field(self._h, level, which, fptr(_f32(arr))))
+
+    def get_level(self, level, which):
+        out = np.zeros(self._level_shape(level), np.float32)
+        check(lib().ubg_mg_get_level_field(self._h, level, which, fptr(out)))
+        return out
+
+    def get_flag(self, level):
+        out = np.zeros(self._level_shape(level), np.float32)
+        check(lib().ubg_mg_get_level_flag(self._h, level, fptr(out)))
+        return out
+
+    def level_op(self, op, level, h, alpha=1.0):
+        check(lib().ubg_mg_level_op(self._h, op, level, h, alpha))
+
+    def solve_resident(self, h, zeroGradientBC=False, ncycles=1):
+        check(lib().ubg_mg_solve(self._h, h, int(bool(zeroGradientBC)), ncycles))
+
+    def rbgs(self, h, alpha=1.0, nsweeps=1):
+        check(lib().ubg_mg_rbgs(self._h, h, alpha, nsweeps))
+
+    def residual(self, h, want_field=True):
+        """calculateResidualField on level 0: (r, ||r||_2)."""
+        r = np.zeros(self._level_shape(0), np.float32) if want_field else None
+        n = C.c_double()
+        check(lib().ubg_mg_residual(self._h, h, fptr(r), C.byref(n)))
+        return r, n.value
+
+    def sync(self):
+        check(lib().ubg_mg_sync(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and not self._borrowed:
+            lib().ubg_mg_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Simulation:
+    """simulation.hpp:18-141: Simulation(flag, pwidth, mu)."""
+
+    def __init__(self, flag, pwidth, mu, params=None):
+        flag = _f32(flag)
+        self.height, self.width = flag.shape
+        h = C.c_void_p()
+        check(lib().ubg_sim_create(fptr(flag), self.width, self.height, pwidth, mu, C.byref(h)))
+        self._h = h
+        self.pwidth, self.mu = float(pwidth), float(mu)
+        hh = C.c_float()
+        check(lib().ubg_sim_dims(self._h, None, None, C.byref(hh), None, None))
+        self.h = hh.value
+        if params is not None:
+            self.set_params(params)
+        mgh = C.c_void_p()
+        check(lib().ubg_sim_get_mg(self._h, C.byref(mgh)))
+        self.mg = MG(None, _handle=mgh, _owner=self)
+        self.mg.set_sizes(self.width, self.height)
+        self.dt = 0.0
+
+    def set_params(self, params):
+        check(lib().ubg_sim_set_params(self._h, C.byref(params._p)))
+
+    def shape(self, name):
+        w, h = C.c_int(), C.c_int()
+        check(lib().ubg_sim_field_shape(self._h, _fid(name), C.byref(w), C.byref(h)))
+        return (h.value, w.value)
+
+    def get(self, name):
+        out = np.zeros(self.shape(name), np.float32)
+        check(lib().ubg_sim_get_field(self._h, _fid(name), fptr(out)))
+        return out
+
+    def set(self, name, arr):
+        arr = _f32(arr)
+        if arr.shape != self.shape(name):
+            raise ValueError(f"{name}: shape {arr.shape} != {self.shape(name)}")
+        check(lib().ubg_sim_set_field(self._h, _fid(name), fptr(arr)))
+
+    # reference methods --------------------------------------------------------
+    def step(self, timestep):
+        self.dt = timestep
+        check(lib().ubg_sim_step(self._h, timestep))
+
+    def run(self, dt, nsteps):
+        check(lib().ubg_sim_run(self._h, dt, nsteps))
+
+    def stage(self, st, dt=None):
+        check(lib().ubg_sim_stage(self._h, st, self.dt if dt is None else dt))
+
+    def applyAccumulatedVelocity(self):
+        self.stage(L.STAGE_ACCUM)
+
+    def diffuse(self):
+        self.stage(L.STAGE_DIFFUSE)
+
+    def advect(self):
+        self.stage(L.STAGE_ADVECT)
+
+    def setVBCs(self):
+        self.stage(L.STAGE_SET_VBCS)
+
+    def project(self):
+        self.stage(L.STAGE_PROJECT)
+
+    def setPBC(self):
+        self.stage(L.STAGE_SET_PBC)
+
+    def saveCurrentVelocityFields(self):
+        self.stage(L.STAGE_SAVE_CURRENT)
+
+    def setGrids(self, x, y, val):
+        check(lib().ubg_sim_set_grids(self._h, int(x), int(y), float(val)))
+
+    def add_sink(self, x, y, z):
+        check(lib().ubg_sim_add_sink(self._h, x, y, z))
+
+    def sinks(self):
+        n = C.c_int()
+        check(lib().ubg_sim_get_sinks(self._h, None, 0, C.byref(n)))
+        out = np.zeros((n.value, 3), np.float32)
+        if n.value:
+            check(lib().ubg_sim_get_sinks(self._h, fptr(out), n.value, C.byref(n)))
+        return out
+
+    def bilinearVel(self, c):
+        c = _f32(c).reshape(-1, 2)
+        out = np.zeros_like(c)
+        check(lib().ubg_sim_bilinear_vel(self._h, fptr(c), fptr(out), c.shape[0]))
+        return out
+
+    def set_tracers(self, xy, age=None):
+        xy = _f32(xy).reshape(-1, 2)
+        a = None if age is None else _f32(age)
+        check(lib().ubg_sim_set_tracers(self._h, fptr(xy), fptr(a), xy.shape[0]))
+        self.ntracers = xy.shape[0]
+
+    def get_tracers(self):
+        n = self.ntracers
+        xy = np.zeros((n, 2), np.float32)
+        age = np.zeros(n, np.float32)
+        check(lib().ubg_sim_get_tracers(self._h, fptr(xy), fptr(age), n))
+        return xy, age
+
+    def advect_tracers(self, dt, seed):
+        check(lib().ubg_sim_advect_tracers(self._h, dt, seed))
+
+    def sync(self):
+        check(lib().ubg_sim_sync(self._h))
+
+    def timer_enable(self, on=True):
+        check(lib().ubg_sim_timer_enable(self._h, L.TIMER_SMOOTH_L0, int(on)))
+
+    def timer_read(self):
+        ms, n = C.c_double(), C.c_long()
+        check(lib().ubg_sim_timer_read(self._h, L.TIMER_SMOOTH_L0, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            self.mg._h = None
+            lib().ubg_sim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
