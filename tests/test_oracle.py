@@ -1,0 +1,627 @@
+"""CPU tests that pin the oracle (oracle/ubg_oracle.cpp) to what the
+mathematics and the reference's own known-answer test fix.
+
+The reference cannot be compiled here (db2dgrid.hpp:5 needs glm, and
+pressure_solver.cpp:2 pulls in GLEW through draw_2dbuf.hpp), so each oracle
+function is pinned by closed forms, invariants, special cases and brute force
+chosen so that a plausible restatement mistake (wrong sign, index, weight,
+colour or loop bound) fails at least one check.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from oracle import pyoracle as O
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def rand_flag(W, H, seed, frac=0.2, walls=True):
+    g = np.random.default_rng(seed)
+    f = (g.random((H, W)) > frac).astype(np.float64)
+    if walls:
+        f[0, :] = f[-1, :] = 0
+        f[:, 0] = f[:, -1] = 0
+    return f
+
+
+# ---------------------------------------------------------------- hierarchy
+def test_level_sizes_known():
+    # pressure_solver.hpp:20-30: halve while both sizes > 3
+    assert O.level_sizes(4096, 2048)[-1] == (8, 4) and len(O.level_sizes(4096, 2048)) == 10
+    assert len(O.level_sizes(512, 256)) == 7
+    # the reference game's default level (resources/level2_hires2.png, 1090x436)
+    assert O.level_sizes(1090, 436) == [(1090, 436), (545, 218), (272, 109), (136, 54),
+                                         (68, 27), (34, 13), (17, 6)]
+    assert O.level_sizes(1025, 1025)[-1] == (4, 4)
+    assert O.level_sizes(8, 8) == [(8, 8), (4, 4)]
+    assert O.level_sizes(7, 100) == [(7, 100)]
+
+
+def test_coarse_flag_rule_all_patterns():
+    """updateFields (pressure_solver.hpp:43-56): coarse = [sum(w f)/16 > 0.2]
+    with weights [1 2 1; 2 4 2; 1 2 1]; for 0/1 flags that is weight >= 4.
+    Every 3x3 pattern on a 5x5 fine grid maps to the single coarse interior cell
+    (1,1) of a 3x3... coarse grid; boundary coarse cells are 1."""
+    wts = np.array([[1, 2, 1], [2, 4, 2], [1, 2, 1]])
+    for bits in range(512):
+        pat = np.array([(bits >> k) & 1 for k in range(9)], np.float32).reshape(3, 3)
+        fine = np.zeros((6, 6), np.float32)
+        fine[1:4, 1:4] = pat  # coarse cell (1,1) reads fine (1..3, 1..3)
+        fc = O.coarsen_flag(fine, 3, 3)
+        expect = 1.0 if (wts * pat).sum() >= 4 else 0.0
+        assert fc[1, 1] == expect
+        assert fc[0, :].min() == 1 and fc[2, :].min() == 1 and fc[:, 0].min() == 1
+
+
+# ---------------------------------------------------------------- smoother
+def laplacian_matrix(flag, h):
+    """Masked 5-point operator of the smoother/residual on interior cells:
+    (A p)_i = (sum_n F_n (p_n - p_i)) / h^2 ; unknowns = interior cells, other
+    cells are fixed data (returned as the boundary coupling)."""
+    H, W = flag.shape
+    idx = -np.ones((H, W), int)
+    cells = [(y, x) for y in range(1, H - 1) for x in range(1, W - 1)]
+    for k, (y, x) in enumerate(cells):
+        idx[y, x] = k
+    rows, cols, vals = [], [], []
+    for k, (y, x) in enumerate(cells):
+        diag = 0.0
+        for dy, dx in ((0, -1), (0, 1), (-1, 0), (1, 0)):
+            Fn = flag[y + dy, x + dx]
+            if Fn:
+                diag -= Fn
+                j = idx[y + dy, x + dx]
+                if j >= 0:
+                    rows.append(k)
+                    cols.append(j)
+                    vals.append(Fn / h / h)
+        rows.append(k)
+        cols.append(k)
+        vals.append(diag / h / h)
+    n = len(cells)
+    return sp.csr_matrix((vals, (rows, cols)), shape=(n, n)), cells, idx
+
+
+def test_rbgs_fixed_point_exact_solution():
+    """A discrete solution of the smoother's equations is left unchanged by a
+    sweep (fp64 oracle): p = (sum F_n p_n + f h^2)/sum F_n on fluid cells."""
+    W, H = 23, 17
+    flag = rand_flag(W, H, 1)
+    g = np.random.default_rng(2)
+    p = g.standard_normal((H, W)) * flag
+    h = 0.1
+    f = np.zeros((H, W))
+    for y in range(1, H - 1):
+        for x in range(1, W - 1):
+            if not flag[y, x]:
+                continue
+            s = flag[y, x - 1] + flag[y, x + 1] + flag[y - 1, x] + flag[y + 1, x]
+            v = (flag[y, x - 1] * p[y, x - 1] + flag[y, x + 1] * p[y, x + 1] +
+                 flag[y - 1, x] * p[y - 1, x] + flag[y + 1, x] * p[y + 1, x])
+            if s:
+                f[y, x] = (s * p[y, x] - v) / (h * h)
+            else:
+                p[y, x] = 0.0
+    q = p.copy()
+    O.rbgs(q, f, flag, h, 1.0)
+    assert np.abs(q - p).max() < 1e-12
+
+
+def test_rbgs_converges_to_direct_solve():
+    """Repeated sweeps converge to the solution of A p = -f (Dirichlet data on
+    a fluid boundary) computed by a sparse direct solver."""
+    W, H = 20, 14
+    flag = np.ones((H, W))
+    flag[5:8, 6:9] = 0  # an obstacle
+    g = np.random.default_rng(3)
+    h = 1.0 / W
+    f = g.standard_normal((H, W)) * flag
+    pb = np.zeros((H, W))
+    pb[0, :] = g.standard_normal(W)  # Dirichlet data on the (fluid) boundary
+    A, cells, idx = laplacian_matrix(flag, h)
+    rhs = np.array([-f[y, x] for (y, x) in cells])
+    for k, (y, x) in enumerate(cells):  # boundary coupling moves to the rhs
+        for dy, dx in ((0, -1), (0, 1), (-1, 0), (1, 0)):
+            if idx[y + dy, x + dx] < 0:
+                rhs[k] -= flag[y + dy, x + dx] * pb[y + dy, x + dx] / h / h
+    # solid interior cells have zero rows -> pin them to 0
+    A = A.tolil()
+    for k, (y, x) in enumerate(cells):
+        if not flag[y, x]:
+            A.rows[k] = [k]
+            A.data[k] = [1.0]
+            rhs[k] = 0.0
+    sol = spla.spsolve(A.tocsr(), rhs)
+    p = pb.copy()
+    for _ in range(3000):
+        O.rbgs(p, f, flag, h, 1.0)
+    got = np.array([p[y, x] for (y, x) in cells])
+    assert rel(got, sol) < 1e-9
+
+
+def test_rbgs_red_black_order():
+    """Red = (x+y) odd is updated first from old black values; black then sees
+    the new red values.  A single nonzero f on a black cell does not change
+    any red cell in the first sweep, and does change its black neighbour's
+    neighbours only through red cells."""
+    W, H = 9, 9
+    flag = np.ones((H, W))
+    f = np.zeros((H, W))
+    f[4, 4] = 1.0  # (x+y) = 8 even: black
+    p = np.zeros((H, W))
+    O.rbgs(p, f, flag, 1.0, 1.0)
+    red = (np.add.outer(np.arange(H), np.arange(W)) % 2) == 1
+    # black (4,4) = f h^2/4 = 0.25 (its red neighbours were zero when red ran)
+    assert p[4, 4] == pytest.approx(0.25)
+    assert np.all(p[red] == 0.0)
+    # with f on a red cell the red cell gets 0.25 and its black neighbours 0.25/4
+    f2 = np.zeros((H, W))
+    f2[4, 5] = 1.0
+    p2 = np.zeros((H, W))
+    O.rbgs(p2, f2, flag, 1.0, 1.0)
+    assert p2[4, 5] == pytest.approx(0.25)
+    assert p2[4, 4] == pytest.approx(0.0625) and p2[3, 5] == pytest.approx(0.0625)
+
+
+def test_rbgs_alpha_and_solid():
+    W, H = 10, 8
+    flag = rand_flag(W, H, 4, 0.3)
+    g = np.random.default_rng(5)
+    p = g.standard_normal((H, W))
+    f = g.standard_normal((H, W))
+    q1 = O.rbgs(p.copy(), f, flag, 0.3, 1.0)
+    q5 = O.rbgs(p.copy(), f, flag, 0.3, 0.5)
+    assert not np.allclose(q1, q5)
+    # solid interior cells are zeroed, boundary untouched
+    inner = flag[1:-1, 1:-1] == 0
+    assert np.all(q1[1:-1, 1:-1][inner] == 0.0)
+    assert np.array_equal(q1[0], p[0]) and np.array_equal(q1[:, -1], p[:, -1])
+
+
+# ---------------------------------------------------------------- residual
+def test_residual_matches_operator():
+    W, H = 19, 13
+    flag = rand_flag(W, H, 6)
+    g = np.random.default_rng(7)
+    p = g.standard_normal((H, W))
+    f = g.standard_normal((H, W))
+    h = 0.05
+    r, n = O.residual(p, f, flag, h)
+    A, cells, idx = laplacian_matrix(flag, h)
+    # r = F (f + A p) with A including boundary couplings
+    for (y, x) in cells:
+        s = 0.0
+        for dy, dx in ((0, -1), (0, 1), (-1, 0), (1, 0)):
+            s += flag[y + dy, x + dx] * (p[y + dy, x + dx] - p[y, x])
+        assert r[y, x] == pytest.approx(flag[y, x] * (f[y, x] + s / h / h), rel=1e-12, abs=1e-9)
+    assert n == pytest.approx(np.sqrt((r ** 2).sum()))
+    assert np.all(r[0] == 0) and np.all(r[:, 0] == 0)
+
+
+# ---------------------------------------------------------------- restrict
+def test_restrict_weights_and_constant():
+    W, H = 20, 16
+    Wc, Hc = 10, 8
+    r = np.zeros((H, W), np.float32)
+    r[6, 8] = 16.0  # fine (8, 6) = centre of coarse (4, 3)
+    r[7, 9] = 16.0  # fine (9, 7) = diagonal neighbour of coarse (4, 3) and (5, 4) ...
+    rc = O.restrict(r, Wc, Hc)
+    assert rc[3, 4] == pytest.approx(4.0 + 1.0)
+    assert rc[4, 5] == pytest.approx(1.0)
+    assert rc[3, 5] == pytest.approx(1.0)  # centre (10,6): (9,7) is a corner, (8,6) outside
+    assert rc[4, 4] == pytest.approx(1.0)  # centre (8,8): (9,7) is a corner
+    r2 = np.zeros((H, W), np.float32)
+    r2[6, 9] = 16.0  # fine (9,6): edge neighbour of coarse (4,3) and (5,3)
+    rc3 = O.restrict(r2, Wc, Hc)
+    assert rc3[3, 4] == pytest.approx(2.0) and rc3[3, 5] == pytest.approx(2.0)
+    assert rc3[2, 4] == 0.0 and rc3[4, 4] == 0.0
+    rc2 = O.restrict(np.full((H, W), 3.0, np.float32), Wc, Hc)
+    assert np.allclose(rc2[1:-1, 1:-1], 3.0) and np.all(rc2[0] == 0) and np.all(rc2[:, -1] == 0)
+
+
+# ---------------------------------------------------------------- prolongate
+def test_prolongate_constants():
+    """Constant coarse correction c, all-fluid flags: even-even c,
+    edge points 2c/2.0001, centre points 4c/4.0001 (pressure_solver.cpp:149)."""
+    W, H = 18, 14
+    Wc, Hc = 9, 7
+    c = 3.0
+    ec = np.full((Hc, Wc), c, np.float32)
+    fc = np.ones((Hc, Wc), np.float32)
+    fl = np.ones((H, W), np.float32)
+    e = O.prolongate(W, H, ec, fc, fl)
+    s2 = np.float32(2.0 + 0.0001)
+    s4 = np.float32(4.0 + 0.0001)
+    assert e[4, 6] == pytest.approx(c)
+    assert e[4, 5] == np.float32(np.float32(2 * c) / s2)
+    assert e[5, 6] == np.float32(np.float32(2 * c) / s2)
+    assert e[5, 5] == np.float32(np.float32(4 * c) / s4)
+    assert np.all(e[0] == 0) and np.all(e[:, 0] == 0) and np.all(e[-1] == 0)
+
+
+def test_prolongate_odd_width_last_column_unreached():
+    W, H = 17, 11  # odd: fine column W-2 = 15 is odd and not reached
+    Wc, Hc = 8, 5
+    ec = np.ones((Hc, Wc), np.float32)
+    e = O.prolongate(W, H, ec, np.ones((Hc, Wc), np.float32), np.ones((H, W), np.float32))
+    assert np.all(e[1:-1, W - 2] == 0.0)
+    assert np.all(e[1:-2, W - 3] != 0.0)
+
+
+def test_prolongate_flag_weighting():
+    W, H = 10, 10
+    ec = np.zeros((5, 5), np.float32)
+    ec[2, 2] = 1.0
+    fc = np.ones((5, 5), np.float32)
+    fc[2, 3] = 0.0  # coarse neighbour solid: the odd-x point between them divides by 1.0001
+    fl = np.ones((H, W), np.float32)
+    e = O.prolongate(W, H, ec, fc, fl)
+    assert e[4, 5] == np.float32(1.0 / np.float32(1.0001))
+    assert e[4, 3] == np.float32(1.0 / np.float32(2.0001))
+    fl[4, 4] = 0
+    e2 = O.prolongate(W, H, ec, fc, fl)
+    assert e2[4, 4] == 0.0
+
+
+def test_correct_and_zero_gradient():
+    g = np.random.default_rng(8)
+    p = g.standard_normal((6, 7)).astype(np.float32)
+    e = g.standard_normal((6, 7)).astype(np.float32)
+    q = O.correct(p.copy(), e)
+    assert np.array_equal(q[1:-1, 1:-1], (p + e)[1:-1, 1:-1]) and np.array_equal(q[0], p[0])
+    z = O.zero_gradient(p.copy())
+    assert np.array_equal(z[1:-1, 0], p[1:-1, 1]) and np.array_equal(z[1:-1, -1], p[1:-1, -2])
+    assert np.array_equal(z[0, 1:-1], p[1, 1:-1]) and np.array_equal(z[-1, 1:-1], p[-2, 1:-1])
+    assert z[0, 0] == p[0, 0] and z[-1, -1] == p[-1, -1]
+
+
+# ---------------------------------------------------------------- V-cycle
+def mgtest_problem(N):
+    h = 1.0 / (N - 1)
+    u = np.zeros((N, N), np.float32)
+    x = np.arange(N)
+    u[N - 1, 1:N - 1] = (np.sinh(np.pi) * np.sin(x[1:N - 1] / (N - 1.0) * np.pi)).astype(np.float32)
+    Y, X = np.meshgrid(np.arange(N), np.arange(N), indexing="ij")
+    ref = np.sinh(Y * h * np.pi) * np.sin(X * h * np.pi)
+    return u, ref, np.float32(h)
+
+
+def test_mgtest_kat_residual_monotone():
+    """mgtest/mgtest.cpp:10-42: harmonic Dirichlet problem; each V-cycle
+    reduces the residual."""
+    N = 257
+    u, ref, h = mgtest_problem(N)
+    rhs = np.zeros_like(u)
+    flag = np.ones_like(u)
+    mg = O.OracleMG(N, N)
+    mg.update_fields(flag)
+    _, prev = O.residual(u, rhs, flag, float(h))
+    for _ in range(8):
+        mg.solve(u, rhs, flag, float(h))
+        _, n = O.residual(u, rhs, flag, float(h))
+        assert n < prev
+        prev = n
+
+
+@pytest.mark.parametrize("N", [65, 129])
+def test_mgtest_kat_converges_to_analytic(N):
+    """fp64 V-cycles converge to the discrete solution, whose error against the
+    analytic harmonic function sinh(pi y) sin(pi x) is O(h^2) (mgtest.cpp:28-53)."""
+    u, ref, h = mgtest_problem(N)
+    u = u.astype(np.float64)
+    rhs = np.zeros_like(u)
+    flag = np.ones_like(u)
+    mg = O.OracleMG(N, N, fp64=True)
+    mg.update_fields(flag)
+    for _ in range(200):
+        mg.solve(u, rhs, flag, float(h))
+    err = np.abs(u - ref)[:, 1:-1].max()
+    # second-order: 5-point Laplacian error ~ h^2 * |u_xxxx + u_yyyy| / 12 * O(1)
+    assert err < 0.5 * float(h) ** 2 * np.pi ** 4 * np.sinh(np.pi) / 12 * 4
+    assert err > 0.01 * float(h) ** 2  # and not exactly zero (it is a discretisation error)
+
+
+def test_mgtest_error_order():
+    errs = []
+    for N in (33, 65):
+        u, ref, h = mgtest_problem(N)
+        u = u.astype(np.float64)
+        mg = O.OracleMG(N, N, fp64=True)
+        mg.update_fields(np.ones_like(u))
+        for _ in range(300):
+            mg.solve(u, np.zeros_like(u), np.ones_like(u), float(h))
+        errs.append(np.abs(u - ref).max())
+    assert 3.3 < errs[0] / errs[1] < 4.7  # halving h quarters the error
+
+
+def test_vcycle_fp64_converges_to_direct_solve():
+    """Neumann problem with walls and obstacles, compatible rhs: the V-cycle's
+    fixed point is the sparse direct solution (up to the constant)."""
+    W, H = 40, 24
+    flag = np.ones((H, W))
+    flag[0, :] = flag[-1, :] = 0
+    flag[:, 0] = flag[:, -1] = 0
+    flag[8:14, 15:20] = 0
+    g = np.random.default_rng(9)
+    f = g.standard_normal((H, W)) * flag
+    f -= flag * (f.sum() / flag.sum())
+    h = 1.0 / W
+    p = np.zeros((H, W))
+    mg = O.OracleMG(W, H, fp64=True)
+    mg.update_fields(flag)
+    norms = []
+    for _ in range(250):
+        mg.solve(p, f, flag, h, True)
+        _, n = O.residual(p, f, flag, h)
+        norms.append(n)
+    # the reference's V-cycle (Dirichlet-0 coarse boundaries under a Neumann
+    # fine problem) contracts by only ~0.87 per cycle here, but monotonically
+    assert all(b < a for a, b in zip(norms, norms[1:]))
+    assert norms[10] < 0.2 * norms[0]
+    assert norms[-1] < 1e-9 * norms[0]
+    A, cells, idx = laplacian_matrix(flag, h)
+    fluid = [k for k, (y, x) in enumerate(cells) if flag[y, x]]
+    Af = A[fluid][:, fluid]
+    got = np.array([p[cells[k]] for k in fluid])
+    r = Af @ got + np.array([f[cells[k]] for k in fluid])
+    assert np.abs(r).max() < 1e-6 * np.abs(f).max() * 1e3
+
+
+# ---------------------------------------------------------------- interpolators
+def test_cubic_hermite_nodes_and_polynomials():
+    L = O.lib()
+    for A, B, Cc, D in ((1.0, 2.0, -3.0, 5.0), (0.5, -1.5, 2.25, 7.0)):
+        assert L.oracle_cubic_hermite(0.0, A, B, Cc, D) == pytest.approx(B)
+        assert L.oracle_cubic_hermite(1.0, A, B, Cc, D) == pytest.approx(Cc)
+    # reproduces quadratics q(t) sampled at -1, 0, 1, 2
+    q = lambda t: 0.7 * t * t - 1.3 * t + 0.4  # noqa: E731
+    for t in (0.0, 0.25, 0.5, 0.9):
+        assert L.oracle_cubic_hermite(t, q(-1), q(0), q(1), q(2)) == pytest.approx(q(t), abs=1e-6)
+
+
+def test_bicubic_reproduces_quadratic_and_clamps():
+    gh, gw = 16, 20
+    Y, X = np.meshgrid(np.arange(gh), np.arange(gw), indexing="ij")
+    g = (0.01 * X * X - 0.02 * X * Y + 0.03 * Y * Y + 0.5 * X - 0.25 * Y + 1.0).astype(np.float32)
+    fn = lambda x, y: 0.01 * x * x - 0.02 * x * y + 0.03 * y * y + 0.5 * x - 0.25 * y + 1.0  # noqa
+    for cx, cy in ((5.3, 7.7), (3.0, 3.0), (16.9, 12.2), (9.5, 4.25)):
+        assert O.bicubic(g, cx, cy) == pytest.approx(fn(cx, cy), rel=1e-5, abs=1e-5)
+    # nodes are interpolated exactly
+    assert O.bicubic(g, 7.0, 9.0) == pytest.approx(float(g[9, 7]), rel=1e-6)
+    # clamp to [3, n-3] (interpolators.hpp:94-97)
+    assert O.bicubic(g, -10.0, 5.5) == pytest.approx(O.bicubic(g, 3.0, 5.5))
+    assert O.bicubic(g, 100.0, 5.5) == pytest.approx(O.bicubic(g, gw - 3.0, 5.5))
+    assert O.bicubic(g, 6.5, 1e9) == pytest.approx(O.bicubic(g, 6.5, gh - 3.0))
+
+
+def test_bilinear_exact_on_linear_and_clamp():
+    gh, gw = 10, 12
+    Y, X = np.meshgrid(np.arange(gh), np.arange(gw), indexing="ij")
+    g = (0.3 * X - 0.7 * Y + 2.0).astype(np.float32)
+    for cx, cy in ((0.0, 0.0), (4.25, 3.5), (9.9, 8.1)):
+        assert O.bilinear(g, cx, cy) == pytest.approx(0.3 * cx - 0.7 * cy + 2.0, abs=1e-5)
+    # clamp to [0, n-1.1] (interpolators.hpp:14-15)
+    assert O.bilinear(g, -5.0, 2.0) == pytest.approx(O.bilinear(g, 0.0, 2.0))
+    assert O.bilinear(g, 50.0, 2.0) == pytest.approx(0.3 * (gw - 1.1) - 0.7 * 2.0 + 2.0, abs=1e-5)
+
+
+def test_wang_hash_and_lcg():
+    """advect_tracer_points.cs:21-33: Thomas Wang's 32-bit integer hash and the
+    Numerical Recipes LCG (x' = 1664525 x + 1013904223 mod 2^32)."""
+    L = O.lib()
+    seq, s = [], 0
+    for _ in range(3):
+        s = L.oracle_lcg(s)
+        seq.append(s)
+    assert seq == [1013904223, 1196435762, 3519870697]
+
+    def wang(x):
+        x = ((x ^ 61) ^ (x >> 16)) & 0xFFFFFFFF
+        x = (x * 9) & 0xFFFFFFFF
+        x ^= x >> 4
+        x = (x * 0x27D4EB2D) & 0xFFFFFFFF
+        return x ^ (x >> 15)
+    for x in (0, 1, 7, 12345, 0xFFFFFFFF, 2 ** 31):
+        assert L.oracle_wang_hash(x) == wang(x)
+    # a bijection on a sample: no collisions among 65536 consecutive seeds
+    vals = {L.oracle_wang_hash(x) for x in range(1 << 16)}
+    assert len(vals) == 1 << 16
+
+
+# ---------------------------------------------------------------- simulation
+def make_sim(W, H, seed=0, **kw):
+    from ubootgl_amd import workloads
+    d = workloads.small_domain(W, H, seed, ndisks=2)
+    s = O.OracleSim(d.flag, float(d.pwidth), float(d.mu), **kw)
+    vx, vy = d.init_simulation_fields()
+    for fid, a in ((O.VX, vx), (O.VX_BACK, vx), (O.VY, vy), (O.VY_BACK, vy)):
+        s.set(fid, a)
+    return d, s
+
+
+def test_diffuse_mu_zero_is_masked_identity():
+    d, s = make_sim(40, 24, mu=None) if False else make_sim(40, 24)
+    s2 = O.OracleSim(d.flag, float(d.pwidth), 0.0)
+    vx, vy = d.init_simulation_fields()
+    for fid, a in ((O.VX, vx), (O.VX_BACK, vx), (O.VY, vy), (O.VY_BACK, vy)):
+        s2.set(fid, a)
+    s2.stage(O.ST_DIFFUSE, float(d.dt))
+    F = d.flag
+    out = s2.get(O.VX)
+    np.testing.assert_array_equal(out[1:-1, 1:-1], (vx * F[:, :-1] * F[:, 1:])[1:-1, 1:-1])
+
+
+def test_advect_zero_velocity_identity():
+    W, H = 48, 32
+    flag = np.ones((H, W), np.float32)
+    flag[0, :] = flag[-1, :] = flag[:, 0] = flag[:, -1] = 0
+    s = O.OracleSim(flag, (W - 1) / W, 0.001)
+    g = np.random.default_rng(1)
+    field = g.standard_normal((H, W - 1)).astype(np.float32)
+    zero_vy = np.zeros((H - 1, W), np.float32)
+    s.set(O.VX, field)
+    s.set(O.VY, zero_vy)
+    s.set(O.VY_BACK, zero_vy)
+    # vx alone moves samples along x; with vx = 0 everywhere the sample is the node
+    s.set(O.VX, np.zeros_like(field))
+    s.set(O.VX_BACK, np.full_like(field, 7.0))
+    s.stage(O.ST_ADVECT, 0.01)
+    out = s.get(O.VX)
+    # advected lanes 1..(8*nck) of fluid rows are 0; tail columns keep the back buffer (7)
+    nck = (W - 1 - 2) // 8
+    assert np.all(out[1:-1, 1:8 * nck + 1] == 0.0)
+    assert np.all(out[1:-1, 8 * nck + 1:W - 2] == 7.0)
+
+
+def test_advect_uniform_shift():
+    """Uniform vx = u with dt*u/h = 1 moves a field sampled at interior lanes by
+    exactly one cell (Catmull-Rom interpolates nodes exactly)."""
+    W, H = 64, 40
+    flag = np.ones((H, W), np.float32)
+    pw = (W - 1) / W
+    s = O.OracleSim(flag, pw, 0.0)
+    h = s.h
+    u = 0.5
+    dt = h / u
+    vx = np.full((H, W - 1), u, np.float32)
+    vy = np.zeros((H - 1, W), np.float32)
+    s.set(O.VX, vx)
+    s.set(O.VX_BACK, vx)
+    s.set(O.VY, vy)
+    s.set(O.VY_BACK, vy)
+    s.stage(O.ST_ADVECT, dt)
+    out = s.get(O.VX)
+    # constant field stays constant wherever the stencil is away from the clamps
+    assert np.allclose(out[5:-5, 8:40], u, atol=1e-6)
+    # vy of a uniform-x flow field: a y-independent linear vy profile shifts by -1 cell in x
+    vy2 = np.tile(np.arange(W, dtype=np.float32) * 0.01, (H - 1, 1))
+    s2 = O.OracleSim(flag, pw, 0.0)
+    for fid, a in ((O.VX, vx), (O.VX_BACK, vx), (O.VY, vy2), (O.VY_BACK, vy2)):
+        s2.set(fid, a)
+    s2.stage(O.ST_ADVECT, dt)
+    out2 = s2.get(O.VY)
+    # vy2 itself advects vy (its own y-velocity), so only check x-shift where vy is small
+    xs = np.arange(W, dtype=np.float32)
+    np.testing.assert_allclose(out2[10, 8:30], (xs[8:30] - 1) * 0.01, atol=2e-3)
+
+
+def test_advect_chunk_skip_rule():
+    """A vx chunk whose lanes never have flag(x-1)=flag(x)=1 is skipped and
+    keeps the back buffer even though its write mask flag(x)flag(x+1) is 1 at
+    the chunk's last lane (simulation.cpp:254-256 vs :292-295)."""
+    W, H = 40, 12
+    flag = np.ones((H, W), np.float32)
+    # row 5: make lanes x = 1..8 (chunk 0) fail the skip test: flag(x-1)*flag(x) = 0
+    flag[5, 0:9:2] = 0.0  # x = 0,2,4,6,8 solid -> pairs (x-1,x) never both fluid for x in 1..8
+    s = O.OracleSim(flag, (W - 1) / W, 0.001)
+    vx = np.ones((H, W - 1), np.float32)
+    back = np.full((H, W - 1), 5.0, np.float32)
+    s.set(O.VX, vx)
+    s.set(O.VX_BACK, back)
+    s.set(O.VY, np.zeros((H - 1, W), np.float32))
+    s.stage(O.ST_ADVECT, 1e-6)
+    out = s.get(O.VX)
+    assert np.all(out[5, 1:9] == 5.0)  # skipped chunk kept the back buffer
+    assert out[4, 3] != 5.0            # a normal row was advected
+
+
+def test_project_identity_divergence_equals_residual():
+    """After project, -div(v_new)/h... the new divergence at interior fluid
+    cells equals the MG residual r = F(f + L p) of the final pressure
+    (simulation.cpp:166-207, pressure_solver.cpp:91-116)."""
+    d, s = make_sim(64, 40, seed=3)
+    s.step(float(d.dt))
+    s.stage(O.ST_PROJECT, float(d.dt))
+    p = s.get(O.P)
+    f = s.get(O.F)
+    r, _ = O.residual(p, f, d.flag, float(s.h))
+    s.stage(O.ST_DIVERGENCE, float(d.dt))
+    fnew = s.get(O.F)
+    F = d.flag
+    inner = np.zeros_like(F, bool)
+    inner[2:-3, 2:-3] = True
+    inner &= F > 0
+    assert rel(fnew[inner], r[inner]) < 1e-4
+    assert np.linalg.norm(fnew[inner]) < 0.9 * np.linalg.norm(f[inner])
+
+
+def test_setvbcs_corner_order():
+    """setVBCs (simulation.cpp:80-102): rows after columns, so the SE corner of
+    vx is the no-slip mirror of the outflow value of row 1."""
+    W, H = 12, 9
+    flag = np.ones((H, W), np.float32)
+    s = O.OracleSim(flag, (W - 1) / W, 0.001)
+    g = np.random.default_rng(4)
+    vx = g.standard_normal((H, W - 1)).astype(np.float32)
+    vy = g.standard_normal((H - 1, W)).astype(np.float32)
+    for fid, a in ((O.VX, vx), (O.VX_BACK, vx), (O.VY, vy), (O.VY_BACK, vy)):
+        s.set(fid, a)
+    s.stage(O.ST_SET_VBCS, 0.0)
+    out = s.get(O.VX)
+    Wx = W - 1
+    assert out[3, 0] == vx[3, 0]                                   # W inflow keeps value
+    assert out[3, Wx - 1] == max(vx[3, Wx - 2], 0.0)               # E outflow
+    assert out[0, 4] == -vx[1, 4]                                  # S no-slip (per)
+    assert out[0, Wx - 1] == -max(vx[1, Wx - 2], 0.0)              # SE corner
+    outy = s.get(O.VY)
+    assert outy[3, 0] == vy[3, 0] and outy[0, 5] == 0.0 and outy[3, W - 1] == max(vy[3, W - 2], 0.0)
+    np.testing.assert_array_equal(s.get(O.VX_BACK)[0], out[0])
+
+
+def test_sinks_stamp_decay_erase():
+    d, s = make_sim(64, 48, seed=5)
+    s.add_sink(0.3, 0.2, 5.0)
+    s.add_sink(0.01, 0.01, 5.0)  # too close to the border: skipped, still decays? no: skipped
+    s.stage(O.ST_DIVERGENCE, float(d.dt))
+    f = s.get(O.F)
+    gx, gy = int(0.3 / s.h + 0.5), int(0.2 / s.h + 0.5)
+    assert np.all(f[gy - 1:gy + 2, gx - 1:gx + 2] == 5.0)
+    s.add_sink(0.4, 0.3, 0.06)
+    s.stage(O.ST_DIVERGENCE, 0.1)  # decay 1e-6^(5) -> below 0.05 -> erased
+    assert s.num_sinks() == 1  # the skipped border sink stays (it is never decayed)
+
+
+def test_tracer_rk2_constant_field_and_freeze():
+    W, H = 40, 30
+    flag = np.ones((H, W), np.float32)
+    flag[10:14, 10:14] = 0
+    pw = (W - 1) / W
+    s = O.OracleSim(flag, pw, 0.0)
+    u, v = 0.3, -0.2
+    s.set(O.VX, np.full((H, W - 1), u, np.float32))
+    s.set(O.VY, np.full((H - 1, W), v, np.float32))
+    xy = np.array([[0.1, 0.1], [0.6, 0.6], [(11.5) * pw / W, (11.5) * pw / W], [-0.1, 0.2]],
+                  np.float32)
+    age = np.array([0.0, 6.2, 0.0, 6.25], np.float32)
+    pts, a = xy.copy(), age.copy()
+    dt = 0.01
+    s.advect_tracers(pts, a, dt, 9)
+    assert pts[0] == pytest.approx([0.1 + u * dt, 0.1 + v * dt], abs=1e-6)
+    assert a[0] == pytest.approx(0.02)
+    assert a[1] == pytest.approx(6.22)          # 6.2 + 0.02, not above 2*3.141
+    np.testing.assert_array_equal(pts[2], xy[2])  # inside the obstacle: frozen
+    assert a[2] == pytest.approx(0.12)
+    # out of bounds and aged past 2*3.141 after freezing: respawned from the hash
+    assert a[3] == 0.0 and 0.0 <= pts[3][0] <= pw
+
+
+def test_step_deterministic_and_thread_independent():
+    d, s1 = make_sim(96, 48, seed=7)
+    O.set_threads(1)
+    for _ in range(2):
+        s1.step(float(d.dt))
+    a = s1.get(O.VX)
+    d, s2 = make_sim(96, 48, seed=7)
+    O.set_threads(4)
+    for _ in range(2):
+        s2.step(float(d.dt))
+    O.set_threads(0)
+    np.testing.assert_array_equal(a, s2.get(O.VX))

@@ -508,7 +508,11 @@ struct ubg_sim {
   int *d_sink_xy = nullptr;
   float *d_sink_val = nullptr;
   int sink_cap = 0;
+  // tracers are stored sorted by 16x16-cell tile of their position (gathers
+  // of one wave then touch few cache lines); tr_perm[slot] = tracer id.
   float *tr_xy = nullptr, *tr_age = nullptr;
+  int *tr_perm = nullptr;
+  std::vector<int> tr_perm_h;
   int ntr = 0, tr_cap = 0;
   uint32_t tr_seed = 1;
   Timer timer;
@@ -777,6 +781,7 @@ UBG_API int ubg_sim_destroy(ubg_sim *s) {
   if (s->d_sink_val) (void)hipFree(s->d_sink_val);
   if (s->tr_xy) (void)hipFree(s->tr_xy);
   if (s->tr_age) (void)hipFree(s->tr_age);
+  if (s->tr_perm) (void)hipFree(s->tr_perm);
   (void)hipStreamDestroy(s->stream);
   s->pool.release();
   delete s;
@@ -868,8 +873,8 @@ UBG_API int ubg_sim_step(ubg_sim *s, float dt) {
 UBG_API int ubg_sim_advect_tracers(ubg_sim *s, float dt, uint32_t seed) {
   if (!s) return fail(UBG_E_INVALID, "null sim");
   std::lock_guard<std::recursive_mutex> g(s->mu_);
-  launch_tracers(s->vel(), s->flag, s->h, s->pwidth, dt, seed, s->tr_xy, s->tr_age, s->ntr,
-                 s->stream);
+  launch_tracers(s->vel(), s->flag, s->h, s->pwidth, dt, seed, s->tr_xy, s->tr_age, s->tr_perm,
+                 s->ntr, s->stream);
   return check_launch();
 }
 
@@ -880,7 +885,7 @@ UBG_API int ubg_sim_run(ubg_sim *s, float dt, int nsteps) {
     UBG_TRY(s->step(dt));
     if (s->ntr > 0) {
       launch_tracers(s->vel(), s->flag, s->h, s->pwidth, dt, s->tr_seed++, s->tr_xy, s->tr_age,
-                     s->ntr, s->stream);
+                     s->tr_perm, s->ntr, s->stream);
     }
   }
   return check_launch();
@@ -985,21 +990,39 @@ UBG_API int ubg_sim_set_tracers(ubg_sim *s, const float *xy, const float *age, i
   if (n > s->tr_cap) {
     if (s->tr_xy) (void)hipFree(s->tr_xy);
     if (s->tr_age) (void)hipFree(s->tr_age);
+    if (s->tr_perm) (void)hipFree(s->tr_perm);
     s->tr_xy = s->tr_age = nullptr;
+    s->tr_perm = nullptr;
     s->tr_cap = 0;
     UBG_HIP(hipMalloc((void **)&s->tr_xy, sizeof(float) * 2 * (size_t)n));
     UBG_HIP(hipMalloc((void **)&s->tr_age, sizeof(float) * (size_t)n));
+    UBG_HIP(hipMalloc((void **)&s->tr_perm, sizeof(int) * (size_t)n));
     s->tr_cap = n;
   }
   s->ntr = n;
   if (n == 0) return UBG_OK;
-  UBG_HIP(hipMemcpyAsync(s->tr_xy, xy, sizeof(float) * 2 * (size_t)n, hipMemcpyHostToDevice, s->stream));
-  if (age) {
-    UBG_HIP(hipMemcpyAsync(s->tr_age, age, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, s->stream));
-  } else {
-    launch_fill_f32(s->tr_age, (size_t)n, 0.0f, s->stream);
+  // sort by 16x16 tile of the grid cell holding the point
+  const int tw = (s->W + 15) / 16;
+  std::vector<int64_t> key(n);
+  for (int i = 0; i < n; i++) {
+    float cx = xy[2 * i] / s->h, cy = xy[2 * i + 1] / s->h;
+    int ix = std::isfinite(cx) ? (int)std::min(std::max(cx, 0.0f), (float)(s->W - 1)) : 0;
+    int iy = std::isfinite(cy) ? (int)std::min(std::max(cy, 0.0f), (float)(s->H - 1)) : 0;
+    key[i] = ((int64_t)(iy >> 4) * tw + (ix >> 4)) * 256 + (iy & 15) * 16 + (ix & 15);
   }
-  UBG_TRY(check_launch());
+  std::vector<int> &perm = s->tr_perm_h;
+  perm.resize(n);
+  for (int i = 0; i < n; i++) perm[i] = i;
+  std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+  std::vector<float> sxy(2 * (size_t)n), sage((size_t)n, 0.0f);
+  for (int j = 0; j < n; j++) {
+    sxy[2 * (size_t)j] = xy[2 * (size_t)perm[j]];
+    sxy[2 * (size_t)j + 1] = xy[2 * (size_t)perm[j] + 1];
+    if (age) sage[j] = age[perm[j]];
+  }
+  UBG_HIP(hipMemcpyAsync(s->tr_xy, sxy.data(), sizeof(float) * 2 * (size_t)n, hipMemcpyHostToDevice, s->stream));
+  UBG_HIP(hipMemcpyAsync(s->tr_age, sage.data(), sizeof(float) * (size_t)n, hipMemcpyHostToDevice, s->stream));
+  UBG_HIP(hipMemcpyAsync(s->tr_perm, perm.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, s->stream));
   UBG_HIP(hipStreamSynchronize(s->stream));
   return UBG_OK;
 }
@@ -1008,9 +1031,21 @@ UBG_API int ubg_sim_get_tracers(ubg_sim *s, float *xy, float *age, int n) {
   if (!s || n < 0 || n > s->ntr) return fail(UBG_E_INVALID, "bad argument");
   std::lock_guard<std::recursive_mutex> g(s->mu_);
   if (n == 0) return UBG_OK;
-  if (xy) UBG_HIP(hipMemcpyAsync(xy, s->tr_xy, sizeof(float) * 2 * (size_t)n, hipMemcpyDeviceToHost, s->stream));
-  if (age) UBG_HIP(hipMemcpyAsync(age, s->tr_age, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, s->stream));
+  const int N = s->ntr;
+  std::vector<float> sxy(2 * (size_t)N), sage((size_t)N);
+  UBG_HIP(hipMemcpyAsync(sxy.data(), s->tr_xy, sizeof(float) * 2 * (size_t)N, hipMemcpyDeviceToHost, s->stream));
+  UBG_HIP(hipMemcpyAsync(sage.data(), s->tr_age, sizeof(float) * (size_t)N, hipMemcpyDeviceToHost, s->stream));
   UBG_HIP(hipStreamSynchronize(s->stream));
+  const std::vector<int> &perm = s->tr_perm_h;
+  for (int j = 0; j < N; j++) {
+    int i = perm[j];
+    if (i >= n) continue;
+    if (xy) {
+      xy[2 * (size_t)i] = sxy[2 * (size_t)j];
+      xy[2 * (size_t)i + 1] = sxy[2 * (size_t)j + 1];
+    }
+    if (age) age[i] = sage[j];
+  }
   return UBG_OK;
 }
 

@@ -26,36 +26,42 @@ __device__ __forceinline__ float vbc_per(int bc, float a, float b) {
 
 // setVBCs (simulation.cpp:80-102).  The reference runs four sequential loops:
 // vx W/E columns, vx S/N rows, vy W/E columns, vy S/N rows.  Only the row
-// loops read values the column loops wrote (the corners' neighbours), so one
-// workgroup runs both column loops, a barrier, then both row loops, which
+// loops read values the column loops wrote (the corners' neighbours), so the
+// two column loops run as one launch and the two row loops as the next, which
 // reproduces the sequential result exactly.
-__global__ void __launch_bounds__(1024) k_set_vbcs(float *__restrict__ xf, float *__restrict__ xb,
-                                                    float *__restrict__ yf, float *__restrict__ yb,
-                                                    int W, int H, int bw, int be, int bn, int bs) {
+__global__ void k_vbcs_cols(float *__restrict__ xf, float *__restrict__ xb,
+                            float *__restrict__ yf, float *__restrict__ yb, int W, int H, int bw,
+                            int be) {
   const int Wx = W - 1, Hy = H - 1;
-  for (int y = threadIdx.x; y < H; y += blockDim.x) {
+  int y = blockIdx.x * blockDim.x + threadIdx.x;
+  if (y < H) {
     size_t r0 = (size_t)y * Wx;
     float w0 = vbc_par(bw, xf[r0 + 1], xf[r0]);
     float e0 = vbc_par(be, xf[r0 + Wx - 2], xf[r0 + Wx - 1]);
     xf[r0] = xb[r0] = w0;
     xf[r0 + Wx - 1] = xb[r0 + Wx - 1] = e0;
   }
-  for (int y = threadIdx.x; y < Hy; y += blockDim.x) {
+  if (y < Hy) {
     size_t r0 = (size_t)y * W;
     float w0 = vbc_per(bw, yf[r0 + 1], yf[r0]);
     float e0 = vbc_per(be, yf[r0 + W - 2], yf[r0 + W - 1]);
     yf[r0] = yb[r0] = w0;
     yf[r0 + W - 1] = yb[r0 + W - 1] = e0;
   }
-  __syncthreads();
-  for (int x = threadIdx.x; x < Wx; x += blockDim.x) {
+}
+__global__ void k_vbcs_rows(float *__restrict__ xf, float *__restrict__ xb,
+                            float *__restrict__ yf, float *__restrict__ yb, int W, int H, int bn,
+                            int bs) {
+  const int Wx = W - 1, Hy = H - 1;
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < Wx) {
     size_t top = (size_t)(H - 1) * Wx + x, top1 = (size_t)(H - 2) * Wx + x;
     float s0 = vbc_per(bs, xf[(size_t)Wx + x], xf[x]);
     float n0 = vbc_per(bn, xf[top1], xf[top]);
     xf[x] = xb[x] = s0;
     xf[top] = xb[top] = n0;
   }
-  for (int x = threadIdx.x; x < W; x += blockDim.x) {
+  if (x < W) {
     size_t top = (size_t)(Hy - 1) * W + x, top1 = (size_t)(Hy - 2) * W + x;
     float s0 = vbc_par(bs, yf[(size_t)W + x], yf[x]);
     float n0 = vbc_par(bn, yf[top1], yf[top]);
@@ -69,22 +75,22 @@ __global__ void __launch_bounds__(1024) k_set_vbcs(float *__restrict__ xf, float
 __device__ __forceinline__ float pbc(int bc, float a) {
   return bc == BC_OUTFLOW_ZERO_PRESSURE ? -a : a;
 }
-__global__ void __launch_bounds__(1024) k_set_pbc(float *__restrict__ p, int W, int H, int bw,
-                                                   int be, int bn, int bs) {
-  for (int y = threadIdx.x; y < H; y += blockDim.x) {
-    size_t r0 = (size_t)y * W;
-    float w0 = pbc(bw, p[r0 + 1]);
-    float e0 = pbc(be, p[r0 + W - 2]);
-    p[r0] = w0;
-    p[r0 + W - 1] = e0;
-  }
-  __syncthreads();
-  for (int x = threadIdx.x; x < W; x += blockDim.x) {
-    float s0 = pbc(bs, p[(size_t)W + x]);
-    float n0 = pbc(bn, p[(size_t)(H - 2) * W + x]);
-    p[x] = s0;
-    p[(size_t)(H - 1) * W + x] = n0;
-  }
+__global__ void k_pbc_cols(float *__restrict__ p, int W, int H, int bw, int be) {
+  int y = blockIdx.x * blockDim.x + threadIdx.x;
+  if (y >= H) return;
+  size_t r0 = (size_t)y * W;
+  float w0 = pbc(bw, p[r0 + 1]);
+  float e0 = pbc(be, p[r0 + W - 2]);
+  p[r0] = w0;
+  p[r0 + W - 1] = e0;
+}
+__global__ void k_pbc_rows(float *__restrict__ p, int W, int H, int bn, int bs) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= W) return;
+  float s0 = pbc(bs, p[(size_t)W + x]);
+  float n0 = pbc(bn, p[(size_t)(H - 2) * W + x]);
+  p[x] = s0;
+  p[(size_t)(H - 1) * W + x] = n0;
 }
 
 // applyAccumulatedVelocity (simulation.cpp:376-396), interior faces only.
@@ -332,16 +338,17 @@ __device__ __forceinline__ float flag_linear(const uint8_t *__restrict__ F, int 
 __global__ void k_tracers(const float *__restrict__ xf, const float *__restrict__ yf,
                           const uint8_t *__restrict__ F, int W, int H, float h, float pwidth,
                           float dt, uint32_t seed, float2 *__restrict__ xy,
-                          float *__restrict__ age, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+                          float *__restrict__ age, const int *__restrict__ perm, int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x; // storage slot
+  if (j >= n) return;
+  const int i = perm ? perm[j] : j; // tracer id (seeds the respawn hash)
   const float pdx = pwidth, pdy = pwidth * H / W;
-  float2 P = xy[i];
+  float2 P = xy[j];
   float2 v1 = bilinear_vel(xf, yf, W, H, P.x / h, P.y / h);
   float mx = P.x + v1.x * dt * 0.5f, my = P.y + v1.y * dt * 0.5f;
   float2 v2 = bilinear_vel(xf, yf, W, H, mx / h, my / h);
   float nx = P.x + v2.x * dt, ny = P.y + v2.y * dt;
-  float a = age[i];
+  float a = age[j];
   if (P.x < 0.0f || P.y < 0.0f || P.x > pdx || P.y > pdy ||
       flag_linear(F, W, H, pwidth, P.x, P.y) < 0.6f) {
     nx = P.x;
@@ -358,20 +365,22 @@ __global__ void k_tracers(const float *__restrict__ xf, const float *__restrict_
   } else {
     a = a + 0.02f;
   }
-  xy[i] = make_float2(nx, ny);
-  age[i] = a;
+  xy[j] = make_float2(nx, ny);
+  age[j] = a;
 }
 
 } // namespace
 
 // ---------------------------------------------------------------------------
 void launch_set_vbcs(const VelView &v, const BCs &bc, hipStream_t s) {
-  hipLaunchKernelGGL(k_set_vbcs, dim3(1), dim3(1024), 0, s, v.vxf, v.vxb, v.vyf, v.vyb, v.W, v.H,
-                     bc.west, bc.east, bc.north, bc.south);
+  hipLaunchKernelGGL(k_vbcs_cols, dim3(ceil_div(v.H, TPB)), dim3(TPB), 0, s, v.vxf, v.vxb, v.vyf,
+                     v.vyb, v.W, v.H, bc.west, bc.east);
+  hipLaunchKernelGGL(k_vbcs_rows, dim3(ceil_div(v.W, TPB)), dim3(TPB), 0, s, v.vxf, v.vxb, v.vyf,
+                     v.vyb, v.W, v.H, bc.north, bc.south);
 }
 void launch_set_pbc(float *p, int W, int H, const BCs &bc, hipStream_t s) {
-  hipLaunchKernelGGL(k_set_pbc, dim3(1), dim3(1024), 0, s, p, W, H, bc.west, bc.east, bc.north,
-                     bc.south);
+  hipLaunchKernelGGL(k_pbc_cols, dim3(ceil_div(H, TPB)), dim3(TPB), 0, s, p, W, H, bc.west, bc.east);
+  hipLaunchKernelGGL(k_pbc_rows, dim3(ceil_div(W, TPB)), dim3(TPB), 0, s, p, W, H, bc.north, bc.south);
 }
 void launch_accumulate(const VelView &v, float *xa, float *ya, hipStream_t s) {
   hipLaunchKernelGGL(k_accumulate, dim3(ceil_div(v.W - 2, TPB), v.H - 2), dim3(TPB), 0, s, v.vxf,
@@ -413,10 +422,10 @@ void launch_bilinear_vel(const VelView &v, const float *c, float *out, int n, hi
                      v.H, c, out, n);
 }
 void launch_tracers(const VelView &v, const uint8_t *flag, float h, float pwidth, float dt,
-                    uint32_t seed, float *xy, float *age, int n, hipStream_t s) {
+                    uint32_t seed, float *xy, float *age, const int *perm, int n, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_tracers, dim3(ceil_div(n, TPB)), dim3(TPB), 0, s, v.vxf, v.vyf, flag, v.W,
-                     v.H, h, pwidth, dt, seed, (float2 *)xy, age, n);
+                     v.H, h, pwidth, dt, seed, (float2 *)xy, age, perm, n);
 }
 
 } // namespace ubg

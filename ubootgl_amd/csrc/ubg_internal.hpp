@@ -81,6 +81,7 @@ void launch_correct_velocity(const VelView &v, const float *p, const uint8_t *fl
                              hipStream_t s);
 void launch_bilinear_vel(const VelView &v, const float *c, float *out, int n, hipStream_t s);
 void launch_tracers(const VelView &v, const uint8_t *flag, float h, float pwidth, float dt,
-                    uint32_t seed, float *xy, float *age, int n, hipStream_t s);
+                    uint32_t seed, float *xy, float *age, const int *perm, int n,
+                    hipStream_t s);
 
 } // namespace ubg

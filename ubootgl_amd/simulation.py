@@ -53,6 +53,7 @@ class MG:
 
     def __init__(self, width_or_flag, height=None, _handle=None, _owner=None):
         self._owner = _owner
+        self.sizes = None
         if _handle is not None:
             self._h = _handle
             self._borrowed = True
@@ -66,14 +67,12 @@ class MG:
             h = C.c_void_p()
             check(lib().ubg_mg_create(W, H, C.byref(h)))
             self._h = h
+            self.set_sizes(W, H)
             if flag is not None:
                 self.updateFields(flag)
         n = C.c_int()
         check(lib().ubg_mg_num_levels(self._h, C.byref(n)))
         self.levels = n.value
-        self.sizes = None
-        if _handle is None:
-            self.set_sizes(W, H)
 
     def _level_shape(self, level):
         if self.sizes is None:
