@@ -170,8 +170,14 @@ UBG_API int ubg_sim_run(ubg_sim *sim, float dt, int nsteps);
 UBG_API int ubg_sim_stage(ubg_sim *sim, int stage, float dt);
 /* setGrids(c, val) (simulation.hpp:82-98) */
 UBG_API int ubg_sim_set_grids(ubg_sim *sim, int x, int y, float val);
+/* boundary conditions bcWest/bcEast/bcNorth/bcSouth (simulation.hpp:124),
+ * values of Simulation::BC: 0 INFLOW, 1 OUTFLOW, 2 OUTFLOW_ZERO_PRESSURE,
+ * 3 NOSLIP (simulation.hpp:69) */
+UBG_API int ubg_sim_set_bcs(ubg_sim *sim, int west, int east, int north, int south);
+UBG_API int ubg_sim_get_bcs(const ubg_sim *sim, int *west, int *east, int *north, int *south);
 /* sinks (simulation.hpp:140, consumed by project simulation.cpp:173-187) */
 UBG_API int ubg_sim_add_sink(ubg_sim *sim, float x, float y, float z);
+UBG_API int ubg_sim_set_sinks(ubg_sim *sim, const float *xyz, int n);
 UBG_API int ubg_sim_get_sinks(const ubg_sim *sim, float *xyz, int cap, int *count);
 /* the simulation's MG (Simulation::mg, simulation.hpp:133); borrowed */
 UBG_API int ubg_sim_get_mg(ubg_sim *sim, ubg_mg **mg);

@@ -89,7 +89,10 @@ SIGNATURES = {
     "ubg_sim_run": (C.c_int, [_vp, C.c_float, C.c_int]),
     "ubg_sim_stage": (C.c_int, [_vp, C.c_int, C.c_float]),
     "ubg_sim_set_grids": (C.c_int, [_vp, C.c_int, C.c_int, C.c_float]),
+    "ubg_sim_set_bcs": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "ubg_sim_get_bcs": (C.c_int, [_vp, _ip, _ip, _ip, _ip]),
     "ubg_sim_add_sink": (C.c_int, [_vp, C.c_float, C.c_float, C.c_float]),
+    "ubg_sim_set_sinks": (C.c_int, [_vp, _fp, C.c_int]),
     "ubg_sim_get_sinks": (C.c_int, [_vp, _fp, C.c_int, _ip]),
     "ubg_sim_get_mg": (C.c_int, [_vp, C.POINTER(_vp)]),
     "ubg_sim_bilinear_vel": (C.c_int, [_vp, _fp, _fp, C.c_int]),

@@ -943,6 +943,38 @@ UBG_API int ubg_sim_set_grids(ubg_sim *s, int x, int y, float val) {
   return UBG_OK;
 }
 
+UBG_API int ubg_sim_set_bcs(ubg_sim *s, int west, int east, int north, int south) {
+  if (!s) return fail(UBG_E_INVALID, "null sim");
+  for (int b : {west, east, north, south})
+    if (b < BC_INFLOW || b > BC_NOSLIP) return fail(UBG_E_INVALID, "unknown boundary condition");
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  if (s->bc.west == west && s->bc.east == east && s->bc.north == north && s->bc.south == south)
+    return UBG_OK;
+  s->bc.west = west;
+  s->bc.east = east;
+  s->bc.north = north;
+  s->bc.south = south;
+  s->invalidate_graphs();
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_get_bcs(const ubg_sim *s, int *west, int *east, int *north, int *south) {
+  if (!s) return fail(UBG_E_INVALID, "null sim");
+  if (west) *west = s->bc.west;
+  if (east) *east = s->bc.east;
+  if (north) *north = s->bc.north;
+  if (south) *south = s->bc.south;
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_set_sinks(ubg_sim *s, const float *xyz, int n) {
+  if (!s || n < 0 || (n > 0 && !xyz)) return fail(UBG_E_INVALID, "bad argument");
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->sinks.clear();
+  for (int i = 0; i < n; i++) s->sinks.push_back({xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]});
+  return UBG_OK;
+}
+
 UBG_API int ubg_sim_add_sink(ubg_sim *s, float x, float y, float z) {
   if (!s) return fail(UBG_E_INVALID, "null sim");
   std::lock_guard<std::recursive_mutex> g(s->mu_);
