@@ -1,8 +1,10 @@
 """GPU parity: libubg.so against the CPU oracle on identical seeded inputs.
 
-Integer/classification outputs must match bit-exactly; floating-point fields
-within 1e-5 relative L2 (the tolerance of the parity contract), stage by stage
-and for whole steps.
+The parity contract is: flags and classification bit-exact, velocity and
+pressure within 1e-5 relative L2.  Both sides are built without FMA
+contraction and perform the same IEEE operations in the same order, so the
+fields are in fact bit-identical; these tests assert that (exact equality),
+which implies the 1e-5 bound.
 """
 import numpy as np
 import pytest
@@ -61,7 +63,7 @@ def test_rbgs_sweep(gpu, W, H):
     ref = p.copy()
     for _ in range(3):
         O.rbgs(ref, f, flag, float(h), 1.0)
-    assert rel(got, ref) < 1e-6
+    np.testing.assert_array_equal(got, ref)
 
 
 def test_rbgs_alpha(gpu):
@@ -75,7 +77,7 @@ def test_rbgs_alpha(gpu):
     mg.set_level(0, L.LEVEL_F, f)
     mg.level_op(L.MGOP_RBGS, 0, 0.02, 0.7)
     ref = O.rbgs(p.copy(), f, flag, 0.02, 0.7)
-    assert rel(mg.get_level(0, L.LEVEL_P), ref) < 1e-6
+    np.testing.assert_array_equal(mg.get_level(0, L.LEVEL_P), ref)
 
 
 @pytest.mark.parametrize("W,H", [(33, 17), (128, 64)])
@@ -93,13 +95,13 @@ def test_residual_restrict_prolong(gpu, W, H):
     # residual field
     r_gpu, n_gpu = mg.residual(h)
     r_ref, n_ref = O.residual(p, f, flag, h)
-    assert rel(r_gpu, r_ref) < 1e-6
-    assert abs(n_gpu - n_ref) <= 1e-5 * n_ref
+    np.testing.assert_array_equal(r_gpu, r_ref)
+    assert abs(n_gpu - n_ref) <= 1e-12 * n_ref
     # residual + restrict
     mg.level_op(L.MGOP_RESTRICT_RESIDUAL, 0, h)
     Wc, Hc = mg.sizes[1]
     rc_ref = O.restrict(r_ref, Wc, Hc)
-    assert rel(mg.get_level(1, L.LEVEL_F), rc_ref) < 1e-6
+    np.testing.assert_array_equal(mg.get_level(1, L.LEVEL_F), rc_ref)
     assert not mg.get_level(1, L.LEVEL_P).any()
     # prolongate + correct with a random coarse correction
     ec = g.standard_normal((Hc, Wc)).astype(np.float32)
@@ -109,7 +111,7 @@ def test_residual_restrict_prolong(gpu, W, H):
     mg.level_op(L.MGOP_PROLONG_CORRECT, 0, h)
     e = O.prolongate(W, H, ec, om.flag_level(1), flag)
     ref = O.correct(p.copy(), e)
-    assert rel(mg.get_level(0, L.LEVEL_P), ref) < 1e-6
+    np.testing.assert_array_equal(mg.get_level(0, L.LEVEL_P), ref)
 
 
 def test_zero_gradient(gpu):
@@ -138,6 +140,7 @@ def test_vcycle(gpu, W, H, zg):
         mg.solve(pg, f, flag, float(h), zg)
         om.solve(pr, f, flag, float(h), zg)
     assert rel(pg, pr) < TOL
+    np.testing.assert_array_equal(pg, pr)
 
 
 def test_mgtest_kat_gpu(gpu):
@@ -155,7 +158,7 @@ def test_mgtest_kat_gpu(gpu):
     for _ in range(5):
         mg.solve(ug, rhs, flag, float(h))
         om.solve(ur, rhs, flag, float(h))
-    assert rel(ug, ur) < TOL
+    np.testing.assert_array_equal(ug, ur)
 
 
 def make_pair(d, params=None, **oparams):
@@ -185,6 +188,7 @@ def compare(sim, ref, fields=FIELDS, tol=TOL):
         r = rel(a, b)
         t = F_TOL if name == "f" else tol
         assert r < t, f"{name}: rel L2 {r:.3e}"
+        np.testing.assert_array_equal(a, b, err_msg=f"{name} not bit-identical")
 
 
 @pytest.mark.parametrize("stage", [L.STAGE_DIFFUSE, L.STAGE_ADVECT, L.STAGE_SET_VBCS,
@@ -263,7 +267,7 @@ def test_bilinear_vel(gpu):
     ref.step(float(d.dt))
     g = np.random.default_rng(16)
     c = np.stack([g.uniform(-2, 66, 500), g.uniform(-2, 42, 500)], 1).astype(np.float32)
-    assert rel(sim.bilinearVel(c), ref.bilinear_vel(c)) < 1e-6
+    np.testing.assert_array_equal(sim.bilinearVel(c), ref.bilinear_vel(c))
 
 
 def test_tracers(gpu):
@@ -279,8 +283,8 @@ def test_tracers(gpu):
         sim.advect_tracers(float(d.dt), 100 + k)
         ref.advect_tracers(rxy, rage, float(d.dt), 100 + k)
     gxy, gage = sim.get_tracers()
-    assert rel(gxy, rxy) < TOL
-    np.testing.assert_allclose(gage, rage, rtol=0, atol=1e-6)
+    np.testing.assert_array_equal(gxy, rxy)
+    np.testing.assert_array_equal(gage, rage)
 
 
 def test_error_codes(gpu):

@@ -144,7 +144,7 @@ struct ubg_mg {
 
   // smoother sweep at level l (zero-gradient only on level 0 if requested)
   void sweep(int l, float h, bool zg) {
-    const Level &L = lv[l];
+    Level &L = lv[l];
     bool timed = timer && timer->on && l == 0;
     if (timed) {
       if (timer->used + 2 > timer->ev.size()) {
@@ -174,7 +174,7 @@ struct ubg_mg {
       return;
     }
     for (int i = 0; i < prm.nu_pre; i++) sweep(l, h, z);
-    const Level &L = lv[l], &C = lv[l + 1];
+    Level &L = lv[l], &C = lv[l + 1];
     launch_residual_restrict(L, C, h, stream);
     float hc = h * (L.w - 1.0f) / (C.w - 1.0f); // pressure_solver.cpp:229
     solve_level(l + 1, hc, false);
@@ -197,8 +197,8 @@ namespace {
 
 // Build an MG hierarchy.  If level-0 buffers are given they are borrowed
 // (the simulation's p, f, flag, mask); otherwise the MG owns them.
-int mg_create(int W, int H, hipStream_t stream, float *p0, float *f0, uint8_t *flag0,
-              uint8_t *mask0, ubg_mg **out) {
+int mg_create(int W, int H, hipStream_t stream, float *p0, float *p0_alt, float *f0,
+              uint8_t *flag0, uint8_t *mask0, ubg_mg **out) {
   if (!out) return fail(UBG_E_INVALID, "null output pointer");
   if (W < 8 || H < 8)
     return fail(UBG_E_DOMAIN_TOO_SMALL, "MG needs at least 8 cells in each dimension");
@@ -214,7 +214,7 @@ int mg_create(int W, int H, hipStream_t stream, float *p0, float *f0, uint8_t *f
     size_t n = (size_t)ws[l] * hs[l];
     bool own = (l > 0) || !p0;
     bytes += align_up(n * 4);              // r
-    if (own) bytes += 2 * align_up(n * 4) + 2 * align_up(n); // p, f, flag, mask
+    if (own) bytes += 3 * align_up(n * 4) + 2 * align_up(n); // p, p_alt, f, flag, mask
   }
   int rc = m->pool.init(bytes);
   if (rc) {
@@ -231,11 +231,13 @@ int mg_create(int W, int H, hipStream_t stream, float *p0, float *f0, uint8_t *f
     L.r = m->pool.take<float>(n);
     if (l == 0 && p0) {
       L.p = p0;
+      L.p_alt = p0_alt;
       L.f = f0;
       L.flag = flag0;
       L.mask = mask0;
     } else {
       L.p = m->pool.take<float>(n);
+      L.p_alt = m->pool.take<float>(n);
       L.f = m->pool.take<float>(n);
       L.flag = m->pool.take<uint8_t>(n);
       L.mask = m->pool.take<uint8_t>(n);
@@ -307,7 +309,7 @@ UBG_API int ubg_mg_create(int width, int height, ubg_mg **out) {
   int n = 0;
   ubg_device_count(&n);
   if (n <= 0) return fail(UBG_E_NO_DEVICE, "no HIP device visible");
-  return mg_create(width, height, nullptr, nullptr, nullptr, nullptr, nullptr, out);
+  return mg_create(width, height, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, out);
 }
 
 UBG_API int ubg_mg_destroy(ubg_mg *m) {
@@ -392,7 +394,7 @@ UBG_API int ubg_mg_level_op(ubg_mg *m, int op, int level, float h, float alpha) 
   const Level &L = m->lv[level];
   switch (op) {
   case UBG_MGOP_RBGS:
-    launch_rbgs_fused(L, h, alpha, false, m->stream);
+    launch_rbgs_fused(m->lv[level], h, alpha, false, m->stream);
     break;
   case UBG_MGOP_ZERO_GRADIENT:
     launch_zero_gradient(L.p, L.w, L.h, m->stream);
@@ -497,7 +499,9 @@ struct ubg_sim {
   float *vx[2] = {nullptr, nullptr}, *vy[2] = {nullptr, nullptr};
   int fx = 0, fy = 0;
   float *vx_acc = nullptr, *vy_acc = nullptr, *vx_cur = nullptr, *vy_cur = nullptr;
-  float *p = nullptr, *f = nullptr, *r = nullptr;
+  float *p_buf[2] = {nullptr, nullptr}; // pressure ping-pong pair, owned; the
+                                        // current one is mg->lv[0].p
+  float *f = nullptr, *r = nullptr;
   uint8_t *flag = nullptr, *mask = nullptr;
   bool accum_dirty = false;
   ubg_mg *mg = nullptr;
@@ -516,14 +520,40 @@ struct ubg_sim {
   int ntr = 0, tr_cap = 0;
   uint32_t tr_seed = 1;
   Timer timer;
-  // captured step graphs, one per front-buffer parity
+  // captured step graphs keyed by the buffer-role state they start from
+  struct State {
+    int fx = 0, fy = 0;
+    std::vector<float *> lp; // p / p_alt of every MG level
+    bool operator==(const State &o) const { return fx == o.fx && fy == o.fy && lp == o.lp; }
+  };
   struct G {
-    bool valid = false;
     hipGraphExec_t exec = nullptr;
     float dt = 0;
-    int fx = 0, fy = 0;
-  } graphs[2];
+    State before, after;
+  };
+  std::vector<G> graphs;
   std::recursive_mutex mu_;
+
+  float *P() const { return mg->lv[0].p; }
+
+  State state() const {
+    State st;
+    st.fx = fx;
+    st.fy = fy;
+    for (auto &L : mg->lv) {
+      st.lp.push_back(L.p);
+      st.lp.push_back(L.p_alt);
+    }
+    return st;
+  }
+  void set_state(const State &st) {
+    fx = st.fx;
+    fy = st.fy;
+    for (size_t l = 0; l < mg->lv.size(); l++) {
+      mg->lv[l].p = st.lp[2 * l];
+      mg->lv[l].p_alt = st.lp[2 * l + 1];
+    }
+  }
 
   VelView vel() const { return VelView{W, H, vx[fx], vx[1 - fx], vy[fy], vy[1 - fy]}; }
 
@@ -587,7 +617,7 @@ struct ubg_sim {
 
   void divergence() { launch_divergence(vel(), f, 1.0f / h, stream); }
 
-  void correct_velocity() { launch_correct_velocity(vel(), p, flag, 1.0f / h, stream); }
+  void correct_velocity() { launch_correct_velocity(vel(), P(), flag, 1.0f / h, stream); }
 
   void mg_solve() { mg->solve_level(0, h, true); }
 
@@ -596,7 +626,7 @@ struct ubg_sim {
     divergence();
     UBG_TRY(apply_sinks());
     for (int c = 0; c < prm.vcycles_per_step; c++) mg_solve();
-    launch_set_pbc(p, W, H, bc, stream);
+    launch_set_pbc(P(), W, H, bc, stream);
     correct_velocity();
     return UBG_OK;
   }
@@ -613,7 +643,7 @@ struct ubg_sim {
     set_vbcs();
     divergence();
     for (int c = 0; c < prm.vcycles_per_step; c++) mg_solve();
-    launch_set_pbc(p, W, H, bc, stream);
+    launch_set_pbc(P(), W, H, bc, stream);
     correct_velocity();
     set_vbcs();
     if (prm.save_current) save_current();
@@ -628,32 +658,30 @@ struct ubg_sim {
     }
     bool graphable = prm.use_graph && sinks.empty() && !timer.on;
     if (graphable) {
-      int slot = fx; // fx and fy always flip together over a step
-      G &g = graphs[slot];
-      if (!g.valid || g.dt != dt || g.fx != fx || g.fy != fy) {
-        if (g.exec) (void)hipGraphExecDestroy(g.exec);
-        g.exec = nullptr;
-        g.valid = false;
-        int fx0 = fx, fy0 = fy;
+      State st = state();
+      G *g = nullptr;
+      for (auto &cand : graphs)
+        if (cand.dt == dt && cand.before == st) g = &cand;
+      if (!g) {
+        if (graphs.size() >= 8) invalidate_graphs();
+        G ng;
+        ng.dt = dt;
+        ng.before = st;
         hipGraph_t graph;
         UBG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         step_body();
         hipError_t e = hipStreamEndCapture(stream, &graph);
+        ng.after = state();
+        set_state(st); // capture executed nothing: restore the buffer roles
         if (e != hipSuccess) return fail(UBG_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-        e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        e = hipGraphInstantiate(&ng.exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         if (e != hipSuccess) return fail(UBG_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
-        g.valid = true;
-        g.dt = dt;
-        g.fx = fx0;
-        g.fy = fy0;
-        // capture did not execute anything: restore the buffer roles
-        fx = fx0;
-        fy = fy0;
+        graphs.push_back(ng);
+        g = &graphs.back();
       }
-      UBG_HIP(hipGraphLaunch(g.exec, stream));
-      fx = 1 - fx;
-      fy = 1 - fy;
+      UBG_HIP(hipGraphLaunch(g->exec, stream));
+      set_state(g->after);
     } else {
       diffuse();
       advect();
@@ -666,11 +694,9 @@ struct ubg_sim {
   }
 
   void invalidate_graphs() {
-    for (auto &g : graphs) {
+    for (auto &g : graphs)
       if (g.exec) (void)hipGraphExecDestroy(g.exec);
-      g.exec = nullptr;
-      g.valid = false;
-    }
+    graphs.clear();
   }
 
   float *field(int id) {
@@ -683,7 +709,7 @@ struct ubg_sim {
     case UBG_FIELD_VY_ACCUM: return vy_acc;
     case UBG_FIELD_VX_CURRENT: return vx_cur;
     case UBG_FIELD_VY_CURRENT: return vy_cur;
-    case UBG_FIELD_P: return p;
+    case UBG_FIELD_P: return P();
     case UBG_FIELD_F: return f;
     case UBG_FIELD_R: return r;
     }
@@ -721,7 +747,7 @@ UBG_API int ubg_sim_create(const float *flag, int width, int height, float pwidt
   default_params(&s->prm);
   const size_t nc = (size_t)width * height, nx = (size_t)(width - 1) * height,
                ny = (size_t)width * (height - 1);
-  size_t bytes = 256 + 4 * align_up(nx * 4) + 4 * align_up(ny * 4) + 3 * align_up(nc * 4) +
+  size_t bytes = 256 + 4 * align_up(nx * 4) + 4 * align_up(ny * 4) + 4 * align_up(nc * 4) +
                  2 * align_up(nc);
   int rc = s->pool.init(bytes);
   if (rc) {
@@ -736,7 +762,8 @@ UBG_API int ubg_sim_create(const float *flag, int width, int height, float pwidt
   s->vy_acc = s->pool.take<float>(ny);
   s->vx_cur = s->pool.take<float>(nx);
   s->vy_cur = s->pool.take<float>(ny);
-  s->p = s->pool.take<float>(nc);
+  s->p_buf[0] = s->pool.take<float>(nc);
+  s->p_buf[1] = s->pool.take<float>(nc);
   s->f = s->pool.take<float>(nc);
   s->r = s->pool.take<float>(nc);
   s->flag = s->pool.take<uint8_t>(nc);
@@ -754,7 +781,8 @@ UBG_API int ubg_sim_create(const float *flag, int width, int height, float pwidt
     (void)hipMemcpyAsync(s->vx[b], col.data(), col.size() * 4, hipMemcpyHostToDevice, s->stream);
   (void)hipMemcpyAsync(s->flag, f8.data(), nc, hipMemcpyHostToDevice, s->stream);
   (void)hipStreamSynchronize(s->stream);
-  rc = mg_create(width, height, s->stream, s->p, s->f, s->flag, s->mask, &s->mg);
+  rc = mg_create(width, height, s->stream, s->p_buf[0], s->p_buf[1], s->f, s->flag, s->mask,
+                 &s->mg);
   if (rc) {
     (void)hipStreamDestroy(s->stream);
     s->pool.release();
@@ -904,7 +932,7 @@ UBG_API int ubg_sim_stage(ubg_sim *s, int stage, float dt) {
   case UBG_STAGE_ADVECT: s->advect(); break;
   case UBG_STAGE_SET_VBCS: s->set_vbcs(); break;
   case UBG_STAGE_PROJECT: UBG_TRY(s->project()); break;
-  case UBG_STAGE_SET_PBC: launch_set_pbc(s->p, s->W, s->H, s->bc, s->stream); break;
+  case UBG_STAGE_SET_PBC: launch_set_pbc(s->P(), s->W, s->H, s->bc, s->stream); break;
   case UBG_STAGE_SAVE_CURRENT: s->save_current(); break;
   case UBG_STAGE_DIVERGENCE:
     s->divergence();
@@ -934,7 +962,7 @@ UBG_API int ubg_sim_set_grids(ubg_sim *s, int x, int y, float val) {
     if (x > 0) UBG_HIP(hipMemcpyAsync(s->vx[s->fx] + (size_t)y * (W - 1) + x - 1, &z, 4, hipMemcpyHostToDevice, s->stream));
     if (y < s->H - 1) UBG_HIP(hipMemcpyAsync(s->vy[s->fy] + (size_t)y * W + x, &z, 4, hipMemcpyHostToDevice, s->stream));
     if (y > 0) UBG_HIP(hipMemcpyAsync(s->vy[s->fy] + (size_t)(y - 1) * W + x, &z, 4, hipMemcpyHostToDevice, s->stream));
-    UBG_HIP(hipMemcpyAsync(s->p + (size_t)y * W + x, &z, 4, hipMemcpyHostToDevice, s->stream));
+    UBG_HIP(hipMemcpyAsync(s->P() + (size_t)y * W + x, &z, 4, hipMemcpyHostToDevice, s->stream));
   }
   // the level-0 neighbour mask around the cell
   launch_build_mask(s->flag, s->mask, s->W, s->H, s->stream);

@@ -1,20 +1,17 @@
 // mg_fast.hip -- fused multigrid kernels for gfx950.
 //
-// 1. k_rbgs_tile: one full red+black sweep of rbgs (pressure_solver.cpp:49-89,
-//    two-pass semantics) in one launch.  A workgroup loads a TX x TY tile of p
-//    with a two-cell halo into LDS (f and the flag mask with a one-cell halo),
-//    updates the red cells of the tile plus a one-cell ring (from the old black
-//    values, exactly what the first pass computes there), then the black cells
-//    of the tile from those red values, and writes the tile once.  Red cells
-//    of the ring are recomputed by the neighbouring tile with identical
-//    inputs, so the result equals the two-pass sweep.  With zero_gradient the
-//    setZeroGradientBC copy (pressure_solver.cpp:183-192) is applied to the
-//    tile's boundary cells from their updated interior neighbours; the tile
-//    grid is shifted by one cell where needed so that every boundary cell
-//    shares its tile with that neighbour.
+// 1. k_rbgs_stream: one full red+black sweep of rbgs (pressure_solver.cpp:49-89,
+//    two-pass semantics) in one launch.  Each wave streams down a strip of
+//    columns holding rows of p in registers; the red cells of row y+1 are
+//    updated before the black cells of row y, which is exactly the order the
+//    two-pass sweep implies (red reads only old black values, black reads only
+//    new red values).  With zero_gradient the setZeroGradientBC copy
+//    (pressure_solver.cpp:183-192) is applied as each row becomes final.  The
+//    sweep reads p from one buffer and writes the other (the level swaps
+//    them), so neighbouring waves never read each other's new values.
 //    HBM traffic per sweep: p read + write, f read, 1-byte mask read =
-//    13 bytes/cell; halo re-reads are served by L2, and tiles are ordered so
-//    that each XCD (own L2) works on a contiguous band of the grid.
+//    13 bytes/cell; halo rows/columns are re-read from L2, and waves are ordered
+//    so each XCD (own L2) works on a contiguous band of the grid.
 //
 // 2. k_coarse_vcycle: the V-cycle of every level from `first` down to the
 //    coarsest solved level L-2 (pressure_solver.cpp:201-248), run by one
@@ -27,12 +24,9 @@ namespace ubg {
 
 namespace {
 
-constexpr int TX = 64, TY = 32, TPB_T = 256;
-
-__device__ __forceinline__ float inv_count(int s) {
-  // 1/sum of neighbour flags; sum in {1..4}
-  return s == 1 ? 1.0f : (s == 2 ? 0.5f : (s == 3 ? (1.0f / 3.0f) : 0.25f));
-}
+constexpr int TPB_T = 256;
+constexpr int SW2 = 124;    // output columns per wave in the streaming smoother
+constexpr int PREFETCH = 4; // rows loaded ahead by each streaming wave
 
 // smoothingKernel (pressure_solver.cpp:10-24) from neighbour values and mask
 __device__ __forceinline__ float smooth(float pc, float pw, float pe, float ps, float pn,
@@ -44,7 +38,7 @@ __device__ __forceinline__ float smooth(float pc, float pw, float pe, float ps, 
   val += (mk & MK_S) ? ps : 0.0f;
   val += (mk & MK_N) ? pn : 0.0f;
   val += fv * h * h;
-  val = (s == 0) ? 0.0f : val * inv_count(s);
+  val = (s == 0) ? 0.0f : val / (float)s;
   float F = (mk & MK_SELF) ? 1.0f : 0.0f;
   return F * (alpha * val + (1.0f - alpha) * pc);
 }
@@ -56,107 +50,139 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   return k * q + min(k, rem) + (b >> 3);
 }
 
+// One wave = one strip of columns; lane l owns the horizontal cell pair
+// (x0, x0+1), x0 = SW2*s - 2 + 2l, so each row of a pair holds exactly one
+// red and one black cell and every smoother evaluation is useful.  Lanes
+// 1..62 own their pair (SW2 = 124 columns per wave); lanes 0 and 63 are the
+// halo the red update needs.  The wave streams down R rows keeping rows
+// y-1..y+1 in registers (plus a D-row prefetch queue): at row y it updates
+// the red cell of row y+1 (old black neighbours) and the black cell of row y
+// (neighbours: red cells of rows y-1, y+1 and the row's own red cells, all
+// already updated), which is exactly the two-pass order; row y is then final
+// and stored.  The horizontal neighbour outside the pair comes from the
+// adjacent lane (one __shfl per colour and row).  Reads come from `src`,
+// writes go to `dst` (ping-pong), so no wave sees another wave's new values.
+// AL: w is even, so every pair is 8-byte aligned and loads/stores are float2.
+template <bool ZG, bool AL>
 __global__ void __launch_bounds__(TPB_T)
-    k_rbgs_tile(float *__restrict__ p, const float *__restrict__ f,
-                const uint8_t *__restrict__ m, int w, int h, float hh, float alpha, int zg,
-                int sx, int sy, int ntx) {
-  __shared__ float sp[TY + 4][TX + 4]; // origin (x0-2, y0-2)
-  __shared__ float sf[TY + 2][TX + 2]; // origin (x0-1, y0-1)
-  __shared__ uint8_t sm[TY + 2][TX + 2];
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
-  const int x0 = (t % ntx) * TX - sx, y0 = (t / ntx) * TY - sy;
-  const int tid = threadIdx.x;
-
-  // Issue every global load of this thread before the first LDS store so the
-  // loads overlap (compile-time trip counts, fully unrolled).
-  constexpr int NP = (TY + 4) * (TX + 4), NPI = (NP + TPB_T - 1) / TPB_T;
-  constexpr int NF = (TY + 2) * (TX + 2), NFI = (NF + TPB_T - 1) / TPB_T;
-  float vp[NPI], vf[NFI];
-  uint8_t vm[NFI];
-#pragma unroll
-  for (int k = 0; k < NPI; k++) {
-    int li = tid + k * TPB_T;
-    int r = li / (TX + 4), c = li - r * (TX + 4);
-    int gx = x0 - 2 + c, gy = y0 - 2 + r;
-    vp[k] = (li < NP && gx >= 0 && gx < w && gy >= 0 && gy < h) ? p[(size_t)gy * w + gx] : 0.0f;
-  }
-#pragma unroll
-  for (int k = 0; k < NFI; k++) {
-    int li = tid + k * TPB_T;
-    int r = li / (TX + 2), c = li - r * (TX + 2);
-    int gx = x0 - 1 + c, gy = y0 - 1 + r;
-    bool in = li < NF && gx >= 1 && gx < w - 1 && gy >= 1 && gy < h - 1;
-    size_t gi = (size_t)gy * w + gx;
-    vf[k] = in ? f[gi] : 0.0f;
-    vm[k] = in ? m[gi] : 0;
-  }
-#pragma unroll
-  for (int k = 0; k < NPI; k++) {
-    int li = tid + k * TPB_T;
-    if (li < NP) (&sp[0][0])[li] = vp[k];
-  }
-#pragma unroll
-  for (int k = 0; k < NFI; k++) {
-    int li = tid + k * TPB_T;
-    if (li < NF) {
-      (&sf[0][0])[li] = vf[k];
-      (&sm[0][0])[li] = vm[k];
+    k_rbgs_pair(const float *__restrict__ src, float *__restrict__ dst,
+                const float *__restrict__ f, const uint8_t *__restrict__ m, int w, int h,
+                float hh, float alpha, int nstrips, int R, int nunits) {
+  const int lane = threadIdx.x & 63;
+  // wave-uniform values are made scalar so branches on them are uniform
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int unit = xcd_tile(blockIdx.x, gridDim.x) * (TPB_T / 64) + wid;
+  if (unit >= nunits) return; // whole wave leaves together
+  const int s = unit % nstrips, r = unit / nstrips;
+  const int x0 = SW2 * s - 2 + 2 * lane, x1 = x0 + 1; // x0 even for every lane
+  const bool in0 = x0 >= 0 && x0 < w, in1 = x1 >= 0 && x1 < w;
+  const bool int0 = x0 >= 1 && x0 <= w - 2, int1 = x1 >= 1 && x1 <= w - 2;
+  const bool own = lane >= 1 && lane <= 62;
+  const int ya = 1 + R * r;
+  const int yb = min(ya + R - 1, h - 2);
+  // Loads use clamped (always valid) addresses; values at clamped positions
+  // only ever feed cells that are not updated (halo lanes, boundary rows).
+  const int c0 = min(max(x0, 0), w - 1), c1 = min(max(x1, 0), w - 1);
+  const int cp = min(max(x0, 0), w - 2); // aligned pair start when AL
+  auto rowoff = [&](int yy) { return (size_t)min(max(yy, 0), h - 1) * w; };
+  auto ldp = [&](int yy, float &a, float &b) {
+    const float *row = src + rowoff(yy);
+    if (AL) {
+      float2 v = *(const float2 *)(row + cp);
+      a = v.x;
+      b = v.y;
+    } else {
+      a = row[c0];
+      b = row[c1];
     }
-  }
-  __syncthreads();
-
-  // red: (x+y) odd over the tile plus a one-cell ring
-  constexpr int RH = (TX + 2) / 2;
-  for (int li = tid; li < (TY + 2) * RH; li += TPB_T) {
-    int r = li / RH, k = li - r * RH;
-    int c = 2 * k + ((x0 + y0 + r + 1) & 1);
-    int gx = x0 - 1 + c, gy = y0 - 1 + r;
-    if (gx >= 1 && gx < w - 1 && gy >= 1 && gy < h - 1) {
-      int R = r + 1, Cc = c + 1;
-      sp[R][Cc] = smooth(sp[R][Cc], sp[R][Cc - 1], sp[R][Cc + 1], sp[R - 1][Cc], sp[R + 1][Cc],
-                         sf[r][c], sm[r][c], hh, alpha);
+  };
+  auto ldfm = [&](int yy, float &fa, float &fb, unsigned &ma, unsigned &mb) {
+    const size_t o = rowoff(yy);
+    if (AL) {
+      float2 v = *(const float2 *)(f + o + cp);
+      uchar2 q = *(const uchar2 *)(m + o + cp);
+      fa = v.x;
+      fb = v.y;
+      ma = q.x;
+      mb = q.y;
+    } else {
+      fa = f[o + c0];
+      fb = f[o + c1];
+      ma = m[o + c0];
+      mb = m[o + c1];
     }
-  }
-  __syncthreads();
+  };
+  // New value of cell a (x0) or cell b (x1) of the pair in an interior row;
+  // the horizontal neighbour outside the pair comes from the adjacent lane.
+  auto relax_a = [&](float a, float b, float dn, float up, float fv, unsigned mk) {
+    float west = __shfl_up(b, 1);
+    float na = smooth(a, west, b, dn, up, fv, mk, hh, alpha);
+    return int0 ? na : a;
+  };
+  auto relax_b = [&](float a, float b, float dn, float up, float fv, unsigned mk) {
+    float east = __shfl_down(a, 1);
+    float nb = smooth(b, a, east, dn, up, fv, mk, hh, alpha);
+    return int1 ? nb : b;
+  };
+// colour of row yy: x0 is even, so cell a has colour (yy & 1), cell b the other
+#define RELAX(A, B, DNA, DNB, UPA, UPB, FA, FB, MA, MB, YY, ODD)                                \
+  do {                                                                                          \
+    if ((((YY) + (ODD)) & 1) == 0) A = relax_a(A, B, DNA, UPA, FA, MA);                         \
+    else B = relax_b(A, B, DNB, UPB, FB, MB);                                                   \
+  } while (0)
 
-  // black: (x+y) even over the tile
-  constexpr int BH = TX / 2;
-  for (int li = tid; li < TY * BH; li += TPB_T) {
-    int r = li / BH, k = li - r * BH;
-    int c = 2 * k + ((x0 + y0 + r) & 1);
-    int gx = x0 + c, gy = y0 + r;
-    if (gx >= 1 && gx < w - 1 && gy >= 1 && gy < h - 1) {
-      int R = r + 2, Cc = c + 2;
-      sp[R][Cc] = smooth(sp[R][Cc], sp[R][Cc - 1], sp[R][Cc + 1], sp[R - 1][Cc], sp[R + 1][Cc],
-                         sf[r + 1][c + 1], sm[r + 1][c + 1], hh, alpha);
+  float pm1a, pm1b, p0a, p0b, p1a, p1b, p2a, p2b, pm2a, pm2b;
+  float fm1a, fm1b, f0a, f0b, f1a, f1b;
+  unsigned mm1a, mm1b, m0a, m0b, m1a, m1b;
+  ldp(ya - 2, pm2a, pm2b);
+  ldp(ya - 1, pm1a, pm1b);
+  ldp(ya, p0a, p0b);
+  ldp(ya + 1, p1a, p1b);
+  ldp(ya + 2, p2a, p2b);
+  ldfm(ya - 1, fm1a, fm1b, mm1a, mm1b);
+  ldfm(ya, f0a, f0b, m0a, m0b);
+  ldfm(ya + 1, f1a, f1b, m1a, m1b);
+  if (ya - 1 >= 1) RELAX(pm1a, pm1b, pm2a, pm2b, p0a, p0b, fm1a, fm1b, mm1a, mm1b, ya - 1, 1);
+  RELAX(p0a, p0b, pm1a, pm1b, p1a, p1b, f0a, f0b, m0a, m0b, ya, 1);
+  for (int y = ya; y <= yb; y++) {
+    float p3a, p3b, f2a, f2b;
+    unsigned m2a, m2b;
+    ldp(y + 3, p3a, p3b);
+    ldfm(y + 2, f2a, f2b, m2a, m2b);
+    if (y + 1 <= h - 2) RELAX(p1a, p1b, p0a, p0b, p2a, p2b, f1a, f1b, m1a, m1b, y + 1, 1);
+    RELAX(p0a, p0b, pm1a, pm1b, p1a, p1b, f0a, f0b, m0a, m0b, y, 0); // row y final
+    float oa = p0a, ob = p0b;
+    if (ZG) { // setZeroGradientBC on the side columns (rows 1..h-2)
+      float left = __shfl_up(p0b, 1);
+      oa = (x0 == 0) ? p0b : ((x0 == w - 1) ? left : oa);
+      ob = (x1 == w - 1) ? p0a : ob;
     }
-  }
-  __syncthreads();
-
-  if (zg) {
-    // setZeroGradientBC on the tile's boundary cells (corners untouched)
-    for (int li = tid; li < TY * TX; li += TPB_T) {
-      int r = li / TX, c = li - r * TX;
-      int gx = x0 + c, gy = y0 + r;
-      if (gx < 0 || gx >= w || gy < 0 || gy >= h) continue;
-      int R = r + 2, Cc = c + 2;
-      if (gy >= 1 && gy < h - 1) {
-        if (gx == 0) sp[R][Cc] = sp[R][Cc + 1];
-        else if (gx == w - 1) sp[R][Cc] = sp[R][Cc - 1];
+    if (own) {
+      float *row = dst + (size_t)y * w;
+      if (AL && in0 && in1) {
+        *(float2 *)(row + x0) = make_float2(oa, ob);
+      } else {
+        if (in0) row[x0] = oa;
+        if (in1) row[x1] = ob;
       }
-      if (gx >= 1 && gx < w - 1) {
-        if (gy == 0) sp[R][Cc] = sp[R + 1][Cc];
-        else if (gy == h - 1) sp[R][Cc] = sp[R - 1][Cc];
-      }
     }
-    __syncthreads();
+    if ((y == 1 || y == h - 2) && own) { // rows 0 / h-1: ZG copies, corners kept
+      float *edge = dst + (y == 1 ? 0 : (size_t)(h - 1) * w);
+      float ea = (ZG && int0) ? p0a : (y == 1 ? pm1a : p1a);
+      float eb = (ZG && int1) ? p0b : (y == 1 ? pm1b : p1b);
+      if (in0) edge[x0] = ea;
+      if (in1) edge[x1] = eb;
+    }
+    pm1a = p0a; pm1b = p0b;
+    p0a = p1a; p0b = p1b;
+    p1a = p2a; p1b = p2b;
+    p2a = p3a; p2b = p3b;
+    f0a = f1a; f0b = f1b;
+    f1a = f2a; f1b = f2b;
+    m0a = m1a; m0b = m1b;
+    m1a = m2a; m1b = m2b;
   }
-
-  for (int li = tid; li < TY * TX; li += TPB_T) {
-    int r = li / TX, c = li - r * TX;
-    int gx = x0 + c, gy = y0 + r;
-    if (gx >= 0 && gx < w && gy >= 0 && gy < h) p[(size_t)gy * w + gx] = sp[r + 2][c + 2];
-  }
+#undef RELAX
 }
 
 // ---------------------------------------------------------------------------
@@ -177,16 +203,14 @@ struct CoarseArgs {
   float alpha;
 };
 
+// 1024 threads as 16 rows x 64 lanes: rows strided by 16, columns by 64
+// (no integer division in the loops)
 __device__ __forceinline__ void lds_sweep(float *P, const float *Fv, const uint8_t *M, int w,
                                           int h, float hh, float alpha) {
-  const int half = (w - 1) / 2;
-  const int n = (h - 2) * half;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   for (int color = 0; color < 2; color++) {
-    for (int li = threadIdx.x; li < n; li += TPB_C) {
-      int r = li / half, k = li - r * half;
-      int y = r + 1;
-      int x = 1 + ((y + color) & 1) + 2 * k;
-      if (x < w - 1) {
+    for (int y = 1 + ty; y < h - 1; y += 16) {
+      for (int x = 1 + ((y + color) & 1) + 2 * tx; x < w - 1; x += 128) {
         int i = y * w + x;
         P[i] = smooth(P[i], P[i - 1], P[i + 1], P[i - w], P[i + w], Fv[i], M[i], hh, alpha);
       }
@@ -240,9 +264,8 @@ __global__ void __launch_bounds__(TPB_C) k_coarse_vcycle(CoarseArgs a) {
     float *Pc = (float *)(smem + a.op[l + 1]);
     const int wc = a.w[l + 1], hc = a.h[l + 1];
     const float ihsq = 1.0f / a.hh[l] / a.hh[l];
-    const int nc = (wc - 2) * (hc - 2);
-    for (int li = tid; li < nc; li += TPB_C) {
-      int y = li / (wc - 2) + 1, x = li % (wc - 2) + 1;
+    for (int y = 1 + (tid >> 6); y < hc - 1; y += 16)
+    for (int x = 1 + (tid & 63); x < wc - 1; x += 64) {
       int c = (2 * y) * w + 2 * x;
       auto R = [&](int dx, int dy) { return lds_residual(P, Fv, M, c + dy * w + dx, w, ihsq); };
       float v = 0.0f;
@@ -271,9 +294,8 @@ __global__ void __launch_bounds__(TPB_C) k_coarse_vcycle(CoarseArgs a) {
     const float *Pc = (const float *)(smem + a.op[l + 1]);
     const uint8_t *Mc = (const uint8_t *)(smem + a.om[l + 1]);
     const int wc = a.w[l + 1];
-    const int n = (w - 2) * (h - 2);
-    for (int li = tid; li < n; li += TPB_C) {
-      int y = li / (w - 2) + 1, x = li % (w - 2) + 1;
+    for (int y = 1 + (tid >> 6); y < h - 1; y += 16)
+    for (int x = 1 + (tid & 63); x < w - 1; x += 64) {
       int i = y * w + x;
       float F = (M[i] & MK_SELF) ? 1.0f : 0.0f;
       int ax = x >> 1, by = y >> 1;
@@ -323,13 +345,23 @@ size_t coarse_lds_bytes(const Level *lv, int first, int levels, CoarseArgs *a) {
 
 } // namespace
 
-void launch_rbgs_fused(const Level &L, float h, float alpha, bool zero_gradient, hipStream_t s) {
+void launch_rbgs_fused(Level &L, float h, float alpha, bool zero_gradient, hipStream_t s) {
   if (L.w < 3 || L.h < 3) return;
-  int sx = ((L.w - 1) % TX == 0) ? 1 : 0;
-  int sy = ((L.h - 1) % TY == 0) ? 1 : 0;
-  int ntx = (L.w + sx + TX - 1) / TX, nty = (L.h + sy + TY - 1) / TY;
-  hipLaunchKernelGGL(k_rbgs_tile, dim3(ntx * nty), dim3(TPB_T), 0, s, L.p, L.f, L.mask, L.w, L.h,
-                     h, alpha, zero_gradient ? 1 : 0, sx, sy, ntx);
+  const int nstrips = (L.w + SW2 - 1) / SW2;
+  // rows per wave: enough waves to fill the chip, at most 64 rows
+  int R = (int)((long)nstrips * (L.h - 2) / 4096);
+  R = R < 4 ? 4 : (R > 64 ? 64 : R);
+  const int nrb = (L.h - 2 + R - 1) / R;
+  const int nunits = nstrips * nrb;
+  const int nblocks = (nunits + (TPB_T / 64) - 1) / (TPB_T / 64);
+  const bool al = (L.w % 2) == 0;
+  auto k = zero_gradient ? (al ? k_rbgs_pair<true, true> : k_rbgs_pair<true, false>)
+                         : (al ? k_rbgs_pair<false, true> : k_rbgs_pair<false, false>);
+  hipLaunchKernelGGL(k, dim3(nblocks), dim3(TPB_T), 0, s, L.p, L.p_alt, L.f, L.mask, L.w, L.h, h,
+                     alpha, nstrips, R, nunits);
+  float *t = L.p;
+  L.p = L.p_alt;
+  L.p_alt = t;
 }
 
 bool coarse_fits_lds(const Level *lv, int first, int levels) {

@@ -181,6 +181,73 @@ __global__ void k_residual_restrict(const float *__restrict__ p, const float *__
   ec[ci] = 0.0f;
 }
 
+// LDS-tiled variant for large levels: a workgroup owns 32x8 coarse cells,
+// computes the 65x17 fine residuals under them once into LDS (p staged with
+// a one-cell halo, all loads issued before the first LDS store), then
+// restricts.  Same arithmetic as k_residual_restrict.
+constexpr int RCX = 32, RCY = 8;
+constexpr int RFX = 2 * RCX + 1, RFY = 2 * RCY + 1;  // residual region
+constexpr int RPX = RFX + 2, RPY = RFY + 2;          // p region
+__global__ void __launch_bounds__(256)
+    k_residual_restrict_tile(const float *__restrict__ p, const float *__restrict__ f,
+                             const uint8_t *__restrict__ m, int wf, int hf, float ihsq,
+                             float *__restrict__ rc, float *__restrict__ ec, int wc, int hc,
+                             int ntx) {
+  __shared__ float sp[RPY][RPX];
+  __shared__ float sr[RFY][RFX + 1];
+  const int tid = threadIdx.x;
+  const int X0 = 1 + (blockIdx.x % ntx) * RCX, Y0 = 1 + (blockIdx.x / ntx) * RCY;
+  const int fx0 = 2 * X0 - 1, fy0 = 2 * Y0 - 1; // residual region origin
+  constexpr int NP = RPX * RPY, NPI = (NP + 255) / 256;
+  float v[NPI];
+#pragma unroll
+  for (int k = 0; k < NPI; k++) {
+    int li = tid + 256 * k;
+    int r = li / RPX, c = li - r * RPX;
+    int gx = fx0 - 1 + c, gy = fy0 - 1 + r;
+    v[k] = (li < NP && gx >= 0 && gx < wf && gy >= 0 && gy < hf) ? p[(size_t)gy * wf + gx] : 0.0f;
+  }
+#pragma unroll
+  for (int k = 0; k < NPI; k++) {
+    int li = tid + 256 * k;
+    if (li < NP) (&sp[0][0])[li] = v[k];
+  }
+  __syncthreads();
+  for (int li = tid; li < RFX * RFY; li += 256) {
+    int r = li / RFX, c = li - r * RFX;
+    int gx = fx0 + c, gy = fy0 + r;
+    float res = 0.0f;
+    if (gx >= 1 && gx < wf - 1 && gy >= 1 && gy < hf - 1) {
+      size_t gi = (size_t)gy * wf + gx;
+      unsigned mk = m[gi];
+      float pc = sp[r + 1][c + 1];
+      float val = 0.0f;
+      val += (mk & MK_W) ? sp[r + 1][c] : pc;
+      val += (mk & MK_E) ? sp[r + 1][c + 2] : pc;
+      val += (mk & MK_S) ? sp[r][c + 1] : pc;
+      val += (mk & MK_N) ? sp[r + 2][c + 1] : pc;
+      val += -4.0f * pc;
+      val *= ihsq;
+      res = (mk & MK_SELF) ? (f[gi] + val) : 0.0f;
+    }
+    sr[r][c] = res;
+  }
+  __syncthreads();
+  const int cx = tid % RCX, cy = tid / RCX; // 256 threads = 32 x 8 coarse cells
+  const int X = X0 + cx, Y = Y0 + cy;
+  if (X < wc - 1 && Y < hc - 1) {
+    const int r = 2 * cy + 1, c = 2 * cx + 1; // fine (2X, 2Y) in the residual region
+    auto R = [&](int dx, int dy) { return sr[r + dy][c + dx]; };
+    float vv = 0.0f;
+    vv = R(-1, -1) * 1.0f + R(0, -1) * 2.0f + R(1, -1) * 1.0f;
+    vv += R(-1, 0) * 2.0f + R(0, 0) * 4.0f + R(1, 0) * 2.0f;
+    vv += R(-1, 1) * 1.0f + R(0, 1) * 2.0f + R(1, 1) * 1.0f;
+    size_t ci = (size_t)Y * wc + X;
+    rc[ci] = vv * (1.0f / 16.0f);
+    ec[ci] = 0.0f;
+  }
+}
+
 // flag sum + the double literal 0.0001, stored as float (pressure_solver.cpp:149)
 __device__ __forceinline__ float eps_sum(int s) { return (float)((double)s + 0.0001); }
 
@@ -265,8 +332,14 @@ void launch_residual(const Level &L, float h, hipStream_t s) {
 void launch_residual_restrict(const Level &F, const Level &C, float h, hipStream_t s) {
   if (C.w < 3 || C.h < 3) return;
   float ihsq = 1.0f / h / h;
-  hipLaunchKernelGGL(k_residual_restrict, dim3(ceil_div(C.w - 2, TPB), C.h - 2), dim3(TPB), 0, s,
-                     F.p, F.f, F.mask, F.w, ihsq, C.f, C.p, C.w);
+  if ((size_t)C.w * C.h >= 16384) {
+    int ntx = (C.w - 2 + RCX - 1) / RCX, nty = (C.h - 2 + RCY - 1) / RCY;
+    hipLaunchKernelGGL(k_residual_restrict_tile, dim3(ntx * nty), dim3(256), 0, s, F.p, F.f,
+                       F.mask, F.w, F.h, ihsq, C.f, C.p, C.w, C.h, ntx);
+  } else {
+    hipLaunchKernelGGL(k_residual_restrict, dim3(ceil_div(C.w - 2, TPB), C.h - 2), dim3(TPB), 0,
+                       s, F.p, F.f, F.mask, F.w, ihsq, C.f, C.p, C.w);
+  }
 }
 void launch_prolong_correct(const Level &F, const Level &C, hipStream_t s) {
   hipLaunchKernelGGL(k_prolong_correct, dim3(ceil_div(F.w - 2, TPB), F.h - 2), dim3(TPB), 0, s,

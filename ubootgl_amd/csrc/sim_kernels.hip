@@ -24,73 +24,97 @@ __device__ __forceinline__ float vbc_per(int bc, float a, float b) {
   return -a;
 }
 
-// setVBCs (simulation.cpp:80-102).  The reference runs four sequential loops:
-// vx W/E columns, vx S/N rows, vy W/E columns, vy S/N rows.  Only the row
-// loops read values the column loops wrote (the corners' neighbours), so the
-// two column loops run as one launch and the two row loops as the next, which
-// reproduces the sequential result exactly.
-__global__ void k_vbcs_cols(float *__restrict__ xf, float *__restrict__ xb,
-                            float *__restrict__ yf, float *__restrict__ yb, int W, int H, int bw,
-                            int be) {
-  const int Wx = W - 1, Hy = H - 1;
-  int y = blockIdx.x * blockDim.x + threadIdx.x;
-  if (y < H) {
-    size_t r0 = (size_t)y * Wx;
-    float w0 = vbc_par(bw, xf[r0 + 1], xf[r0]);
-    float e0 = vbc_par(be, xf[r0 + Wx - 2], xf[r0 + Wx - 1]);
-    xf[r0] = xb[r0] = w0;
-    xf[r0 + Wx - 1] = xb[r0 + Wx - 1] = e0;
+// setVBCs (simulation.cpp:80-102) in one launch.  The reference runs four
+// sequential loops: vx W/E columns (all rows), vx S/N rows (all columns), vy
+// W/E columns, vy S/N rows; BC values are written to both buffers.  A row loop
+// reads the column loop's result only at the corners' neighbours, and a
+// column value at rows 0/H-1 is overwritten by the row loop.  So every
+// boundary cell is a function of interior values and old boundary values,
+// except the corners: each corner group -- the corner plus its two boundary
+// neighbours -- is computed by one thread from old values in the sequential
+// order; all other boundary cells by one thread each.  No thread reads a
+// cell another thread writes.
+struct BCGrid {
+  float *f, *b;
+  int w, h; // grid size
+  int pw, pe, ps, pn; // BC kinds: west, east, south, north
+  bool xpar;          // W/E use VBCPar (vx) or VBCPer (vy)
+};
+__device__ __forceinline__ float bc_we(const BCGrid &g, int kind, float a, float bb) {
+  return g.xpar ? vbc_par(kind, a, bb) : vbc_per(kind, a, bb);
+}
+__device__ __forceinline__ float bc_sn(const BCGrid &g, int kind, float a, float bb) {
+  return g.xpar ? vbc_per(kind, a, bb) : vbc_par(kind, a, bb);
+}
+__device__ __forceinline__ void put(const BCGrid &g, int x, int y, float v) {
+  size_t i = (size_t)y * g.w + x;
+  g.f[i] = v;
+  g.b[i] = v;
+}
+__device__ void bc_thread(const BCGrid &g, int t) {
+  const int w = g.w, h = g.h;
+  auto F = [&](int x, int y) { return g.f[(size_t)y * w + x]; };
+  if (t >= 2 && t <= h - 3) { // columns, rows away from the corners
+    float a = bc_we(g, g.pw, F(1, t), F(0, t));
+    float b = bc_we(g, g.pe, F(w - 2, t), F(w - 1, t));
+    put(g, 0, t, a);
+    put(g, w - 1, t, b);
   }
-  if (y < Hy) {
-    size_t r0 = (size_t)y * W;
-    float w0 = vbc_per(bw, yf[r0 + 1], yf[r0]);
-    float e0 = vbc_per(be, yf[r0 + W - 2], yf[r0 + W - 1]);
-    yf[r0] = yb[r0] = w0;
-    yf[r0 + W - 1] = yb[r0 + W - 1] = e0;
+  if (t >= 2 && t <= w - 3) { // rows, columns away from the corners
+    float a = bc_sn(g, g.ps, F(t, 1), F(t, 0));
+    float b = bc_sn(g, g.pn, F(t, h - 2), F(t, h - 1));
+    put(g, t, 0, a);
+    put(g, t, h - 1, b);
+  }
+  if (t < 4) { // corner groups: SW, SE, NW, NE
+    const int cx = (t & 1) ? w - 1 : 0, ix = (t & 1) ? w - 2 : 1;  // boundary / interior column
+    const int cy = (t & 2) ? h - 1 : 0, iy = (t & 2) ? h - 2 : 1;  // boundary / interior row
+    const int kx = (t & 1) ? g.pe : g.pw, ky = (t & 2) ? g.pn : g.ps;
+    float side = bc_we(g, kx, F(ix, iy), F(cx, iy));      // (cx, iy) after the column loop
+    float cornA = bc_we(g, kx, F(ix, cy), F(cx, cy));     // (cx, cy) after the column loop
+    float edge = bc_sn(g, ky, F(ix, iy), F(ix, cy));      // (ix, cy) after the row loop
+    float corn = bc_sn(g, ky, side, cornA);               // (cx, cy) after the row loop
+    put(g, cx, iy, side);
+    put(g, ix, cy, edge);
+    put(g, cx, cy, corn);
   }
 }
-__global__ void k_vbcs_rows(float *__restrict__ xf, float *__restrict__ xb,
-                            float *__restrict__ yf, float *__restrict__ yb, int W, int H, int bn,
-                            int bs) {
-  const int Wx = W - 1, Hy = H - 1;
-  int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x < Wx) {
-    size_t top = (size_t)(H - 1) * Wx + x, top1 = (size_t)(H - 2) * Wx + x;
-    float s0 = vbc_per(bs, xf[(size_t)Wx + x], xf[x]);
-    float n0 = vbc_per(bn, xf[top1], xf[top]);
-    xf[x] = xb[x] = s0;
-    xf[top] = xb[top] = n0;
-  }
-  if (x < W) {
-    size_t top = (size_t)(Hy - 1) * W + x, top1 = (size_t)(Hy - 2) * W + x;
-    float s0 = vbc_par(bs, yf[(size_t)W + x], yf[x]);
-    float n0 = vbc_par(bn, yf[top1], yf[top]);
-    yf[x] = yb[x] = s0;
-    yf[top] = yb[top] = n0;
-  }
+__global__ void k_set_vbcs(BCGrid gx, BCGrid gy) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  bc_thread(gx, t);
+  bc_thread(gy, t);
 }
 
-// setPBC (simulation.cpp:36-45): W/E columns over all rows, then S/N rows over
-// all columns; only OUTFLOW_ZERO_PRESSURE negates.
+// setPBC (simulation.cpp:36-45) in one launch: W/E columns over all rows,
+// then S/N rows over all columns; only OUTFLOW_ZERO_PRESSURE negates.  Corner
+// groups as in setVBCs.
 __device__ __forceinline__ float pbc(int bc, float a) {
   return bc == BC_OUTFLOW_ZERO_PRESSURE ? -a : a;
 }
-__global__ void k_pbc_cols(float *__restrict__ p, int W, int H, int bw, int be) {
-  int y = blockIdx.x * blockDim.x + threadIdx.x;
-  if (y >= H) return;
-  size_t r0 = (size_t)y * W;
-  float w0 = pbc(bw, p[r0 + 1]);
-  float e0 = pbc(be, p[r0 + W - 2]);
-  p[r0] = w0;
-  p[r0 + W - 1] = e0;
-}
-__global__ void k_pbc_rows(float *__restrict__ p, int W, int H, int bn, int bs) {
-  int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= W) return;
-  float s0 = pbc(bs, p[(size_t)W + x]);
-  float n0 = pbc(bn, p[(size_t)(H - 2) * W + x]);
-  p[x] = s0;
-  p[(size_t)(H - 1) * W + x] = n0;
+__global__ void k_set_pbc(float *__restrict__ p, int w, int h, int bw, int be, int bn, int bs) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  auto P = [&](int x, int y) -> float & { return p[(size_t)y * w + x]; };
+  if (t >= 2 && t <= h - 3) {
+    float a = pbc(bw, P(1, t)), b = pbc(be, P(w - 2, t));
+    P(0, t) = a;
+    P(w - 1, t) = b;
+  }
+  if (t >= 2 && t <= w - 3) {
+    float a = pbc(bs, P(t, 1)), b = pbc(bn, P(t, h - 2));
+    P(t, 0) = a;
+    P(t, h - 1) = b;
+  }
+  if (t < 4) {
+    const int cx = (t & 1) ? w - 1 : 0, ix = (t & 1) ? w - 2 : 1;
+    const int cy = (t & 2) ? h - 1 : 0, iy = (t & 2) ? h - 2 : 1;
+    const int kx = (t & 1) ? be : bw, ky = (t & 2) ? bn : bs;
+    float side = pbc(kx, P(ix, iy));
+    float edge = pbc(ky, P(ix, iy));
+    float corn = pbc(ky, side);
+    P(cx, iy) = side;
+    P(ix, cy) = edge;
+    P(cx, cy) = corn;
+  }
 }
 
 // applyAccumulatedVelocity (simulation.cpp:376-396), interior faces only.
@@ -373,14 +397,15 @@ __global__ void k_tracers(const float *__restrict__ xf, const float *__restrict_
 
 // ---------------------------------------------------------------------------
 void launch_set_vbcs(const VelView &v, const BCs &bc, hipStream_t s) {
-  hipLaunchKernelGGL(k_vbcs_cols, dim3(ceil_div(v.H, TPB)), dim3(TPB), 0, s, v.vxf, v.vxb, v.vyf,
-                     v.vyb, v.W, v.H, bc.west, bc.east);
-  hipLaunchKernelGGL(k_vbcs_rows, dim3(ceil_div(v.W, TPB)), dim3(TPB), 0, s, v.vxf, v.vxb, v.vyf,
-                     v.vyb, v.W, v.H, bc.north, bc.south);
+  BCGrid gx{v.vxf, v.vxb, v.W - 1, v.H, bc.west, bc.east, bc.south, bc.north, true};
+  BCGrid gy{v.vyf, v.vyb, v.W, v.H - 1, bc.west, bc.east, bc.south, bc.north, false};
+  int n = v.W > v.H ? v.W : v.H;
+  hipLaunchKernelGGL(k_set_vbcs, dim3(ceil_div(n, TPB)), dim3(TPB), 0, s, gx, gy);
 }
 void launch_set_pbc(float *p, int W, int H, const BCs &bc, hipStream_t s) {
-  hipLaunchKernelGGL(k_pbc_cols, dim3(ceil_div(H, TPB)), dim3(TPB), 0, s, p, W, H, bc.west, bc.east);
-  hipLaunchKernelGGL(k_pbc_rows, dim3(ceil_div(W, TPB)), dim3(TPB), 0, s, p, W, H, bc.north, bc.south);
+  int n = W > H ? W : H;
+  hipLaunchKernelGGL(k_set_pbc, dim3(ceil_div(n, TPB)), dim3(TPB), 0, s, p, W, H, bc.west, bc.east,
+                     bc.north, bc.south);
 }
 void launch_accumulate(const VelView &v, float *xa, float *ya, hipStream_t s) {
   hipLaunchKernelGGL(k_accumulate, dim3(ceil_div(v.W - 2, TPB), v.H - 2), dim3(TPB), 0, s, v.vxf,

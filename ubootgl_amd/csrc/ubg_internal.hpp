@@ -29,6 +29,7 @@ enum BCKind : int { BC_INFLOW = 0, BC_OUTFLOW = 1, BC_OUTFLOW_ZERO_PRESSURE = 2,
 struct Level {
   int w = 0, h = 0;
   float *p = nullptr;      // solution: level 0 pressure, coarse levels ecs
+  float *p_alt = nullptr;  // second buffer: sweeps read p, write p_alt, then swap
   float *f = nullptr;      // right-hand side: level 0 f, coarse levels rcs
   float *r = nullptr;      // residual scratch (stage-wise residual only)
   uint8_t *flag = nullptr; // 0/1 flag (level 0 input, coarse flagcs)
@@ -56,9 +57,10 @@ void launch_build_mask(const uint8_t *flag, uint8_t *mask, int w, int h, hipStre
 void launch_coarsen(const uint8_t *ff, int wf, uint8_t *fc, int wc, int hc, hipStream_t s);
 // one red+black sweep, two colour passes
 void launch_rbgs_sweep(const Level &L, float h, float alpha, hipStream_t s);
-// one red+black sweep fused in one launch (LDS tile with a two-cell halo);
-// zero_gradient applies setZeroGradientBC to the result.
-void launch_rbgs_fused(const Level &L, float h, float alpha, bool zero_gradient, hipStream_t s);
+// one red+black sweep fused in one launch (streaming waves, ping-pong: reads
+// L.p, writes L.p_alt, then swaps the two pointers); zero_gradient applies
+// setZeroGradientBC to the result.
+void launch_rbgs_fused(Level &L, float h, float alpha, bool zero_gradient, hipStream_t s);
 void launch_zero_gradient(float *p, int w, int h, hipStream_t s);
 void launch_residual(const Level &L, float h, hipStream_t s);
 void launch_residual_restrict(const Level &F, const Level &C, float h, hipStream_t s);
