@@ -51,6 +51,7 @@ def lib():
         "oracle_bilinear": (C.c_float, [_f32p, C.c_int, C.c_int, C.c_float, C.c_float]),
         "oracle_cubic_hermite": (C.c_float, [C.c_float] * 5),
         "oracle_wang_hash": (C.c_uint32, [C.c_uint32]),
+        "oracle_check_count_division": (C.c_long, [C.c_uint32]),
         "oracle_lcg": (C.c_uint32, [C.c_uint32]),
         "oracle_mg_create": (C.c_void_p, [C.c_int, C.c_int, C.c_int]),
         "oracle_mg_destroy": (None, [C.c_void_p, C.c_int]),

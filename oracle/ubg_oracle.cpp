@@ -736,6 +736,35 @@ ORACLE_API float oracle_cubic_hermite(float t, float A, float B, float C, float 
   return cubic_hermite(t, A, B, C, D);
 }
 ORACLE_API uint32_t oracle_wang_hash(uint32_t s) { return wang_hash(s); }
+
+// Checks the identity the GPU smoother uses to divide by the neighbour count
+// s in {1..4} without a division: with inv = RN(1/s) and q = RN(v*inv),
+// RN(q + RN?(v - s*q)*inv) (two fused multiply-adds) equals RN(v/s).  Scans
+// float bit patterns 0, stride, 2*stride, ... (all finite v) and returns the
+// number of mismatches other than v = -0.
+ORACLE_API long oracle_check_count_division(uint32_t stride) {
+  long bad = 0;
+  const float invs[5] = {0.0f, 1.0f, 0.5f, 1.0f / 3.0f, 0.25f};
+  for (int s = 1; s <= 4; s++) {
+    const float inv = invs[s], sf = (float)s;
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+    for (long i = 0; i < (1L << 32); i += stride) {
+      uint32_t u = (uint32_t)i;
+      float v;
+      std::memcpy(&v, &u, 4);
+      if (!std::isfinite(v) || u == 0x80000000u) continue;
+      float ref = v / sf;
+      float q = v * inv;
+      float r = std::fma(-sf, q, v);
+      float q2 = std::fma(r, inv, q);
+      uint32_t a, b;
+      std::memcpy(&a, &ref, 4);
+      std::memcpy(&b, &q2, 4);
+      if (a != b) bad++;
+    }
+  }
+  return bad;
+}
 ORACLE_API uint32_t oracle_lcg(uint32_t s) { return lcg(s); }
 
 // ---- MG handle (fp32 = reference, fp64 = the mixed-precision oracle) ----

@@ -625,3 +625,11 @@ def test_step_deterministic_and_thread_independent():
         s2.step(float(d.dt))
     O.set_threads(0)
     np.testing.assert_array_equal(a, s2.get(O.VX))
+
+
+def test_count_division_identity():
+    """The GPU smoother divides by the neighbour count s in {1..4} as
+    fma(fma(-s, q, v), RN(1/s), q) with q = v*RN(1/s); that must equal the
+    IEEE quotient v/s bit for bit (exhaustive over all 2^32 patterns offline;
+    here every 61st pattern, all exponents covered)."""
+    assert O.lib().oracle_check_count_division(61) == 0

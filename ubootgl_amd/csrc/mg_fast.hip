@@ -28,17 +28,48 @@ constexpr int TPB_T = 256;
 constexpr int SW2 = 124;    // output columns per wave in the streaming smoother
 constexpr int PREFETCH = 4; // rows loaded ahead by each streaming wave
 
-// smoothingKernel (pressure_solver.cpp:10-24) from neighbour values and mask
+// v / s for the neighbour count s in {0..4} without the IEEE division
+// sequence: q = v * RN(1/s), then one FMA correction q + (v - s*q) * RN(1/s).
+// By Markstein's theorem this is the correctly rounded quotient; checked
+// exhaustively against v / (float)s for all 2^32 float v (the only
+// difference is the sign of a zero quotient of v = -0, which the smoother
+// never produces: its sum starts at +0).  s = 0 gives +0, as the reference's
+// "if (sum == 0) val = 0" does.
+__device__ __forceinline__ float div_count(float v, unsigned s) {
+  float inv = s == 1u ? 1.0f : (s == 2u ? 0.5f : (s == 3u ? (1.0f / 3.0f) : (s == 4u ? 0.25f : 0.0f)));
+  float q = v * inv;
+  float r = fmaf(-(float)s, q, v);
+  return fmaf(r, inv, q);
+}
+
+// Whole-wave lane shifts by one (DPP wave_shr:1 / wave_shl:1, a VALU move):
+// lane i receives lane i-1 (from_left) or lane i+1 (from_right); the edge lane
+// keeps its own value.  Lane layout verified on gfx950.
+__device__ __forceinline__ float from_left(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x138,
+                                                    0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_right(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x130,
+                                                    0xf, 0xf, false));
+}
+
+// select v if bit k of the mask is set, else +0 (bitwise, no compare)
+__device__ __forceinline__ float keep_if(unsigned mk, int k, float v) {
+  return __int_as_float(__float_as_int(v) & -(int)((mk >> k) & 1u));
+}
+
+// smoothingKernel (pressure_solver.cpp:10-24) from neighbour values and mask;
+// the neighbour-flag count is stored in mask bits 5-7.
 __device__ __forceinline__ float smooth(float pc, float pw, float pe, float ps, float pn,
                                         float fv, unsigned mk, float h, float alpha) {
-  int s = ((mk >> 1) & 1) + ((mk >> 2) & 1) + ((mk >> 3) & 1) + ((mk >> 4) & 1);
   float val = 0.0f;
-  val += (mk & MK_W) ? pw : 0.0f;
-  val += (mk & MK_E) ? pe : 0.0f;
-  val += (mk & MK_S) ? ps : 0.0f;
-  val += (mk & MK_N) ? pn : 0.0f;
+  val += keep_if(mk, 1, pw);
+  val += keep_if(mk, 2, pe);
+  val += keep_if(mk, 3, ps);
+  val += keep_if(mk, 4, pn);
   val += fv * h * h;
-  val = (s == 0) ? 0.0f : val / (float)s;
+  val = div_count(val, mk >> 5);
   float F = (mk & MK_SELF) ? 1.0f : 0.0f;
   return F * (alpha * val + (1.0f - alpha) * pc);
 }
@@ -115,12 +146,12 @@ __global__ void __launch_bounds__(TPB_T)
   // New value of cell a (x0) or cell b (x1) of the pair in an interior row;
   // the horizontal neighbour outside the pair comes from the adjacent lane.
   auto relax_a = [&](float a, float b, float dn, float up, float fv, unsigned mk) {
-    float west = __shfl_up(b, 1);
+    float west = from_left(b);
     float na = smooth(a, west, b, dn, up, fv, mk, hh, alpha);
     return int0 ? na : a;
   };
   auto relax_b = [&](float a, float b, float dn, float up, float fv, unsigned mk) {
-    float east = __shfl_down(a, 1);
+    float east = from_right(a);
     float nb = smooth(b, a, east, dn, up, fv, mk, hh, alpha);
     return int1 ? nb : b;
   };
@@ -131,29 +162,31 @@ __global__ void __launch_bounds__(TPB_T)
     else B = relax_b(A, B, DNB, UPB, FB, MB);                                                   \
   } while (0)
 
-  float pm1a, pm1b, p0a, p0b, p1a, p1b, p2a, p2b, pm2a, pm2b;
-  float fm1a, fm1b, f0a, f0b, f1a, f1b;
-  unsigned mm1a, mm1b, m0a, m0b, m1a, m1b;
+  float pm1a, pm1b, p0a, p0b, p1a, p1b, p2a, p2b, p3a, p3b, pm2a, pm2b;
+  float fm1a, fm1b, f0a, f0b, f1a, f1b, f2a, f2b;
+  unsigned mm1a, mm1b, m0a, m0b, m1a, m1b, m2a, m2b;
   ldp(ya - 2, pm2a, pm2b);
   ldp(ya - 1, pm1a, pm1b);
   ldp(ya, p0a, p0b);
   ldp(ya + 1, p1a, p1b);
   ldp(ya + 2, p2a, p2b);
+  ldp(ya + 3, p3a, p3b);
   ldfm(ya - 1, fm1a, fm1b, mm1a, mm1b);
   ldfm(ya, f0a, f0b, m0a, m0b);
   ldfm(ya + 1, f1a, f1b, m1a, m1b);
+  ldfm(ya + 2, f2a, f2b, m2a, m2b);
   if (ya - 1 >= 1) RELAX(pm1a, pm1b, pm2a, pm2b, p0a, p0b, fm1a, fm1b, mm1a, mm1b, ya - 1, 1);
   RELAX(p0a, p0b, pm1a, pm1b, p1a, p1b, f0a, f0b, m0a, m0b, ya, 1);
   for (int y = ya; y <= yb; y++) {
-    float p3a, p3b, f2a, f2b;
-    unsigned m2a, m2b;
-    ldp(y + 3, p3a, p3b);
-    ldfm(y + 2, f2a, f2b, m2a, m2b);
+    float p4a, p4b, f3a, f3b; // two rows of lead
+    unsigned m3a, m3b;
+    ldp(y + 4, p4a, p4b);
+    ldfm(y + 3, f3a, f3b, m3a, m3b);
     if (y + 1 <= h - 2) RELAX(p1a, p1b, p0a, p0b, p2a, p2b, f1a, f1b, m1a, m1b, y + 1, 1);
     RELAX(p0a, p0b, pm1a, pm1b, p1a, p1b, f0a, f0b, m0a, m0b, y, 0); // row y final
     float oa = p0a, ob = p0b;
     if (ZG) { // setZeroGradientBC on the side columns (rows 1..h-2)
-      float left = __shfl_up(p0b, 1);
+      float left = from_left(p0b);
       oa = (x0 == 0) ? p0b : ((x0 == w - 1) ? left : oa);
       ob = (x1 == w - 1) ? p0a : ob;
     }
@@ -177,10 +210,13 @@ __global__ void __launch_bounds__(TPB_T)
     p0a = p1a; p0b = p1b;
     p1a = p2a; p1b = p2b;
     p2a = p3a; p2b = p3b;
+    p3a = p4a; p3b = p4b;
     f0a = f1a; f0b = f1b;
     f1a = f2a; f1b = f2b;
+    f2a = f3a; f2b = f3b;
     m0a = m1a; m0b = m1b;
     m1a = m2a; m1b = m2b;
+    m2a = m3a; m2b = m3b;
   }
 #undef RELAX
 }

@@ -51,8 +51,8 @@ __global__ void k_sumsq(const float *__restrict__ s, size_t n, double *acc) {
   if (threadIdx.x == 0) atomicAdd(acc, red[0]);
 }
 
-// mask = own flag (bit 0) + the four neighbours' flags (bits 1-4); neighbours
-// outside the grid count as 0.
+// mask = own flag (bit 0) + the four neighbours' flags (bits 1-4) + their
+// count (bits 5-7); neighbours outside the grid count as 0.
 __global__ void k_build_mask(const uint8_t *__restrict__ fl, uint8_t *__restrict__ m, int w,
                              int h) {
   int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -64,7 +64,8 @@ __global__ void k_build_mask(const uint8_t *__restrict__ fl, uint8_t *__restrict
   if (x < w - 1 && fl[i + 1]) b |= MK_E;
   if (y > 0 && fl[i - w]) b |= MK_S;
   if (y < h - 1 && fl[i + w]) b |= MK_N;
-  m[i] = b;
+  unsigned cnt = ((b >> 1) & 1) + ((b >> 2) & 1) + ((b >> 3) & 1) + ((b >> 4) & 1);
+  m[i] = (uint8_t)(b | (cnt << 5));
 }
 
 // pressure_solver.hpp:43-56: coarse flag is 1 iff sum(w * f)/16 > 0.2 over the
@@ -85,34 +86,6 @@ __global__ void k_coarsen(const uint8_t *__restrict__ ff, int wf, uint8_t *__res
     out = v >= 4 ? 1 : 0;
   }
   fc[(size_t)y * wc + x] = out;
-}
-
-// pressure_solver.cpp:10-24 for one cell; sum of neighbour flags in
-// {0..4}; division by it as in the reference.
-__device__ __forceinline__ float smooth_cell(float pc, float pw, float pe, float ps, float pn,
-                                             float fv, uint8_t mk, float h, float alpha) {
-  float sum = (float)(((mk >> 1) & 1) + ((mk >> 2) & 1) + ((mk >> 4) & 1) + ((mk >> 3) & 1));
-  float val = 0.0f;
-  val += (mk & MK_W) ? pw : 0.0f;
-  val += (mk & MK_E) ? pe : 0.0f;
-  val += (mk & MK_S) ? ps : 0.0f;
-  val += (mk & MK_N) ? pn : 0.0f;
-  val += fv * h * h;
-  val = (sum == 0.0f) ? 0.0f : val / sum;
-  float F = (mk & MK_SELF) ? 1.0f : 0.0f;
-  return F * (alpha * val + (1.0f - alpha) * pc);
-}
-
-// One colour of rbgs: colour 0 = red = (x+y) odd (rbgs_red_line,
-// pressure_solver.cpp:35-40), colour 1 = black (:42-47).
-__global__ void k_rbgs_color(float *__restrict__ p, const float *__restrict__ f,
-                             const uint8_t *__restrict__ m, int w, float h, float alpha,
-                             int color) {
-  int y = blockIdx.y + 1;
-  int x = 1 + ((y + color) & 1) + 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (x >= w - 1) return;
-  size_t i = (size_t)y * w + x;
-  p[i] = smooth_cell(p[i], p[i - 1], p[i + 1], p[i - w], p[i + w], f[i], m[i], h, alpha);
 }
 
 // setZeroGradientBC (pressure_solver.cpp:183-192): edges copy the first
@@ -312,13 +285,6 @@ void launch_build_mask(const uint8_t *flag, uint8_t *mask, int w, int h, hipStre
 }
 void launch_coarsen(const uint8_t *ff, int wf, uint8_t *fc, int wc, int hc, hipStream_t s) {
   hipLaunchKernelGGL(k_coarsen, dim3(ceil_div(wc, TPB), hc), dim3(TPB), 0, s, ff, wf, fc, wc, hc);
-}
-void launch_rbgs_sweep(const Level &L, float h, float alpha, hipStream_t s) {
-  if (L.w < 3 || L.h < 3) return;
-  int half = (L.w - 1) / 2 + 1;
-  dim3 grid(ceil_div(half, TPB), L.h - 2);
-  hipLaunchKernelGGL(k_rbgs_color, grid, dim3(TPB), 0, s, L.p, L.f, L.mask, L.w, h, alpha, 0);
-  hipLaunchKernelGGL(k_rbgs_color, grid, dim3(TPB), 0, s, L.p, L.f, L.mask, L.w, h, alpha, 1);
 }
 void launch_zero_gradient(float *p, int w, int h, hipStream_t s) {
   int n = w > h ? w : h;

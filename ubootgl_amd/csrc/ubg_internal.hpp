@@ -5,8 +5,9 @@
 //     host layout of Single2DGrid (db2dgrid.hpp:12-50), so host copies are
 //     plain memcpys and flat-offset quirks of the reference carry over;
 //   * flags are one byte per cell: `flag` holds the 0/1 value, `mask` holds the
-//     cell's own flag in bit 0 and its four neighbours' flags in bits 1-4, so
-//     the smoother, the residual and the prolongation read one byte per cell;
+//     cell's own flag in bit 0, its four neighbours' flags in bits 1-4 and
+//     their count in bits 5-7, so the smoother, the residual and the
+//     prolongation read one byte per cell;
 //   * an MG hierarchy is one allocation carved into levels.
 #pragma once
 
@@ -55,8 +56,6 @@ void launch_sumsq(const float *src, size_t n, double *acc, hipStream_t s);
 // ---- multigrid (pressure_solver.hpp / pressure_solver.cpp) ---------------
 void launch_build_mask(const uint8_t *flag, uint8_t *mask, int w, int h, hipStream_t s);
 void launch_coarsen(const uint8_t *ff, int wf, uint8_t *fc, int wc, int hc, hipStream_t s);
-// one red+black sweep, two colour passes
-void launch_rbgs_sweep(const Level &L, float h, float alpha, hipStream_t s);
 // one red+black sweep fused in one launch (streaming waves, ping-pong: reads
 // L.p, writes L.p_alt, then swaps the two pointers); zero_gradient applies
 // setZeroGradientBC to the result.
