@@ -22,9 +22,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-SMOOTH_BYTES_PER_CELL = 13  # p r/w 8 + f 4 + flag 1 per red+black sweep
+SMOOTH_BYTES_PER_CELL = 13  # one red+black sweep: p r/w 8 + f 4 + flag 1
+RR_BYTES_PER_CELL = 10      # residual + restriction (p 4, f 4, flag 1, coarse rhs 1)
+PC_BYTES_PER_CELL = 10.25   # prolongation + correction (coarse 1, coarse flag 0.25, flag 1, p r/w 8)
+# compulsory HBM bytes per cell of the two fused level-0 passes
+PRE_PASS_BYTES = 15.0       # p r 4 + f 4 + flag 1 + p w 4 + coarse rhs w 1 + coarse p w 1
+POST_PASS_BYTES = 14.25     # p r 4 + f 4 + flag 1 + coarse p 1 + coarse flag 0.25 + p w 4
 STEP_BYTES_PER_CELL_22 = 278.6  # step at 2/2 smoothing, 2 V-cycles (algorithmic)
 TRACER_BYTES = 24  # xy r/w 16 + age r/w 8 per point
+KERNEL_NAMES = {"sweep": "level-0 rbgs sweep (red + black, one launch)",
+                "pre": "level-0 fused pass: rbgs sweep + residual + restriction",
+                "post": "level-0 fused pass: prolongation + correction + rbgs sweep"}
 
 
 def log(msg):
@@ -90,23 +98,42 @@ def main():
     steps_per_s = args.steps / (t2 - t1)
     log(f"timed: {args.steps} steps in {(t2 - t1):.3f}s -> {steps_per_s:.1f} steps/s")
 
-    # dominant kernel: HIP events on the simulation's stream around every
-    # level-0 smoother launch, over an instrumented pass of the same steps
+    # dominant kernels: HIP events on the simulation's stream around every
+    # level-0 launch of the three multigrid kernels (plain rbgs sweep; fused
+    # sweep+residual+restriction; fused prolongation+correction+sweep) over an
+    # instrumented pass of the same steps
+    from ubootgl_amd import _lib as LL
     ninstr = max(10, min(args.steps, 50))
-    sim.timer_enable(True)
+    timers = ((LL.TIMER_SMOOTH_L0, "sweep", SMOOTH_BYTES_PER_CELL, SMOOTH_BYTES_PER_CELL),
+              (LL.TIMER_L0_PRE, "pre", PRE_PASS_BYTES, SMOOTH_BYTES_PER_CELL + RR_BYTES_PER_CELL),
+              (LL.TIMER_L0_POST, "post", POST_PASS_BYTES, PC_BYTES_PER_CELL + SMOOTH_BYTES_PER_CELL))
+    for t, *_ in timers:
+        sim.timer_enable(t, True)
     sim.run(dt, ninstr)
     sim.sync()
-    kms, klaunch = sim.timer_read()
-    sim.timer_enable(False)
     cells = d.W * d.H
-    kavg_ms = kms / max(klaunch, 1)
-    achieved = SMOOTH_BYTES_PER_CELL * cells / (kavg_ms * 1e-3) / 1e9
+    kern = {}
+    for t, name, pass_b, alg_b in timers:
+        ms, n = sim.timer_read(t)
+        sim.timer_enable(t, False)
+        if n == 0:
+            continue
+        avg = ms / n
+        kern[name] = dict(total_ms=ms, launches=n, avg_us=avg * 1e3,
+                          pass_GBps=pass_b * cells / (avg * 1e-3) / 1e9,
+                          unfused_equiv_GBps=alg_b * cells / (avg * 1e-3) / 1e9,
+                          pass_bytes=pass_b * cells)
+        log(f"level-0 {name}: {n} launches, avg {avg * 1e3:.2f} us -> "
+            f"{kern[name]['pass_GBps']:.0f} GB/s compulsory, "
+            f"{kern[name]['unfused_equiv_GBps']:.0f} GB/s unfused-equivalent")
+    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    K = kern[dom]
+    achieved = K["pass_GBps"]
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "rbgs_l0_4096x2048_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", f"mg_{dom}_l0_4096x2048_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
-    log(f"smoother: {klaunch} launches, avg {kavg_ms * 1e3:.2f} us -> {achieved:.0f} GB/s algorithmic")
 
     vxg = sim.get("vx")
     finite = bool(np.isfinite(vxg).all())
@@ -140,15 +167,17 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "rbgs red+black sweep, level 0",
+            "kernel": KERNEL_NAMES[dom],
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "avg_launch_us": round(kavg_ms * 1e3, 3),
-            "launches": klaunch,
-            "bytes_per_launch": SMOOTH_BYTES_PER_CELL * cells,
+            "avg_launch_us": round(K["avg_us"], 3),
+            "launches": K["launches"],
+            "bytes_per_launch": K["pass_bytes"],
+            "unfused_equiv_GBps": round(K["unfused_equiv_GBps"], 1),
+            "other_pass": {k: round(v["avg_us"], 3) for k, v in kern.items() if k != dom},
         },
         "cpu_baseline": cpu,
     }

@@ -53,6 +53,9 @@ int check_launch() {
 
 size_t align_up(size_t n, size_t a = 256) { return (n + a - 1) / a * a; }
 
+// levels with at least this many cells run the fused two-pass level kernels
+constexpr size_t kFuseMinCells = (size_t)1 << 22;
+
 // A simple bump allocator over one hipMalloc.
 struct Pool {
   char *base = nullptr;
@@ -139,33 +142,43 @@ struct ubg_mg {
   bool own_stream = false;
   ubg_params prm{};
   double *d_acc = nullptr;
-  Timer *timer = nullptr; // borrowed from the owning simulation
+  Timer *timers = nullptr; // borrowed from the owning simulation (UBG_TIMER_COUNT)
   std::recursive_mutex mu;
 
-  // smoother sweep at level l (zero-gradient only on level 0 if requested)
-  void sweep(int l, float h, bool zg) {
-    Level &L = lv[l];
-    bool timed = timer && timer->on && l == 0;
-    if (timed) {
-      if (timer->used + 2 > timer->ev.size()) {
-        size_t n = timer->ev.size();
-        timer->ev.resize(n + 512);
-        for (size_t i = n; i < timer->ev.size(); i++) (void)hipEventCreate(&timer->ev[i]);
+  // run `fn` (one kernel launch) bracketed by HIP events on this stream when
+  // timer `id` is enabled and the work is on level 0
+  template <typename Fn> void timed(int id, int l, Fn fn) {
+    Timer *t = timers ? &timers[id] : nullptr;
+    bool on = t && t->on && l == 0;
+    if (on) {
+      if (t->used + 2 > t->ev.size()) {
+        size_t n = t->ev.size();
+        t->ev.resize(n + 512);
+        for (size_t i = n; i < t->ev.size(); i++) (void)hipEventCreate(&t->ev[i]);
       }
-      (void)hipEventRecord(timer->ev[timer->used], stream);
+      (void)hipEventRecord(t->ev[t->used], stream);
     }
-    launch_rbgs_fused(L, h, prm.alpha, zg, stream);
-    if (timed) {
-      (void)hipEventRecord(timer->ev[timer->used + 1], stream);
-      timer->used += 2;
+    fn();
+    if (on) {
+      (void)hipEventRecord(t->ev[t->used + 1], stream);
+      t->used += 2;
     }
   }
 
-  // MG::solveLevel (pressure_solver.cpp:201-248)
+  // n sweeps at level l, one launch each
+  void sweeps(int l, int n, float h, bool zg) {
+    Level &L = lv[l];
+    for (int i = 0; i < n; i++)
+      timed(UBG_TIMER_SMOOTH_L0, l, [&] { launch_rbgs_fused(L, h, prm.alpha, zg, stream); });
+  }
+
+  // MG::solveLevel (pressure_solver.cpp:201-248).  A level costs two fused
+  // passes: (pre-smoothing, residual, restriction) and (prolongation,
+  // correction, post-smoothing); setZeroGradientBC happens inside them.
   void solve_level(int l, float h, bool zg) {
     const bool z = zg && l == 0;
     if (l == levels - 2) {
-      for (int i = 0; i < prm.nu_coarse; i++) sweep(l, h, false);
+      sweeps(l, prm.nu_coarse, h, false); // no ZG on the coarsest solved level
       return;
     }
     if (l > 0 && coarse_fits_lds(lv.data(), l, levels)) {
@@ -173,14 +186,28 @@ struct ubg_mg {
                            prm.alpha, stream);
       return;
     }
-    for (int i = 0; i < prm.nu_pre; i++) sweep(l, h, z);
     Level &L = lv[l], &C = lv[l + 1];
-    launch_residual_restrict(L, C, h, stream);
+    if ((size_t)L.w * L.h < kFuseMinCells) {
+      // small level: few waves, so short per-wave work wins over fewer passes
+      sweeps(l, prm.nu_pre, h, z);
+      launch_residual_restrict(L, C, h, stream);
+      float hc = h * (L.w - 1.0f) / (C.w - 1.0f); // pressure_solver.cpp:229
+      solve_level(l + 1, hc, false);
+      launch_prolong_correct(L, C, stream);
+      if (z) launch_zero_gradient(L.p, L.w, L.h, stream);
+      sweeps(l, prm.nu_post, h, z);
+      return;
+    }
+    // the last pre-smoothing sweep and the first post-smoothing sweep ride in
+    // the grid-transfer passes
+    const int pre_fused = std::min(prm.nu_pre, 1);
+    sweeps(l, prm.nu_pre - pre_fused, h, z);
+    timed(UBG_TIMER_L0_PRE, l, [&] { launch_presmooth_restrict(L, C, pre_fused, h, prm.alpha, z, stream); });
     float hc = h * (L.w - 1.0f) / (C.w - 1.0f); // pressure_solver.cpp:229
     solve_level(l + 1, hc, false);
-    launch_prolong_correct(L, C, stream);
-    if (z) launch_zero_gradient(L.p, L.w, L.h, stream);
-    for (int i = 0; i < prm.nu_post; i++) sweep(l, h, z);
+    const int post_fused = std::min(prm.nu_post, 1);
+    timed(UBG_TIMER_L0_POST, l, [&] { launch_prolong_postsmooth(L, C, post_fused, h, prm.alpha, z, stream); });
+    sweeps(l, prm.nu_post - post_fused, h, z);
   }
 
   // coarse flags of levels 1..L-1 from the level-0 flag (updateFields)
@@ -453,7 +480,10 @@ UBG_API int ubg_mg_solve_host(ubg_mg *m, float *p, const float *f, const float *
 UBG_API int ubg_mg_rbgs(ubg_mg *m, float h, float alpha, int nsweeps) {
   if (!m) return fail(UBG_E_INVALID, "null mg");
   std::lock_guard<std::recursive_mutex> g(m->mu);
-  for (int i = 0; i < nsweeps; i++) launch_rbgs_fused(m->lv[0], h, alpha, false, m->stream);
+  float saved = m->prm.alpha;
+  m->prm.alpha = alpha;
+  m->sweeps(0, nsweeps, h, false);
+  m->prm.alpha = saved;
   UBG_TRY(check_launch());
   return UBG_OK;
 }
@@ -519,7 +549,7 @@ struct ubg_sim {
   std::vector<int> tr_perm_h;
   int ntr = 0, tr_cap = 0;
   uint32_t tr_seed = 1;
-  Timer timer;
+  Timer timers[UBG_TIMER_COUNT];
   // captured step graphs keyed by the buffer-role state they start from
   struct State {
     int fx = 0, fy = 0;
@@ -656,7 +686,9 @@ struct ubg_sim {
       launch_accumulate(vel(), vx_acc, vy_acc, stream);
       accum_dirty = false;
     }
-    bool graphable = prm.use_graph && sinks.empty() && !timer.on;
+    bool any_timer = false;
+    for (auto &t : timers) any_timer = any_timer || t.on;
+    bool graphable = prm.use_graph && sinks.empty() && !any_timer;
     if (graphable) {
       State st = state();
       G *g = nullptr;
@@ -789,7 +821,7 @@ UBG_API int ubg_sim_create(const float *flag, int width, int height, float pwidt
     delete s;
     return rc;
   }
-  s->mg->timer = &s->timer;
+  s->mg->timers = s->timers;
   s->mg->classify(); // MG(flag) -> updateFields (simulation.hpp:62)
   rc = check_launch();
   if (rc) return rc;
@@ -803,7 +835,8 @@ UBG_API int ubg_sim_destroy(ubg_sim *s) {
   if (!s) return UBG_OK;
   (void)hipStreamSynchronize(s->stream);
   s->invalidate_graphs();
-  for (auto ev : s->timer.ev) (void)hipEventDestroy(ev);
+  for (auto &t : s->timers)
+    for (auto ev : t.ev) (void)hipEventDestroy(ev);
   ubg_mg_destroy(s->mg);
   if (s->d_sink_xy) (void)hipFree(s->d_sink_xy);
   if (s->d_sink_val) (void)hipFree(s->d_sink_val);
@@ -1116,21 +1149,22 @@ UBG_API int ubg_sim_sync(ubg_sim *s) {
 }
 
 UBG_API int ubg_sim_timer_enable(ubg_sim *s, int timer, int on) {
-  if (!s || timer != UBG_TIMER_SMOOTH_L0) return fail(UBG_E_INVALID, "bad argument");
+  if (!s || timer < 0 || timer >= UBG_TIMER_COUNT) return fail(UBG_E_INVALID, "bad argument");
   std::lock_guard<std::recursive_mutex> g(s->mu_);
   UBG_HIP(hipStreamSynchronize(s->stream));
-  s->timer.on = on != 0;
-  s->timer.used = 0;
-  s->timer.total_ms = 0.0;
-  s->timer.launches = 0;
+  Timer &t = s->timers[timer];
+  t.on = on != 0;
+  t.used = 0;
+  t.total_ms = 0.0;
+  t.launches = 0;
   return UBG_OK;
 }
 
 UBG_API int ubg_sim_timer_read(ubg_sim *s, int timer, double *total_ms, long *launches) {
-  if (!s || timer != UBG_TIMER_SMOOTH_L0) return fail(UBG_E_INVALID, "bad argument");
+  if (!s || timer < 0 || timer >= UBG_TIMER_COUNT) return fail(UBG_E_INVALID, "bad argument");
   std::lock_guard<std::recursive_mutex> g(s->mu_);
   UBG_HIP(hipStreamSynchronize(s->stream));
-  Timer &t = s->timer;
+  Timer &t = s->timers[timer];
   for (size_t i = 0; i + 1 < t.used; i += 2) {
     float ms = 0.0f;
     UBG_HIP(hipEventElapsedTime(&ms, t.ev[i], t.ev[i + 1]));

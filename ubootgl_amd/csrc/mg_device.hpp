@@ -1,0 +1,90 @@
+// mg_device.hpp -- device helpers shared by the multigrid kernels: the
+// smoothing point update, the exact division by the neighbour count, lane
+// shifts and the XCD-aware block order.
+#pragma once
+
+#include "ubg_internal.hpp"
+
+namespace ubg {
+
+// v / s for the neighbour count s in {0..4} without the IEEE division
+// sequence: q = v * RN(1/s), then one FMA correction q + (v - s*q) * RN(1/s).
+// By Markstein's theorem this is the correctly rounded quotient; checked
+// exhaustively against v / (float)s for all 2^32 float v (the only
+// difference is the sign of a zero quotient of v = -0, which the smoother
+// never produces: its sum starts at +0).  s = 0 gives +0, as the reference's
+// "if (sum == 0) val = 0" does.
+__device__ __forceinline__ float div_count(float v, unsigned s) {
+  // RN(1/s) without branches: 2^-(s>>1) for s = 1, 2, 4 from the exponent
+  // bits, one select for s = 3
+  const float pow2 = __int_as_float((int)((127u - (s >> 1)) << 23));
+  const float inv = (s == 3u) ? (1.0f / 3.0f) : pow2;
+  const float q = v * inv;
+  const float r = fmaf(-(float)s, q, v);
+  const float q2 = fmaf(r, inv, q);
+  return (s == 0u) ? 0.0f : q2;
+}
+
+// select v if bit k of the mask is set, else +0 (bitwise, no compare)
+__device__ __forceinline__ float keep_if(unsigned mk, int k, float v) {
+  return __int_as_float(__float_as_int(v) & -(int)((mk >> k) & 1u));
+}
+
+// smoothingKernel (pressure_solver.cpp:10-24) from neighbour values and mask;
+// the neighbour-flag count is stored in mask bits 5-7.
+__device__ __forceinline__ float smooth(float pc, float pw, float pe, float ps, float pn,
+                                        float fv, unsigned mk, float h, float alpha) {
+  float val = 0.0f;
+  val += keep_if(mk, 1, pw);
+  val += keep_if(mk, 2, pe);
+  val += keep_if(mk, 3, ps);
+  val += keep_if(mk, 4, pn);
+  val += fv * h * h;
+  val = div_count(val, mk >> 5);
+  float F = (mk & MK_SELF) ? 1.0f : 0.0f;
+  return F * (alpha * val + (1.0f - alpha) * pc);
+}
+
+// calculateResidualField for one cell (pressure_solver.cpp:100-108):
+// r = F*(f + (sum_n(F_n p_n + (1-F_n) p) - 4p)/h^2), neighbours in the order
+// x-1, x+1, y-1, y+1.
+__device__ __forceinline__ float residual_of(float pc, float pw, float pe, float ps, float pn,
+                                             float fv, unsigned mk, float ihsq) {
+  float val = 0.0f;
+  val += (mk & MK_W) ? pw : pc;
+  val += (mk & MK_E) ? pe : pc;
+  val += (mk & MK_S) ? ps : pc;
+  val += (mk & MK_N) ? pn : pc;
+  val += -4.0f * pc;
+  val *= ihsq;
+  return (mk & MK_SELF) ? (fv + val) : 0.0f;
+}
+
+// Whole-wave lane shifts by one (DPP wave_shr:1 / wave_shl:1, a VALU move):
+// lane i receives lane i-1 (from_left) or lane i+1 (from_right); the edge lane
+// keeps its own value.  Lane layout verified on gfx950.
+__device__ __forceinline__ float from_left(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x138,
+                                                    0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_right(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x130,
+                                                    0xf, 0xf, false));
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs; hand each XCD a
+// contiguous range of work units so halo re-reads hit its own L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  int q = nb >> 3, rem = nb & 7, k = b & 7;
+  return k * q + min(k, rem) + (b >> 3);
+}
+
+// flag count + the double literal 0.0001, stored as float (pressure_solver.cpp:149)
+__device__ __forceinline__ float eps_count(int s) {
+  return s == 0 ? (float)(0.0 + 0.0001)
+                : (s == 1 ? (float)(1.0 + 0.0001)
+                          : (s == 2 ? (float)(2.0 + 0.0001)
+                                    : (s == 3 ? (float)(3.0 + 0.0001) : (float)(4.0 + 0.0001))));
+}
+
+} // namespace ubg

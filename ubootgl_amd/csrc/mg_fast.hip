@@ -18,7 +18,7 @@
 //    workgroup entirely in LDS (p, f and mask of all those levels fit in
 //    160 KB once the level is <= ~128x64).  Replaces ~40 latency-bound
 //    launches per V-cycle with one.
-#include "ubg_internal.hpp"
+#include "mg_device.hpp"
 
 namespace ubg {
 
@@ -27,59 +27,6 @@ namespace {
 constexpr int TPB_T = 256;
 constexpr int SW2 = 124;    // output columns per wave in the streaming smoother
 constexpr int PREFETCH = 4; // rows loaded ahead by each streaming wave
-
-// v / s for the neighbour count s in {0..4} without the IEEE division
-// sequence: q = v * RN(1/s), then one FMA correction q + (v - s*q) * RN(1/s).
-// By Markstein's theorem this is the correctly rounded quotient; checked
-// exhaustively against v / (float)s for all 2^32 float v (the only
-// difference is the sign of a zero quotient of v = -0, which the smoother
-// never produces: its sum starts at +0).  s = 0 gives +0, as the reference's
-// "if (sum == 0) val = 0" does.
-__device__ __forceinline__ float div_count(float v, unsigned s) {
-  float inv = s == 1u ? 1.0f : (s == 2u ? 0.5f : (s == 3u ? (1.0f / 3.0f) : (s == 4u ? 0.25f : 0.0f)));
-  float q = v * inv;
-  float r = fmaf(-(float)s, q, v);
-  return fmaf(r, inv, q);
-}
-
-// Whole-wave lane shifts by one (DPP wave_shr:1 / wave_shl:1, a VALU move):
-// lane i receives lane i-1 (from_left) or lane i+1 (from_right); the edge lane
-// keeps its own value.  Lane layout verified on gfx950.
-__device__ __forceinline__ float from_left(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x138,
-                                                    0xf, 0xf, false));
-}
-__device__ __forceinline__ float from_right(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x130,
-                                                    0xf, 0xf, false));
-}
-
-// select v if bit k of the mask is set, else +0 (bitwise, no compare)
-__device__ __forceinline__ float keep_if(unsigned mk, int k, float v) {
-  return __int_as_float(__float_as_int(v) & -(int)((mk >> k) & 1u));
-}
-
-// smoothingKernel (pressure_solver.cpp:10-24) from neighbour values and mask;
-// the neighbour-flag count is stored in mask bits 5-7.
-__device__ __forceinline__ float smooth(float pc, float pw, float pe, float ps, float pn,
-                                        float fv, unsigned mk, float h, float alpha) {
-  float val = 0.0f;
-  val += keep_if(mk, 1, pw);
-  val += keep_if(mk, 2, pe);
-  val += keep_if(mk, 3, ps);
-  val += keep_if(mk, 4, pn);
-  val += fv * h * h;
-  val = div_count(val, mk >> 5);
-  float F = (mk & MK_SELF) ? 1.0f : 0.0f;
-  return F * (alpha * val + (1.0f - alpha) * pc);
-}
-
-// Blocks are dispatched round-robin over the 8 XCDs; hand each XCD a
-// contiguous range of tiles so halo re-reads hit its own L2.
-__device__ __forceinline__ int xcd_tile(int b, int nb) {
-  int q = nb >> 3, rem = nb & 7, k = b & 7;
-  return k * q + min(k, rem) + (b >> 3);
-}
 
 // One wave = one strip of columns; lane l owns the horizontal cell pair
 // (x0, x0+1), x0 = SW2*s - 2 + 2l, so each row of a pair holds exactly one
@@ -190,13 +137,13 @@ __global__ void __launch_bounds__(TPB_T)
       oa = (x0 == 0) ? p0b : ((x0 == w - 1) ? left : oa);
       ob = (x1 == w - 1) ? p0a : ob;
     }
-    if (own) {
+    {
       float *row = dst + (size_t)y * w;
-      if (AL && in0 && in1) {
-        *(float2 *)(row + x0) = make_float2(oa, ob);
+      if (AL) { // w even: an owned in-domain pair is whole and 8-byte aligned
+        if (own && in0) *(float2 *)(row + x0) = make_float2(oa, ob);
       } else {
-        if (in0) row[x0] = oa;
-        if (in1) row[x1] = ob;
+        if (own && in0) row[x0] = oa;
+        if (own && in1) row[x1] = ob;
       }
     }
     if ((y == 1 || y == h - 2) && own) { // rows 0 / h-1: ZG copies, corners kept
@@ -257,19 +204,8 @@ __device__ __forceinline__ void lds_sweep(float *P, const float *Fv, const uint8
 
 __device__ __forceinline__ float lds_residual(const float *P, const float *Fv, const uint8_t *M,
                                               int i, int w, float ihsq) {
-  unsigned mk = M[i];
-  float pc = P[i];
-  float val = 0.0f;
-  val += (mk & MK_W) ? P[i - 1] : pc;
-  val += (mk & MK_E) ? P[i + 1] : pc;
-  val += (mk & MK_S) ? P[i - w] : pc;
-  val += (mk & MK_N) ? P[i + w] : pc;
-  val += -4.0f * pc;
-  val *= ihsq;
-  return (mk & MK_SELF) ? (Fv[i] + val) : 0.0f;
+  return residual_of(P[i], P[i - 1], P[i + 1], P[i - w], P[i + w], Fv[i], M[i], ihsq);
 }
-
-__device__ __forceinline__ float eps_sum(int s) { return (float)((double)s + 0.0001); }
 
 __global__ void __launch_bounds__(TPB_C) k_coarse_vcycle(CoarseArgs a) {
   extern __shared__ __align__(16) char smem[];
@@ -342,13 +278,13 @@ __global__ void __launch_bounds__(TPB_C) k_coarse_vcycle(CoarseArgs a) {
       if (xe && ye) {
         e = E(ax, by) * F;
       } else if (!xe && ye) {
-        if (x <= w - 3) e = F * (E(ax, by) + E(ax + 1, by)) / eps_sum(FC(ax, by) + FC(ax + 1, by));
+        if (x <= w - 3) e = F * (E(ax, by) + E(ax + 1, by)) / eps_count(FC(ax, by) + FC(ax + 1, by));
       } else if (xe && !ye) {
-        if (y <= h - 3) e = F * (E(ax, by) + E(ax, by + 1)) / eps_sum(FC(ax, by) + FC(ax, by + 1));
+        if (y <= h - 3) e = F * (E(ax, by) + E(ax, by + 1)) / eps_count(FC(ax, by) + FC(ax, by + 1));
       } else {
         if (x <= w - 3 && y <= h - 3)
           e = F * (E(ax, by) + E(ax + 1, by + 1) + E(ax + 1, by) + E(ax, by + 1)) /
-              eps_sum(FC(ax, by) + FC(ax + 1, by + 1) + FC(ax + 1, by) + FC(ax, by + 1));
+              eps_count(FC(ax, by) + FC(ax + 1, by + 1) + FC(ax + 1, by) + FC(ax, by + 1));
       }
       P[i] += e;
     }

@@ -61,6 +61,14 @@ void launch_coarsen(const uint8_t *ff, int wf, uint8_t *fc, int wc, int hc, hipS
 // setZeroGradientBC to the result.
 void launch_rbgs_fused(Level &L, float h, float alpha, bool zero_gradient, hipStream_t s);
 void launch_zero_gradient(float *p, int w, int h, hipStream_t s);
+// Row-pipelined level passes (ping-pong: read L.p, write L.p_alt, swap):
+//   ns (0 or 1) sweeps, then residual + restriction into C
+//   (C.f = restricted residual, C.p = 0);
+void launch_presmooth_restrict(Level &L, Level &C, int ns, float h, float alpha, bool zg,
+                               hipStream_t s);
+//   prolongation of C.p + correction (+ZG), then ns (0 or 1) sweeps.
+void launch_prolong_postsmooth(Level &L, const Level &C, int ns, float h, float alpha, bool zg,
+                               hipStream_t s);
 void launch_residual(const Level &L, float h, hipStream_t s);
 void launch_residual_restrict(const Level &F, const Level &C, float h, hipStream_t s);
 void launch_prolong_correct(const Level &F, const Level &C, hipStream_t s);

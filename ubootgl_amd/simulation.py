@@ -258,12 +258,13 @@ class Simulation:
     def sync(self):
         check(lib().ubg_sim_sync(self._h))
 
-    def timer_enable(self, on=True):
-        check(lib().ubg_sim_timer_enable(self._h, L.TIMER_SMOOTH_L0, int(on)))
+    def timer_enable(self, timer=L.TIMER_SMOOTH_L0, on=True):
+        check(lib().ubg_sim_timer_enable(self._h, timer, int(on)))
 
-    def timer_read(self):
+    def timer_read(self, timer=L.TIMER_SMOOTH_L0):
+        """(total ms, launches) of the kernel class `timer` since enabling."""
         ms, n = C.c_double(), C.c_long()
-        check(lib().ubg_sim_timer_read(self._h, L.TIMER_SMOOTH_L0, C.byref(ms), C.byref(n)))
+        check(lib().ubg_sim_timer_read(self._h, timer, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
     def close(self):
