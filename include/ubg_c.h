@@ -143,6 +143,45 @@ UBG_API int ubg_mg_rbgs(ubg_mg *mg, float h, float alpha, int nsweeps);
 UBG_API int ubg_mg_residual(ubg_mg *mg, float h, float *r_dst, double *norm);
 UBG_API int ubg_mg_sync(ubg_mg *mg);
 
+/* ---- FP64 iterative refinement around FP32 V-cycles ---------------------
+ * No counterpart in the reference (an extension of MG::solve,
+ * pressure_solver.hpp:59-62).  Solves the flag-masked Poisson problem of
+ * calculateResidualField (pressure_solver.cpp:91-116),
+ *     F(c) * (f(c) + sum_n F(n) (x(n) - x(c)) / h^2) = 0   on interior cells,
+ * with x on the outer ring held as Dirichlet data (mgtest/mgtest.cpp:18-29),
+ * in float64: x and the fine residual r are float64, each refinement step runs
+ * FP32 V-cycles on A e = r (level-0 rhs = (float)r, e starts at 0) and adds e
+ * to x in float64; the convergence norm ||r||_2 is summed in float64 in a
+ * fixed order.  The rhs f is float32 (the problem's data).  The x of the
+ * previous solve is the next solve's initial guess unless
+ * ubg_mg_ir_set_guess is called; the level-0 p/f of the MG are scratch. */
+#define UBG_IR_HISTORY 64
+typedef struct ubg_ir_params {
+  double tol;           /* stop when ||r_k||_2 <= tol * ||r_0||_2, 1e-10 */
+  int vcycles_per_iter; /* FP32 V-cycles per refinement step, 4 */
+  int max_iter;         /* refinement steps at most, 200 */
+} ubg_ir_params;
+typedef struct ubg_ir_result {
+  int iterations;       /* refinement steps taken */
+  int vcycles;          /* FP32 V-cycles run */
+  int converged;        /* 1 if the tolerance was met */
+  double r0_norm;       /* ||r_0||_2 of the initial guess (= ||F f|| for x = 0) */
+  double rel_residual;  /* ||r_k||_2 / ||r_0||_2 at exit */
+  double history[UBG_IR_HISTORY]; /* rel. residual after steps 1..64 */
+} ubg_ir_result;
+UBG_API void ubg_ir_params_default(ubg_ir_params *out);
+/* the float32 rhs f, W x H host array (copied to the device) */
+UBG_API int ubg_mg_ir_set_rhs(ubg_mg *mg, const float *f);
+/* the float64 initial guess incl. the Dirichlet ring, W x H host array;
+ * NULL sets x = 0 */
+UBG_API int ubg_mg_ir_set_guess(ubg_mg *mg, const double *x);
+/* refine until tol or max_iter; UBG_E_NOT_CONVERGED (result filled) if the
+ * limit is reached or the residual stops decreasing */
+UBG_API int ubg_mg_ir_solve(ubg_mg *mg, float h, const ubg_ir_params *prm, ubg_ir_result *res);
+UBG_API int ubg_mg_ir_get_solution(ubg_mg *mg, double *x);
+/* device addresses of the resident float64 solution and float32 rhs */
+UBG_API int ubg_mg_ir_device_ptrs(ubg_mg *mg, double **x, float **f);
+
 /* ---- Simulation: simulation.hpp:18-141 ---------------------------------- */
 /* Simulation(flag, pwidth, mu) (simulation.hpp:32-67): BCs S/N no-slip,
  * W inflow, E zero-pressure outflow; vx(0,y)=1, vy(x,0)=0 in both buffers;

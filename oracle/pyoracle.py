@@ -264,3 +264,27 @@ class OracleSim:
             lib().oracle_sim_destroy(self.h_)
         except Exception:
             pass
+
+
+def solve_poisson_fp64(flag, f, h, tol=1e-13, nu_pre=3, nu_post=3, nu_coarse=5, max_cycles=2000,
+                       x0=None):
+    """The FP64 CPU multigrid reference for the refinement solves: float64
+    V-cycles (pressure_solver.cpp:201-248 in double) on x from x0 (default 0,
+    the Dirichlet ring included) until ||r||_2 <= tol ||r_0||_2.  Returns
+    (x, history of relative residuals)."""
+    H, W = flag.shape
+    mg = OracleMG(W, H, fp64=True, nu_pre=nu_pre, nu_post=nu_post, nu_coarse=nu_coarse)
+    mg.update_fields(flag)
+    x = np.zeros((H, W), np.float64) if x0 is None else np.array(x0, np.float64, order="C")
+    ff, fl = f64(f), f64(flag)
+    _, r0 = residual(x, ff, fl, float(h))
+    hist = []
+    if r0 == 0.0:
+        return x, hist
+    for _ in range(max_cycles):
+        mg.solve(x, ff, fl, float(h))
+        _, rn = residual(x, ff, fl, float(h))
+        hist.append(rn / r0)
+        if hist[-1] <= tol:
+            break
+    return x, hist

@@ -49,6 +49,19 @@ class Params(C.Structure):
                 ("save_current", C.c_int)]
 
 
+IR_HISTORY = 64
+
+
+class IRParams(C.Structure):
+    _fields_ = [("tol", C.c_double), ("vcycles_per_iter", C.c_int), ("max_iter", C.c_int)]
+
+
+class IRResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("vcycles", C.c_int), ("converged", C.c_int),
+                ("r0_norm", C.c_double), ("rel_residual", C.c_double),
+                ("history", C.c_double * IR_HISTORY)]
+
+
 _vp = C.c_void_p
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int)
@@ -76,6 +89,12 @@ SIGNATURES = {
     "ubg_mg_rbgs": (C.c_int, [_vp, C.c_float, C.c_float, C.c_int]),
     "ubg_mg_residual": (C.c_int, [_vp, C.c_float, _fp, _dp]),
     "ubg_mg_sync": (C.c_int, [_vp]),
+    "ubg_ir_params_default": (None, [C.POINTER(IRParams)]),
+    "ubg_mg_ir_set_rhs": (C.c_int, [_vp, _fp]),
+    "ubg_mg_ir_set_guess": (C.c_int, [_vp, _dp]),
+    "ubg_mg_ir_solve": (C.c_int, [_vp, C.c_float, C.POINTER(IRParams), C.POINTER(IRResult)]),
+    "ubg_mg_ir_get_solution": (C.c_int, [_vp, _dp]),
+    "ubg_mg_ir_device_ptrs": (C.c_int, [_vp, C.POINTER(_dp), C.POINTER(_fp)]),
     "ubg_sim_create": (C.c_int, [_fp, C.c_int, C.c_int, C.c_float, C.c_float, C.POINTER(_vp)]),
     "ubg_sim_destroy": (C.c_int, [_vp]),
     "ubg_sim_set_params": (C.c_int, [_vp, C.POINTER(Params)]),

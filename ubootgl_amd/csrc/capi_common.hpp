@@ -1,0 +1,233 @@
+// capi_common.hpp -- internal to libubg.so: error plumbing, the device
+// memory pool, parameter checks and the MG handle (V-cycle driver) shared by
+// the C ABI translation units.
+#pragma once
+
+#include "ubg_internal.hpp"
+#include "../../include/ubg_c.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace ubg {
+
+
+inline thread_local std::string g_err; // ubg_last_error()
+
+inline int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define UBG_HIP(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(e_ == hipErrorOutOfMemory ? UBG_E_OOM : UBG_E_HIP,                   \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+  } while (0)
+
+#define UBG_TRY(expr)                                                                  \
+  do {                                                                                 \
+    int rc_ = (expr);                                                                  \
+    if (rc_ != UBG_OK) return rc_;                                                     \
+  } while (0)
+
+inline int check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(UBG_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return UBG_OK;
+}
+
+inline size_t align_up(size_t n, size_t a = 256) { return (n + a - 1) / a * a; }
+
+// levels with at least this many cells run the fused two-pass level kernels
+constexpr size_t kFuseMinCells = (size_t)1 << 22;
+
+// A simple bump allocator over one hipMalloc.
+struct Pool {
+  char *base = nullptr;
+  size_t size = 0, off = 0;
+  int init(size_t bytes) {
+    size = bytes;
+    UBG_HIP(hipMalloc((void **)&base, bytes));
+    UBG_HIP(hipMemset(base, 0, bytes));
+    return UBG_OK;
+  }
+  template <typename T> T *take(size_t n) {
+    T *p = (T *)(base + off);
+    off += align_up(n * sizeof(T));
+    return p;
+  }
+  void release() {
+    if (base) (void)hipFree(base);
+    base = nullptr;
+  }
+};
+
+inline void level_sizes(int W, int H, std::vector<int> &ws, std::vector<int> &hs) {
+  ws.clear();
+  hs.clear();
+  int cw = W, ch = H;
+  while (cw > 3 && ch > 3) { // pressure_solver.hpp:20-30
+    ws.push_back(cw);
+    hs.push_back(ch);
+    cw /= 2;
+    ch /= 2;
+  }
+}
+
+inline int validate_flag(const float *flag, size_t n, std::vector<uint8_t> &out) {
+  out.resize(n);
+  for (size_t i = 0; i < n; i++) {
+    float v = flag[i];
+    if (v == 0.0f) out[i] = 0;
+    else if (v == 1.0f) out[i] = 1;
+    else
+      return fail(UBG_E_FLAG_VALUE, "flag value at index " + std::to_string(i) + " is " +
+                                        std::to_string(v) + " (flags must be 0 or 1)");
+  }
+  return UBG_OK;
+}
+
+struct Timer {
+  bool on = false;
+  std::vector<hipEvent_t> ev; // pairs
+  size_t used = 0;
+  double total_ms = 0.0;
+  long launches = 0;
+};
+
+inline void default_params(ubg_params *p) {
+  p->nu_pre = 3;
+  p->nu_post = 3;
+  p->nu_coarse = 5;
+  p->vcycles_per_step = 2;
+  p->alpha = 1.0f;
+  p->use_graph = 1;
+  p->save_current = 1;
+}
+
+inline int check_params(const ubg_params *p) {
+  if (!p) return fail(UBG_E_INVALID, "null params");
+  if (p->nu_pre < 0 || p->nu_post < 0 || p->nu_coarse < 0 || p->vcycles_per_step < 0 ||
+      p->nu_pre > 64 || p->nu_post > 64 || p->nu_coarse > 256 || p->vcycles_per_step > 1024)
+    return fail(UBG_E_INVALID, "smoothing counts out of range");
+  if (!(p->alpha > 0.0f && p->alpha < 2.0f)) return fail(UBG_E_INVALID, "alpha must be in (0,2)");
+  return UBG_OK;
+}
+
+
+} // namespace ubg
+
+using namespace ubg;
+
+struct IRState; // FP64 iterative refinement buffers (refine.hip)
+
+// ===========================================================================
+// MG
+// ===========================================================================
+struct ubg_mg {
+  int W = 0, H = 0, levels = 0;
+  std::vector<Level> lv;
+  Pool pool;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  ubg_params prm{};
+  double *d_acc = nullptr;
+  IRState *ir = nullptr;   // allocated by the first refinement solve
+  Timer *timers = nullptr; // borrowed from the owning simulation (UBG_TIMER_COUNT)
+  std::recursive_mutex mu;
+
+  // run `fn` (one kernel launch) bracketed by HIP events on this stream when
+  // timer `id` is enabled and the work is on level 0
+  template <typename Fn> void timed(int id, int l, Fn fn) {
+    Timer *t = timers ? &timers[id] : nullptr;
+    bool on = t && t->on && l == 0;
+    if (on) {
+      if (t->used + 2 > t->ev.size()) {
+        size_t n = t->ev.size();
+        t->ev.resize(n + 512);
+        for (size_t i = n; i < t->ev.size(); i++) (void)hipEventCreate(&t->ev[i]);
+      }
+      (void)hipEventRecord(t->ev[t->used], stream);
+    }
+    fn();
+    if (on) {
+      (void)hipEventRecord(t->ev[t->used + 1], stream);
+      t->used += 2;
+    }
+  }
+
+  // n sweeps at level l, one launch each
+  void sweeps(int l, int n, float h, bool zg) {
+    Level &L = lv[l];
+    for (int i = 0; i < n; i++)
+      timed(UBG_TIMER_SMOOTH_L0, l, [&] { launch_rbgs_fused(L, h, prm.alpha, zg, stream); });
+  }
+
+  // MG::solveLevel (pressure_solver.cpp:201-248).  A level costs two fused
+  // passes: (pre-smoothing, residual, restriction) and (prolongation,
+  // correction, post-smoothing); setZeroGradientBC happens inside them.
+  void solve_level(int l, float h, bool zg) {
+    const bool z = zg && l == 0;
+    if (l == levels - 2) {
+      sweeps(l, prm.nu_coarse, h, false); // no ZG on the coarsest solved level
+      return;
+    }
+    if (l > 0 && coarse_fits_lds(lv.data(), l, levels)) {
+      launch_coarse_vcycle(lv.data(), l, levels, h, prm.nu_pre, prm.nu_post, prm.nu_coarse,
+                           prm.alpha, stream);
+      return;
+    }
+    Level &L = lv[l], &C = lv[l + 1];
+    if ((size_t)L.w * L.h < kFuseMinCells) {
+      // small level: few waves, so short per-wave work wins over fewer passes
+      sweeps(l, prm.nu_pre, h, z);
+      launch_residual_restrict(L, C, h, stream);
+      float hc = h * (L.w - 1.0f) / (C.w - 1.0f); // pressure_solver.cpp:229
+      solve_level(l + 1, hc, false);
+      launch_prolong_correct(L, C, stream);
+      if (z) launch_zero_gradient(L.p, L.w, L.h, stream);
+      sweeps(l, prm.nu_post, h, z);
+      return;
+    }
+    // the last pre-smoothing sweep and the first post-smoothing sweep ride in
+    // the grid-transfer passes
+    const int pre_fused = std::min(prm.nu_pre, 1);
+    sweeps(l, prm.nu_pre - pre_fused, h, z);
+    timed(UBG_TIMER_L0_PRE, l, [&] { launch_presmooth_restrict(L, C, pre_fused, h, prm.alpha, z, stream); });
+    float hc = h * (L.w - 1.0f) / (C.w - 1.0f); // pressure_solver.cpp:229
+    solve_level(l + 1, hc, false);
+    const int post_fused = std::min(prm.nu_post, 1);
+    timed(UBG_TIMER_L0_POST, l, [&] { launch_prolong_postsmooth(L, C, post_fused, h, prm.alpha, z, stream); });
+    sweeps(l, prm.nu_post - post_fused, h, z);
+  }
+
+  // coarse flags of levels 1..L-1 from the level-0 flag (updateFields)
+  void classify() {
+    launch_build_mask(lv[0].flag, lv[0].mask, lv[0].w, lv[0].h, stream);
+    for (int l = 1; l < levels; l++) {
+      launch_coarsen(lv[l - 1].flag, lv[l - 1].w, lv[l].flag, lv[l].w, lv[l].h, stream);
+      launch_build_mask(lv[l].flag, lv[l].mask, lv[l].w, lv[l].h, stream);
+    }
+  }
+};
+
+namespace ubg {
+
+// Build an MG hierarchy.  If level-0 buffers are given they are borrowed
+// (the simulation's p, f, flag, mask); otherwise the MG owns them.
+int mg_create(int W, int H, hipStream_t stream, float *p0, float *p0_alt, float *f0,
+              uint8_t *flag0, uint8_t *mask0, ubg_mg **out);
+int mg_upload_flag(ubg_mg *m, const float *flag);
+// releases the refinement buffers of an MG handle (refine.hip)
+void ir_release(ubg_mg *m);
+
+} // namespace ubg

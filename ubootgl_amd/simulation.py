@@ -135,6 +135,36 @@ class MG:
     def sync(self):
         check(lib().ubg_mg_sync(self._h))
 
+    # FP64 iterative refinement around FP32 V-cycles (extension) --------------
+    def ir_set_rhs(self, f):
+        """float32 right-hand side of the refinement solve (H x W)."""
+        check(lib().ubg_mg_ir_set_rhs(self._h, fptr(_f32(f))))
+
+    def ir_set_guess(self, x=None):
+        """float64 initial guess incl. the Dirichlet ring; None = zeros."""
+        if x is None:
+            check(lib().ubg_mg_ir_set_guess(self._h, None))
+            return
+        x = np.ascontiguousarray(x, np.float64)
+        check(lib().ubg_mg_ir_set_guess(self._h, x.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def ir_solve(self, h, tol=1e-10, vcycles_per_iter=4, max_iter=200, raise_on_stall=True):
+        """Refine to ||r|| <= tol ||r0||; returns a dict of the ubg_ir_result."""
+        prm = L.IRParams(tol, vcycles_per_iter, max_iter)
+        res = L.IRResult()
+        rc = lib().ubg_mg_ir_solve(self._h, h, C.byref(prm), C.byref(res))
+        if rc != L.E_NOT_CONVERGED or raise_on_stall:
+            check(rc)
+        n = min(res.iterations, L.IR_HISTORY)
+        return dict(iterations=res.iterations, vcycles=res.vcycles,
+                    converged=bool(res.converged), r0_norm=res.r0_norm,
+                    rel_residual=res.rel_residual, history=list(res.history[:n]))
+
+    def ir_solution(self):
+        out = np.zeros(self._level_shape(0), np.float64)
+        check(lib().ubg_mg_ir_get_solution(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
     def close(self):
         if getattr(self, "_h", None) is not None and not self._borrowed:
             lib().ubg_mg_destroy(self._h)

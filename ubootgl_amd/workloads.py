@@ -114,10 +114,12 @@ def config0(seed=0):
     return Domain(flag, vx, vy)
 
 
-def disks_to_fraction(W, H, g, ndisks, rmin, rmax, target, tol=0.0005):
+def disks_to_fraction(W, H, g, ndisks, rmin, rmax, target, tol=0.0005, base=None):
     """`ndisks` disks with log-uniform base radii in [rmin, rmax]; the scale s
     in r = clamp(s*rho, rmin, rmax) is bisected until the rasterised solid
-    fraction (interior cells) is within tol of target."""
+    fraction (interior cells) is within tol of target.  `base` makes the
+    starting flag field (default: walls)."""
+    base = base or walls
     cx = g.uniform(0, W, ndisks)
     cy = g.uniform(0, H, ndisks)
     rho = np.exp(g.uniform(math.log(rmin), math.log(rmax), ndisks))
@@ -126,7 +128,7 @@ def disks_to_fraction(W, H, g, ndisks, rmin, rmax, target, tol=0.0005):
     for _ in range(40):
         s = 0.5 * (lo + hi)
         r = np.clip(s * rho, rmin, rmax)
-        fl = stamp_disks(walls(W, H), cx, cy, r)
+        fl = stamp_disks(base(W, H), cx, cy, r)
         fr = solid_fraction(fl)
         best = fl
         if abs(fr - target) <= tol:
@@ -170,3 +172,79 @@ def small_domain(W, H, seed, ndisks=3, nmodes=4):
                     g.uniform(1.5, rmax, ndisks))
     vx, vy = velocity(flag, g, nmodes, 4)
     return Domain(flag, vx, vy)
+
+
+# ---------------------------------------------------------------------------
+# Standalone projection solves (mixed-precision, batched and served solves)
+# ---------------------------------------------------------------------------
+
+def open_ring(W, H):
+    """All fluid: the outer ring holds Dirichlet values, as in mgtest
+    (mgtest/mgtest.cpp:14-29)."""
+    return np.ones((H, W), np.float64)
+
+
+def fill_enclosed(flag):
+    """Make solid every fluid cell with no 4-connected fluid path to the
+    outer ring: such a pocket would be a pure-Neumann (singular) problem."""
+    from scipy import ndimage
+    lab, _ = ndimage.label(flag > 0.5)  # 4-connectivity, the stencil's
+    ring = np.unique(np.concatenate([lab[0], lab[-1], lab[:, 0], lab[:, -1]]))
+    keep = np.isin(lab, ring[ring > 0])
+    return np.where(keep, 1.0, 0.0)
+
+
+def divergence_rhs(flag, vx, vy, h):
+    """f = -(1/h)(vx(x) - vx(x-1) + vy(y) - vy(y-1)) on interior fluid cells
+    (the divergence of simulation.cpp:166-171), 0 on the ring and solids;
+    float64."""
+    H, W = flag.shape
+    f = np.zeros((H, W), np.float64)
+    f[1:-1, 1:-1] = -(vx[1:-1, 1:] - vx[1:-1, :-1] + vy[1:, 1:-1] - vy[:-1, 1:-1]) / h
+    return f * flag
+
+
+class PoissonProblem:
+    """One standalone projection solve: flag (1 = fluid; ring fluid and held
+    at x = 0), float32 rhs f, grid spacing h = 1/W."""
+
+    def __init__(self, flag, f, h):
+        self.flag = np.ascontiguousarray(flag, np.float32)
+        self.H, self.W = self.flag.shape
+        self.f = np.ascontiguousarray(f, np.float32)
+        self.h = np.float32(h)
+
+    @property
+    def cells(self):
+        return self.W * self.H
+
+
+def poisson_problem(W, H, seed, solid=0.03, ndisks=None, rmin=None, rmax=None, nmodes=16,
+                    kmax=6):
+    """Seeded projection solve on a W x H grid: disks (log-uniform radii in
+    [rmin, rmax], scaled to the `solid` fraction), enclosed pockets filled,
+    f = -div of a face-masked `nmodes`-mode velocity field."""
+    g = rng(seed)
+    if ndisks is None:
+        ndisks = max(16, W * H // 20000)
+    rmin = rmin if rmin is not None else max(1.5, min(W, H) / 256)
+    rmax = rmax if rmax is not None else max(2 * rmin, min(W, H) / 8)
+    if solid > 0:
+        flag = disks_to_fraction(W, H, g, ndisks, rmin, rmax, solid,
+                                 tol=max(0.0005, 0.05 * solid), base=open_ring)
+    else:
+        flag = open_ring(W, H)
+    flag[0, :] = flag[-1, :] = 1.0
+    flag[:, 0] = flag[:, -1] = 1.0
+    flag = fill_enclosed(flag)
+    h = np.float32(1.0 / W)
+    vx, vy = velocity(flag, g, nmodes, kmax)
+    f = divergence_rhs(flag, vx, vy, float(h))
+    return PoissonProblem(flag, f, h)
+
+
+def config3(seed=3):
+    """8192x4096 mixed-precision solve: 800 disks (radius 8..128) scaled to
+    3% solid, rhs = -div of a 16-mode field."""
+    return poisson_problem(8192, 4096, seed, solid=0.03, ndisks=800, rmin=8.0, rmax=128.0,
+                           nmodes=16)
