@@ -182,6 +182,44 @@ UBG_API int ubg_mg_ir_get_solution(ubg_mg *mg, double *x);
 /* device addresses of the resident float64 solution and float32 rhs */
 UBG_API int ubg_mg_ir_device_ptrs(ubg_mg *mg, double **x, float **f);
 
+/* ---- Batched solves -------------------------------------------------------
+ * No counterpart in the reference: n independent refinement solves (the
+ * problem of ubg_mg_ir_solve) on domains of uneven sizes, advanced together.
+ * Every multigrid stage is one launch over a tile list spanning all domains
+ * (sweeps, residual+restriction, prolongation+correction; the coarse levels
+ * of each domain in one LDS workgroup), and a refinement step of all domains
+ * (V-cycles, FP64 update, per-domain norm and convergence test) is replayed as
+ * a captured hipGraph; a converged domain drops out on the device.  Each
+ * domain's result is bit-identical to ubg_mg_ir_solve on that domain alone.
+ * Domains need W, H >= 16. */
+typedef struct ubg_batch ubg_batch;
+typedef struct ubg_batch_tuning {
+  int tile_w, tile_h; /* smoother tile (cells), one of the compiled shapes */
+  int blocks;         /* workgroups per launch (0: one per tile)            */
+  double sweep_ms;    /* measured time of the tuning pass (output)          */
+} ubg_batch_tuning;
+/* device bytes a batch of these domains allocates */
+UBG_API int ubg_batch_bytes(int n, const int *widths, const int *heights, size_t *bytes);
+UBG_API int ubg_batch_create(int n, const int *widths, const int *heights, const ubg_params *prm,
+                             ubg_batch **out);
+UBG_API int ubg_batch_destroy(ubg_batch *b);
+/* domain d: flag (W x H, 0/1; the outer ring must be fluid), float32 rhs, h */
+UBG_API int ubg_batch_set_domain(ubg_batch *b, int d, const float *flag, const float *rhs, float h);
+UBG_API int ubg_batch_set_rhs(ubg_batch *b, int d, const float *rhs);
+/* float64 initial guess incl. the Dirichlet ring; NULL = 0 */
+UBG_API int ubg_batch_set_guess(ubg_batch *b, int d, const double *x);
+/* solve every domain to prm->tol (relative to its own initial residual);
+ * UBG_E_NOT_CONVERGED if any domain stopped short (see ubg_batch_result) */
+UBG_API int ubg_batch_solve(ubg_batch *b, const ubg_ir_params *prm, int *graph_replays);
+UBG_API int ubg_batch_result(ubg_batch *b, int d, ubg_ir_result *res);
+UBG_API int ubg_batch_get_solution(ubg_batch *b, int d, double *x);
+/* time the compiled smoother tile shapes and workgroup counts on this batch
+ * and keep the fastest; with db_path, reuse / append the choice keyed by the
+ * batch's size signature (one "key tile_w tile_h blocks ms" line each) */
+UBG_API int ubg_batch_autotune(ubg_batch *b, const char *db_path, ubg_batch_tuning *out);
+UBG_API int ubg_batch_set_tuning(ubg_batch *b, const ubg_batch_tuning *t);
+UBG_API int ubg_batch_get_tuning(ubg_batch *b, ubg_batch_tuning *t);
+
 /* ---- Simulation: simulation.hpp:18-141 ---------------------------------- */
 /* Simulation(flag, pwidth, mu) (simulation.hpp:32-67): BCs S/N no-slip,
  * W inflow, E zero-pressure outflow; vx(0,y)=1, vy(x,0)=0 in both buffers;

@@ -62,6 +62,11 @@ class IRResult(C.Structure):
                 ("history", C.c_double * IR_HISTORY)]
 
 
+class BatchTuning(C.Structure):
+    _fields_ = [("tile_w", C.c_int), ("tile_h", C.c_int), ("blocks", C.c_int),
+                ("sweep_ms", C.c_double)]
+
+
 _vp = C.c_void_p
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int)
@@ -95,6 +100,18 @@ SIGNATURES = {
     "ubg_mg_ir_solve": (C.c_int, [_vp, C.c_float, C.POINTER(IRParams), C.POINTER(IRResult)]),
     "ubg_mg_ir_get_solution": (C.c_int, [_vp, _dp]),
     "ubg_mg_ir_device_ptrs": (C.c_int, [_vp, C.POINTER(_dp), C.POINTER(_fp)]),
+    "ubg_batch_bytes": (C.c_int, [C.c_int, _ip, _ip, C.POINTER(C.c_size_t)]),
+    "ubg_batch_create": (C.c_int, [C.c_int, _ip, _ip, C.POINTER(Params), C.POINTER(_vp)]),
+    "ubg_batch_destroy": (C.c_int, [_vp]),
+    "ubg_batch_set_domain": (C.c_int, [_vp, C.c_int, _fp, _fp, C.c_float]),
+    "ubg_batch_set_rhs": (C.c_int, [_vp, C.c_int, _fp]),
+    "ubg_batch_set_guess": (C.c_int, [_vp, C.c_int, _dp]),
+    "ubg_batch_solve": (C.c_int, [_vp, C.POINTER(IRParams), _ip]),
+    "ubg_batch_result": (C.c_int, [_vp, C.c_int, C.POINTER(IRResult)]),
+    "ubg_batch_get_solution": (C.c_int, [_vp, C.c_int, _dp]),
+    "ubg_batch_autotune": (C.c_int, [_vp, C.c_char_p, C.POINTER(BatchTuning)]),
+    "ubg_batch_set_tuning": (C.c_int, [_vp, C.POINTER(BatchTuning)]),
+    "ubg_batch_get_tuning": (C.c_int, [_vp, C.POINTER(BatchTuning)]),
     "ubg_sim_create": (C.c_int, [_fp, C.c_int, C.c_int, C.c_float, C.c_float, C.POINTER(_vp)]),
     "ubg_sim_destroy": (C.c_int, [_vp]),
     "ubg_sim_set_params": (C.c_int, [_vp, C.POINTER(Params)]),

@@ -1,0 +1,212 @@
+// batch_engine.hpp -- internal to libubg.so: the engine that advances many
+// independent refinement solves together (batched and served solves).
+//
+// Layout: every domain owns, from a device arena, the float64 solution x
+// (two buffers), its float32 rhs, the FP32 V-cycle buffers of the levels down
+// to the first one solved in LDS (p twice, f) and its solve state; the flag
+// and mask pyramid (a "geometry") can be shared by domains of the same shape.
+// The engine keeps device tables of the current member domains and, per
+// step, a row of per-domain buffer indices for every stage, so that one
+// launch of a stage serves every domain whatever its level count.
+#pragma once
+
+#include "capi_common.hpp"
+#include "coarse_device.hpp"
+
+#include <map>
+#include <memory>
+
+namespace ubg {
+
+constexpr int BMAXL = 16;        // levels per domain at most
+constexpr int B_UPD_W = 256;     // update tile: 256 x 8 cells, as the
+constexpr int B_UPD_H = 8;       //   single-domain refinement's blocks
+constexpr int B_MIN_SIZE = 16;
+
+// solve state of one domain (device; written by the norm kernel)
+struct DState {
+  double r0, rel, best, tol;
+  int active, converged, iters, vcycles, stall, max_iter, xcur, k;
+  int nonfinite, pad;
+  double hist[UBG_IR_HISTORY];
+};
+
+// device table rows
+struct BLev {
+  float *p[2];
+  float *f;
+  const uint8_t *mask;
+  int w, h;
+  float hh, ihsq;
+};
+struct BDom {
+  DState *st;
+  double *x[2];
+  const float *rhs;
+  double *partial;
+  int w, h, gx, gy, nlev, cfirst;
+};
+struct Tile {
+  int d, x0, y0, i;
+};
+
+// first-fit device sub-allocator over one hipMalloc (256-byte granules)
+class Arena {
+public:
+  int init(size_t bytes);
+  void release();
+  char *alloc(size_t bytes); // nullptr if it does not fit
+  void free(char *p);
+  size_t capacity() const { return cap_; }
+  size_t used() const { return used_; }
+  char *base() const { return base_; }
+
+private:
+  char *base_ = nullptr;
+  size_t cap_ = 0, used_ = 0;
+  std::map<size_t, size_t> free_; // offset -> size
+  std::map<size_t, size_t> live_; // offset -> size
+};
+
+// flag/mask pyramid of one domain shape (device, read-only once built)
+struct BGeom {
+  int W = 0, H = 0, nlev = 0, cfirst = 0;
+  int w[BMAXL] = {}, h[BMAXL] = {};
+  uint8_t *flag[BMAXL] = {}, *mask[BMAXL] = {};
+  char *mem = nullptr;
+  size_t bytes = 0;
+  size_t lds_bytes = 0;
+  int refs = 0;
+};
+
+struct BDomain {
+  BGeom *g = nullptr;
+  float h = 0.0f;
+  float hh[BMAXL] = {};
+  float *p[BMAXL][2] = {};
+  float *f[BMAXL] = {};
+  double *x[2] = {};
+  float *rhs = nullptr;
+  double *partial = nullptr;
+  DState *st = nullptr;
+  int gx = 0, gy = 0;
+  char *mem = nullptr;
+  size_t bytes = 0;
+  // host copy of the buffer indices (which p of each level, which x is current)
+  int cur[BMAXL] = {};
+  int xcur = 0;
+};
+
+struct Tuning {
+  int tx = 64, ty = 8, blocks = 0;
+};
+
+// Level sizes, first LDS level and byte counts of a W x H domain.
+struct BShape {
+  int nlev = 0, cfirst = 0;
+  int w[BMAXL] = {}, h[BMAXL] = {};
+  size_t lds_bytes = 0, geom_bytes = 0, dom_bytes = 0;
+};
+int batch_shape(int W, int H, BShape *s);
+
+class BatchEngine {
+public:
+  ~BatchEngine();
+  int init(hipStream_t s, const ubg_params &prm, int vcycles_per_step);
+  hipStream_t stream() const { return stream_; }
+  const ubg_params &params() const { return prm_; }
+  int vcycles() const { return k_; }
+  void set_vcycles(int k) {
+    if (k != k_) drop_graphs();
+    k_ = k;
+  }
+
+  // geometry / domain lifetime (memory from `arena`)
+  int geom_create(Arena &arena, int W, int H, const float *flag_host, BGeom **out);
+  void geom_release(Arena &arena, BGeom *g);
+  int domain_create(Arena &arena, BGeom *g, float h, BDomain **out);
+  void domain_release(Arena &arena, BDomain *d);
+  int domain_set_rhs(BDomain *d, const float *rhs_host);
+  int domain_set_guess(BDomain *d, const double *x_host);
+  // reset a domain's solve state: active, tol/max_iter, fresh (next update
+  // computes r0 and does not add e)
+  int domain_start(BDomain *d, double tol, int max_iter);
+  int domain_read_state(BDomain *d, DState *out);
+  int domain_get_solution(BDomain *d, double *x_host);
+
+  // the member set of the following steps: `running` domains take V-cycles
+  // and an accumulating update; `fresh` ones only their first update
+  int set_members(const std::vector<BDomain *> &running, const std::vector<BDomain *> &fresh);
+  // enqueue `nsteps` refinement steps of the members; the active-domain
+  // count lands in *count_host (pinned) when the stream reaches it
+  int enqueue_steps(int nsteps);
+  // the first update (r0, no V-cycles) of every member, as if all were fresh
+  int enqueue_first_update();
+  // eager steps, or one captured graph per index state, replayed
+  int run_steps_graph(int nsteps);
+  int active_count() const { return *count_host_; }
+  // members whose solve was still active at the last completed step
+  std::vector<BDomain *> active_members() const;
+  size_t members() const { return all_.size(); }
+  int sync();
+
+  void set_tuning(const Tuning &t);
+  const Tuning &tuning() const { return tune_; }
+  // time each compiled smoother tile shape and block count over the member
+  // levels; keeps the fastest
+  int autotune(double *best_ms);
+
+private:
+  struct StageRec {
+    int kind, level;
+    size_t off;
+  };
+  int build_tables();
+  int build_tiles();
+  void gen_steps(int nsteps, std::vector<StageRec> &st, std::vector<uint8_t> &idx,
+                 bool all_fresh = false);
+  int launch_stages(const std::vector<StageRec> &st, const uint8_t *d_idx);
+  int upload_idx(const std::vector<uint8_t> &idx, uint8_t **d_idx);
+  int launch_sweeps_only(int reps);
+  std::string state_key() const;
+
+  hipStream_t stream_ = nullptr;
+  ubg_params prm_{};
+  int k_ = 4;
+  Tuning tune_{};
+  std::vector<BDomain *> run_, fresh_, all_;
+  int lmax_ = 0;
+  // device tables
+  BLev *d_lev_ = nullptr;
+  BDom *d_dom_ = nullptr;
+  CoarseArgs *d_cargs_ = nullptr;
+  size_t tab_cap_ = 0;
+  size_t coarse_lds_ = 0;
+  // tile lists: per level sweep / rr / pc (running members), update (all)
+  std::vector<Tile> h_tiles_;
+  Tile *d_tiles_ = nullptr;
+  size_t tiles_cap_ = 0;
+  struct Span {
+    size_t off = 0;
+    int n = 0;
+  };
+  Span sw_[BMAXL], rr_[BMAXL], pc_[BMAXL], upd_;
+  // stage index rows
+  uint8_t *d_idx_ = nullptr;
+  size_t idx_cap_ = 0;
+  uint8_t *h_idx_pinned_ = nullptr;
+  size_t h_idx_cap_ = 0;
+  int *d_count_ = nullptr;
+  int *count_host_ = nullptr;
+  uint8_t *d_active_ = nullptr; // per slot, written with the count
+  uint8_t *h_active_ = nullptr; // pinned copy
+  // graphs by index state (batched solves with a fixed member set)
+  struct Graph {
+    hipGraphExec_t exec = nullptr;
+    uint8_t *d_idx = nullptr;
+  };
+  std::map<std::string, Graph> graphs_;
+  void drop_graphs();
+};
+
+} // namespace ubg

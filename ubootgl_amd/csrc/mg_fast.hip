@@ -18,7 +18,7 @@
 //    workgroup entirely in LDS (p, f and mask of all those levels fit in
 //    160 KB once the level is <= ~128x64).  Replaces ~40 latency-bound
 //    launches per V-cycle with one.
-#include "mg_device.hpp"
+#include "coarse_device.hpp"
 
 namespace ubg {
 
@@ -170,149 +170,8 @@ __global__ void __launch_bounds__(TPB_T)
 
 // ---------------------------------------------------------------------------
 // single-workgroup V-cycle in LDS
-constexpr int MAXL = 16;
-constexpr int TPB_C = 1024;
-constexpr size_t LDS_BUDGET = 152 * 1024;
-
-struct CoarseArgs {
-  int n; // levels first .. L-2
-  int w[MAXL], h[MAXL];
-  float hh[MAXL];
-  int op[MAXL], of[MAXL], om[MAXL]; // LDS byte offsets of p, f, mask
-  const uint8_t *mask[MAXL];
-  const float *f0;
-  float *p0;
-  int nu_pre, nu_post, nu_coarse;
-  float alpha;
-};
-
-// 1024 threads as 16 rows x 64 lanes: rows strided by 16, columns by 64
-// (no integer division in the loops)
-__device__ __forceinline__ void lds_sweep(float *P, const float *Fv, const uint8_t *M, int w,
-                                          int h, float hh, float alpha) {
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  for (int color = 0; color < 2; color++) {
-    for (int y = 1 + ty; y < h - 1; y += 16) {
-      for (int x = 1 + ((y + color) & 1) + 2 * tx; x < w - 1; x += 128) {
-        int i = y * w + x;
-        P[i] = smooth(P[i], P[i - 1], P[i + 1], P[i - w], P[i + w], Fv[i], M[i], hh, alpha);
-      }
-    }
-    __syncthreads();
-  }
-}
-
-__device__ __forceinline__ float lds_residual(const float *P, const float *Fv, const uint8_t *M,
-                                              int i, int w, float ihsq) {
-  return residual_of(P[i], P[i - 1], P[i + 1], P[i - w], P[i + w], Fv[i], M[i], ihsq);
-}
-
 __global__ void __launch_bounds__(TPB_C) k_coarse_vcycle(CoarseArgs a) {
-  extern __shared__ __align__(16) char smem[];
-  const int tid = threadIdx.x;
-  // load: f of the first level, masks of all levels; p = 0 everywhere
-  for (int l = 0; l < a.n; l++) {
-    float *P = (float *)(smem + a.op[l]);
-    float *Fv = (float *)(smem + a.of[l]);
-    uint8_t *M = (uint8_t *)(smem + a.om[l]);
-    int n = a.w[l] * a.h[l];
-    for (int i = tid; i < n; i += TPB_C) {
-      P[i] = 0.0f;
-      Fv[i] = (l == 0) ? a.f0[i] : 0.0f;
-      M[i] = a.mask[l][i];
-    }
-  }
-  __syncthreads();
-
-  // descend
-  for (int l = 0; l < a.n - 1; l++) {
-    float *P = (float *)(smem + a.op[l]);
-    float *Fv = (float *)(smem + a.of[l]);
-    const uint8_t *M = (const uint8_t *)(smem + a.om[l]);
-    const int w = a.w[l], h = a.h[l];
-    for (int s = 0; s < a.nu_pre; s++) lds_sweep(P, Fv, M, w, h, a.hh[l], a.alpha);
-    // residual + restrict into level l+1 (its p starts at 0)
-    float *Fc = (float *)(smem + a.of[l + 1]);
-    float *Pc = (float *)(smem + a.op[l + 1]);
-    const int wc = a.w[l + 1], hc = a.h[l + 1];
-    const float ihsq = 1.0f / a.hh[l] / a.hh[l];
-    for (int y = 1 + (tid >> 6); y < hc - 1; y += 16)
-    for (int x = 1 + (tid & 63); x < wc - 1; x += 64) {
-      int c = (2 * y) * w + 2 * x;
-      auto R = [&](int dx, int dy) { return lds_residual(P, Fv, M, c + dy * w + dx, w, ihsq); };
-      float v = 0.0f;
-      v = R(-1, -1) * 1.0f + R(0, -1) * 2.0f + R(1, -1) * 1.0f;
-      v += R(-1, 0) * 2.0f + R(0, 0) * 4.0f + R(1, 0) * 2.0f;
-      v += R(-1, 1) * 1.0f + R(0, 1) * 2.0f + R(1, 1) * 1.0f;
-      Fc[y * wc + x] = v * (1.0f / 16.0f);
-      Pc[y * wc + x] = 0.0f;
-    }
-    __syncthreads();
-  }
-  // coarsest solved level
-  {
-    int l = a.n - 1;
-    float *P = (float *)(smem + a.op[l]);
-    float *Fv = (float *)(smem + a.of[l]);
-    const uint8_t *M = (const uint8_t *)(smem + a.om[l]);
-    for (int s = 0; s < a.nu_coarse; s++) lds_sweep(P, Fv, M, a.w[l], a.h[l], a.hh[l], a.alpha);
-  }
-  // ascend
-  for (int l = a.n - 2; l >= 0; l--) {
-    float *P = (float *)(smem + a.op[l]);
-    float *Fv = (float *)(smem + a.of[l]);
-    const uint8_t *M = (const uint8_t *)(smem + a.om[l]);
-    const int w = a.w[l], h = a.h[l];
-    const float *Pc = (const float *)(smem + a.op[l + 1]);
-    const uint8_t *Mc = (const uint8_t *)(smem + a.om[l + 1]);
-    const int wc = a.w[l + 1];
-    for (int y = 1 + (tid >> 6); y < h - 1; y += 16)
-    for (int x = 1 + (tid & 63); x < w - 1; x += 64) {
-      int i = y * w + x;
-      float F = (M[i] & MK_SELF) ? 1.0f : 0.0f;
-      int ax = x >> 1, by = y >> 1;
-      bool xe = !(x & 1), ye = !(y & 1);
-      auto E = [&](int xx, int yy) { return Pc[yy * wc + xx]; };
-      auto FC = [&](int xx, int yy) { return (int)(Mc[yy * wc + xx] & MK_SELF); };
-      float e = 0.0f;
-      if (xe && ye) {
-        e = E(ax, by) * F;
-      } else if (!xe && ye) {
-        if (x <= w - 3) e = F * (E(ax, by) + E(ax + 1, by)) / eps_count(FC(ax, by) + FC(ax + 1, by));
-      } else if (xe && !ye) {
-        if (y <= h - 3) e = F * (E(ax, by) + E(ax, by + 1)) / eps_count(FC(ax, by) + FC(ax, by + 1));
-      } else {
-        if (x <= w - 3 && y <= h - 3)
-          e = F * (E(ax, by) + E(ax + 1, by + 1) + E(ax + 1, by) + E(ax, by + 1)) /
-              eps_count(FC(ax, by) + FC(ax + 1, by + 1) + FC(ax + 1, by) + FC(ax, by + 1));
-      }
-      P[i] += e;
-    }
-    __syncthreads();
-    for (int s = 0; s < a.nu_post; s++) lds_sweep(P, Fv, M, w, h, a.hh[l], a.alpha);
-  }
-  // write the first level's solution
-  {
-    const float *P = (const float *)(smem + a.op[0]);
-    int n = a.w[0] * a.h[0];
-    for (int i = tid; i < n; i += TPB_C) a.p0[i] = P[i];
-  }
-}
-
-size_t coarse_lds_bytes(const Level *lv, int first, int levels, CoarseArgs *a) {
-  size_t off = 0;
-  int k = 0;
-  for (int l = first; l <= levels - 2; l++, k++) {
-    if (k >= MAXL) return (size_t)-1;
-    size_t n = (size_t)lv[l].w * lv[l].h;
-    if (a) {
-      a->op[k] = (int)off;
-      a->of[k] = (int)(off + ((n * 4 + 15) & ~size_t(15)));
-      a->om[k] = (int)(off + 2 * ((n * 4 + 15) & ~size_t(15)));
-    }
-    off += 2 * ((n * 4 + 15) & ~size_t(15)) + ((n + 15) & ~size_t(15));
-  }
-  return off;
+  coarse_vcycle_body(a, a.f0, a.p0);
 }
 
 } // namespace

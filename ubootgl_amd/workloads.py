@@ -248,3 +248,28 @@ def config3(seed=3):
     3% solid, rhs = -div of a 16-mode field."""
     return poisson_problem(8192, 4096, seed, solid=0.03, ndisks=800, rmin=8.0, rmax=128.0,
                            nmodes=16)
+
+
+def batch_sizes(n, seed, wmin=128, wmax=2048):
+    """Domain sizes of the batched workload: W = 32 round(exp(U[ln wmin,
+    ln wmax]) / 32), aspect a ~ U[1, 4], H = max(32, 32 round(W / a / 32))."""
+    g = rng(seed)
+    out = []
+    for _ in range(n):
+        W = int(32 * round(math.exp(g.uniform(math.log(wmin), math.log(wmax))) / 32))
+        a = g.uniform(1.0, 4.0)
+        H = max(32, int(32 * round(W / a / 32)))
+        out.append((W, H))
+    return out
+
+
+def config2(n=256, seed=2):
+    """256 independent solve domains of uneven size (batch_sizes), each with
+    U[0, 8%] disk obstacles and its own 8-mode velocity field."""
+    g = rng(seed)
+    sizes = batch_sizes(n, seed)
+    probs = []
+    for i, (W, H) in enumerate(sizes):
+        solid = float(g.uniform(0.0, 0.08))
+        probs.append(poisson_problem(W, H, seed=10_000 * seed + i, solid=solid, nmodes=8))
+    return probs
