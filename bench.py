@@ -50,9 +50,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--workload", choices=["step", "ir", "batch"], default="step",
+    ap.add_argument("--workload", choices=["step", "ir", "batch", "serve"], default="step",
                     help="step: the 4096x2048 fluid step (default); ir: the 8192x4096 "
-                         "FP64-refined solve to 1e-10; batch: 256 uneven domains to 1e-6")
+                         "FP64-refined solve to 1e-10; batch: 256 uneven domains to 1e-6; "
+                         "serve: Poisson arrivals at --rate of 3-timestep solve requests")
+    ap.add_argument("--rate", type=float, default=2000.0)
+    ap.add_argument("--duration", type=float, default=1.0)
+    ap.add_argument("--templates", type=int, default=48)
     ap.add_argument("--domains", type=int, default=256)
     args = ap.parse_args()
 
@@ -73,6 +77,8 @@ def main():
         return bench_ir(args)
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     t0 = time.time()
     if (args.width, args.height) == (4096, 2048):
@@ -304,6 +310,68 @@ def bench_batch(args):
                    "sequential_solves_per_s": round(sequential, 3),
                    "speedup_vs_sequential": round(batched / sequential, 2),
                    "algorithmic_GBps": round(alg / (el / steps) / 1e9, 1)},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_serve(args):
+    """Requests arrive as a Poisson stream (seed 7) at --rate per second for
+    --duration seconds; each is 3 consecutive projection solves (to 1e-6
+    relative to ||F f||, warm-started) on one of --templates domains (W
+    log-uniform 64..4096, H log-uniform 64..2048, multiples of 32, 0-8% disks,
+    8-mode velocity whose phases advance per timestep), drawn uniformly.  The
+    server admits by a 200 GB HBM budget and batches continuously.
+    value = solves/s over the run; p50/p99 = submit-to-done latency."""
+    import numpy as np
+
+    from ubootgl_amd import serve as S
+    from ubootgl_amd import workloads
+
+    t0 = time.time()
+    tpls = workloads.serving_templates(args.templates, seed=7)
+    log(f"{len(tpls)} templates, mean {np.mean([t.cells for t in tpls]) / 1e3:.0f} k cells, "
+        f"generated in {time.time() - t0:.1f}s")
+    srv = S.Server(hbm_budget=200 * 10**9, tol=1e-6, vcycles_per_iter=4)
+    gids = [srv.add_geometry(t.flag, t.h) for t in tpls]
+    # warm-up: one request per template
+    for g, t in zip(gids, tpls):
+        srv.wait(srv.submit(g, t.rhs, t.fnorm))
+    arr = workloads.arrivals(args.rate, args.duration, seed=7)
+    pick = workloads.rng(8).integers(0, len(tpls), arr.size)
+    base = srv.clock()
+    rids = []
+    tw = time.perf_counter()
+    for ta, i in zip(arr, pick):
+        while time.perf_counter() - tw < ta:
+            time.sleep(min(0.0005, max(0.0, ta - (time.perf_counter() - tw))))
+        rids.append(srv.submit(gids[i], tpls[i].rhs, tpls[i].fnorm))
+    infos = [srv.wait(r) for r in rids]
+    st = srv.stats()
+    srv.close()
+    lat = np.array([i["t_done"] - i["t_submit"] for i in infos])
+    ok = sum(1 for i in infos if i["state"] == "done" and i["converged"])
+    span = max(i["t_done"] for i in infos) - min(i["t_submit"] for i in infos)
+    solves = 3 * len(infos)
+    log(f"served {len(infos)} requests ({ok} converged) in {span:.2f}s: "
+        f"{solves / span:.1f} solves/s, p50 {np.percentile(lat, 50) * 1e3:.1f} ms, "
+        f"p99 {np.percentile(lat, 99) * 1e3:.1f} ms, peak active {st['peak_active']}, "
+        f"peak {st['peak_bytes'] / 1e9:.2f} GB")
+    out = {
+        "metric": "served solves/s with p50/p99 latency @1e-6, Poisson arrivals",
+        "value": round(solves / span, 3), "unit": "solves/s", "n_gpus": 1,
+        "steps": len(infos), "warmup": len(tpls), "ms_per_step": round(span * 1e3 / len(infos), 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32",
+        "data": "synthetic",
+        "config": {"workload": f"Poisson arrivals {args.rate:g}/s for {args.duration:g}s (seed 7), "
+                               f"3 warm-started solves per request to 1e-6, {len(tpls)} domains "
+                               f"64x64..4096x2048, 200 GB budget",
+                   "requests": len(infos), "converged": ok,
+                   "requests_per_s": round(len(infos) / span, 2),
+                   "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+                   "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+                   "peak_active": st["peak_active"],
+                   "peak_GB": round(st["peak_bytes"] / 1e9, 3),
+                   "worker_steps": st["steps"], "worker_busy_s": round(st["busy_s"], 3)},
     }
     print(json.dumps(out), flush=True)
 

@@ -220,6 +220,61 @@ UBG_API int ubg_batch_autotune(ubg_batch *b, const char *db_path, ubg_batch_tuni
 UBG_API int ubg_batch_set_tuning(ubg_batch *b, const ubg_batch_tuning *t);
 UBG_API int ubg_batch_get_tuning(ubg_batch *b, ubg_batch_tuning *t);
 
+/* ---- Served solves ----------------------------------------------------------
+ * No counterpart in the reference: a request loop on its own host thread.
+ * A request is `timesteps` consecutive refinement solves on one registered
+ * geometry (flag + h), each to tol relative to ||F f|| of its rhs, the next
+ * warm-started from the previous solution; its fields stay resident on the
+ * device until the last one is done.  Requests are admitted in arrival order
+ * while the device arena (hbm_budget bytes) and max_active allow, and every
+ * refinement step advances all admitted requests together (continuous
+ * batching: a request arriving while others run joins at the next step).  A
+ * request's results do not depend on what else is in flight (bit-identical
+ * to running it alone). */
+typedef struct ubg_server ubg_server;
+typedef struct ubg_server_params {
+  size_t hbm_budget;    /* device bytes for geometries + resident requests  */
+  int max_active;       /* requests on the device at once                  */
+  double tol;           /* per timestep, relative to ||F f||, 1e-6          */
+  int vcycles_per_iter; /* FP32 V-cycles per refinement step, 4            */
+  int max_iter;         /* refinement steps per timestep at most, 200      */
+} ubg_server_params;
+typedef struct ubg_request_info {
+  uint64_t id;
+  int state;            /* 0 queued, 1 running, 2 done, 3 failed           */
+  int timesteps_done;
+  int iterations;       /* refinement steps over all timesteps             */
+  int converged;        /* every timestep met tol                          */
+  double rel_residual;  /* of the last timestep                            */
+  double t_submit, t_admit, t_done; /* seconds on the server's clock       */
+} ubg_request_info;
+typedef struct ubg_server_stats {
+  long submitted, completed, failed, solves, steps;
+  int active, queued, peak_active;
+  size_t bytes_in_use, peak_bytes;
+  double busy_s;        /* worker time spent in steps                      */
+} ubg_server_stats;
+UBG_API void ubg_server_params_default(ubg_server_params *out);
+UBG_API int ubg_server_create(const ubg_server_params *prm, const ubg_params *mg,
+                              ubg_server **out);
+/* stops the worker after the requests in flight finish; queued ones fail */
+UBG_API int ubg_server_destroy(ubg_server *srv);
+/* register a geometry (flag W x H, 0/1, fluid outer ring; h) shared by
+ * requests; resident until the server is destroyed */
+UBG_API int ubg_server_add_geometry(ubg_server *srv, int width, int height, const float *flag,
+                                    float h, int *geom_id);
+/* rhs[t], t < timesteps: W x H float32 host arrays the caller keeps alive
+ * until the request is done; fnorm: ||F f|| per timestep or NULL (computed
+ * here); x_out: W x H float64 host array for the final solution or NULL */
+UBG_API int ubg_server_submit(ubg_server *srv, int geom_id, int timesteps, const float *const *rhs,
+                              const double *fnorm, double *x_out, uint64_t *id);
+/* wait up to timeout_s (< 0: forever) for request id to finish; returns
+ * UBG_E_STATE on timeout with info filled so far */
+UBG_API int ubg_server_wait(ubg_server *srv, uint64_t id, double timeout_s, ubg_request_info *info);
+UBG_API int ubg_server_drain(ubg_server *srv);
+UBG_API int ubg_server_stats_get(ubg_server *srv, ubg_server_stats *st);
+UBG_API double ubg_server_clock(ubg_server *srv);
+
 /* ---- Simulation: simulation.hpp:18-141 ---------------------------------- */
 /* Simulation(flag, pwidth, mu) (simulation.hpp:32-67): BCs S/N no-slip,
  * W inflow, E zero-pressure outflow; vx(0,y)=1, vy(x,0)=0 in both buffers;

@@ -1,0 +1,119 @@
+"""Served solves (ubg_server_*): continuous batching with HBM-budget
+admission.
+
+Contract: each request's result (every timestep to 1e-6 relative to ||F f||,
+warm-started from the previous timestep) is bit-identical whether it runs
+alone or among other requests that arrive while it is on the device; its
+residual, recomputed in float64 by the oracle's calculateResidualField, meets
+the tolerance; the device memory in use never exceeds the budget.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from ubootgl_amd import _lib as L
+from ubootgl_amd import batch as B
+from ubootgl_amd import serve as S
+from ubootgl_amd import workloads as W
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(64, 64), (96, 160), (256, 128), (480, 96), (128, 384), (320, 320)]
+
+
+@pytest.fixture(scope="module")
+def templates():
+    return [W.serving_template(w, h, seed=70 + i) for i, (w, h) in enumerate(SIZES)]
+
+
+def run_alone(tpl, gpu_budget=2 * 10**9):
+    srv = S.Server(hbm_budget=gpu_budget, tol=1e-6)
+    gid = srv.add_geometry(tpl.flag, tpl.h)
+    rid = srv.submit(gid, tpl.rhs, tpl.fnorm, want_x=True)
+    info = srv.wait(rid)
+    srv.close()
+    return info
+
+
+def test_serve_alone_equals_mix(gpu, templates):
+    alone = [run_alone(t) for t in templates]
+    srv = S.Server(hbm_budget=2 * 10**9, tol=1e-6)
+    gids = [srv.add_geometry(t.flag, t.h) for t in templates]
+    rng = np.random.default_rng(3)
+    order = list(rng.permutation(len(templates))) * 3  # each template three times
+    rids = []
+    for k, i in enumerate(order):
+        rids.append((i, srv.submit(gids[i], templates[i].rhs, templates[i].fnorm, want_x=True)))
+        if k % 4 == 3:
+            time.sleep(0.002)  # later requests arrive while earlier ones run
+    for i, rid in rids:
+        info = srv.wait(rid)
+        assert info["state"] == "done" and info["converged"]
+        assert info["timesteps_done"] == 3
+        assert info["iterations"] == alone[i]["iterations"]
+        assert np.array_equal(info["x"], alone[i]["x"]), f"template {i}"
+    st = srv.stats()
+    assert st["completed"] == len(order) and st["failed"] == 0
+    assert st["solves"] == 3 * len(order)
+    assert st["peak_active"] > 1  # requests did overlap on the device
+    srv.close()
+
+
+def test_serve_residual_independent_check(gpu, templates):
+    """Last timestep's solution against a float64 residual computed by the
+    oracle: ||r|| <= tol * ||F f|| (with a little rounding slack)."""
+    t = templates[2]
+    info = run_alone(t)
+    _, rn = O.residual(info["x"], t.rhs[-1], t.flag, float(t.h))
+    assert rn <= 1.0001e-6 * t.fnorm[-1]
+    assert info["rel_residual"] <= 1e-6
+
+
+def test_serve_budget_admission(gpu, templates):
+    """A budget that holds the geometries and about two requests: the rest
+    queue and are admitted as memory frees up; all finish, the arena's peak
+    stays within the budget."""
+    t = templates[5]
+    per = B.batch_bytes([(t.W, t.H)])
+    budget = int(2.6 * per)
+    srv = S.Server(hbm_budget=budget, tol=1e-6)
+    gid = srv.add_geometry(t.flag, t.h)
+    rids = [srv.submit(gid, t.rhs, t.fnorm) for _ in range(6)]
+    infos = [srv.wait(r) for r in rids]
+    assert all(i["state"] == "done" for i in infos)
+    st = srv.stats()
+    assert st["peak_bytes"] <= budget
+    assert st["peak_active"] <= 2
+    # admission is in arrival order
+    admits = [i["t_admit"] for i in infos]
+    assert admits == sorted(admits)
+    srv.close()
+
+
+def test_serve_errors(gpu, templates):
+    srv = S.Server(hbm_budget=64 * 2**20, tol=1e-6, max_active=4)
+    walls = W.walls(64, 64).astype(np.float32)
+    with pytest.raises(L.UbgError) as e:
+        srv.add_geometry(walls, 1 / 64)
+    assert e.value.code == L.E_FLAG_VALUE
+    big = templates[5]
+    gid = srv.add_geometry(templates[0].flag, templates[0].h)
+    with pytest.raises(L.UbgError) as e:
+        srv.submit(99, templates[0].rhs)
+    assert e.value.code == L.E_INVALID
+    # fnorm computed by the server when not given
+    rid = srv.submit(gid, templates[0].rhs, want_x=True)
+    info = srv.wait(rid)
+    assert info["state"] == "done" and info["converged"]
+    with pytest.raises(L.UbgError) as e:
+        srv.wait(12345)
+    assert e.value.code == L.E_INVALID
+    srv.close()
+    srv = S.Server(hbm_budget=2**21, tol=1e-6)
+    with pytest.raises(L.UbgError) as e:
+        gid = srv.add_geometry(big.flag, big.h)
+        srv.submit(gid, big.rhs)
+    assert e.value.code == L.E_BUDGET
+    srv.close()

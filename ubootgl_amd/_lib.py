@@ -67,6 +67,24 @@ class BatchTuning(C.Structure):
                 ("sweep_ms", C.c_double)]
 
 
+class ServerParams(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int), ("tol", C.c_double),
+                ("vcycles_per_iter", C.c_int), ("max_iter", C.c_int)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("state", C.c_int), ("timesteps_done", C.c_int),
+                ("iterations", C.c_int), ("converged", C.c_int), ("rel_residual", C.c_double),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_long), ("completed", C.c_long), ("failed", C.c_long),
+                ("solves", C.c_long), ("steps", C.c_long), ("active", C.c_int),
+                ("queued", C.c_int), ("peak_active", C.c_int), ("bytes_in_use", C.c_size_t),
+                ("peak_bytes", C.c_size_t), ("busy_s", C.c_double)]
+
+
 _vp = C.c_void_p
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int)
@@ -112,6 +130,16 @@ SIGNATURES = {
     "ubg_batch_autotune": (C.c_int, [_vp, C.c_char_p, C.POINTER(BatchTuning)]),
     "ubg_batch_set_tuning": (C.c_int, [_vp, C.POINTER(BatchTuning)]),
     "ubg_batch_get_tuning": (C.c_int, [_vp, C.POINTER(BatchTuning)]),
+    "ubg_server_params_default": (None, [C.POINTER(ServerParams)]),
+    "ubg_server_create": (C.c_int, [C.POINTER(ServerParams), C.POINTER(Params), C.POINTER(_vp)]),
+    "ubg_server_destroy": (C.c_int, [_vp]),
+    "ubg_server_add_geometry": (C.c_int, [_vp, C.c_int, C.c_int, _fp, C.c_float, _ip]),
+    "ubg_server_submit": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(_fp), _dp, _dp,
+                                    C.POINTER(C.c_uint64)]),
+    "ubg_server_wait": (C.c_int, [_vp, C.c_uint64, C.c_double, C.POINTER(RequestInfo)]),
+    "ubg_server_drain": (C.c_int, [_vp]),
+    "ubg_server_stats_get": (C.c_int, [_vp, C.POINTER(ServerStats)]),
+    "ubg_server_clock": (C.c_double, [_vp]),
     "ubg_sim_create": (C.c_int, [_fp, C.c_int, C.c_int, C.c_float, C.c_float, C.POINTER(_vp)]),
     "ubg_sim_destroy": (C.c_int, [_vp]),
     "ubg_sim_set_params": (C.c_int, [_vp, C.POINTER(Params)]),

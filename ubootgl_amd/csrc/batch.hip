@@ -32,7 +32,8 @@ constexpr int TPB = 256;
 struct StageArgs {
   const BLev *lev;
   const BDom *dom;
-  const Tile *tiles;
+  const int *off; // tile prefix over domain slots [ndom + 1], then tiles per row [ndom]
+  int ndom;
   int ntiles;
   int level;
   const uint8_t *ia; // per-domain index row a
@@ -44,9 +45,48 @@ __device__ __forceinline__ bool dom_active(const BDom &D) {
   return __builtin_amdgcn_readfirstlane(D.st->active) != 0;
 }
 
-// Block b of a persistent grid of G handles tiles v, v+G, ... with v the
-// XCD-grouped block order, so neighbouring tiles share an XCD's L2.
-__device__ __forceinline__ int first_tile() { return xcd_tile(blockIdx.x, gridDim.x); }
+// Tiles of a stage are split into equal contiguous chunks, one per block of
+// the (persistent) grid, in XCD-grouped block order so that neighbouring
+// chunks share an XCD's L2.  A block locates the domain slot of its first tile
+// once (the last d with off[d] <= t; domains without tiles at this level
+// have off[d] == off[d+1]) and then walks forward.
+struct TileWalk {
+  int t, t1; // current tile, end of the chunk
+  int d, j;  // domain slot, tile index within the domain
+  int nt, ntx;
+  __device__ __forceinline__ bool init(const StageArgs &a) {
+    const int per = (a.ntiles + gridDim.x - 1) / gridDim.x;
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    t = b * per;
+    t1 = min(t + per, a.ntiles);
+    if (t >= t1) return false;
+    int lo = 0, hi = a.ndom;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (a.off[mid] <= t) lo = mid;
+      else hi = mid;
+    }
+    d = lo;
+    j = t - a.off[d];
+    nt = a.off[d + 1] - a.off[d];
+    ntx = a.off[a.ndom + 1 + d];
+    return true;
+  }
+  __device__ __forceinline__ bool next(const StageArgs &a) {
+    if (++t >= t1) return false;
+    if (++j == nt) {
+      do {
+        d++;
+      } while (a.off[d + 1] == a.off[d]);
+      j = 0;
+      nt = a.off[d + 1] - a.off[d];
+      ntx = a.off[a.ndom + 1 + d];
+    }
+    return true;
+  }
+  __device__ __forceinline__ int tx() const { return j % ntx; }
+  __device__ __forceinline__ int ty() const { return j / ntx; }
+};
 
 // One red+black rbgs sweep of a TX x TY tile, ping-pong (read p[a], write
 // p[a^1]).  p is staged with a 2-cell halo; red cells ((x+y) odd) of the tile
@@ -58,8 +98,10 @@ __global__ void __launch_bounds__(TPB) kb_sweep(StageArgs a) {
   static_assert(TX % 2 == 0, "pairs need an even tile width");
   constexpr int PX = TX + 4, PY = TY + 4, RX = TX + 2, RY = TY + 2;
   __shared__ float sp[PY * PX];
-  for (int t = first_tile(); t < a.ntiles; t += gridDim.x) {
-    const Tile T = a.tiles[t];
+  TileWalk tw;
+  if (!tw.init(a)) return;
+  do {
+    const Tile T{tw.d, tw.tx() * TX, tw.ty() * TY, tw.j};
     const BDom &D = a.dom[T.d];
     if (!dom_active(D)) continue;
     const BLev &L = a.lev[T.d * BMAXL + a.level];
@@ -113,7 +155,7 @@ __global__ void __launch_bounds__(TPB) kb_sweep(StageArgs a) {
       if (c0 < w) dst[g0] = v0;
       if (c0 + 1 < w) dst[g0 + 1] = v1;
     }
-  }
+  } while (tw.next(a));
 }
 
 // calculateResidualField + restrict (pressure_solver.cpp:91-132) for a tile
@@ -124,8 +166,10 @@ constexpr int RCX = 32, RCY = 8, RFX = 2 * RCX + 1, RFY = 2 * RCY + 1, RPX = RFX
 __global__ void __launch_bounds__(TPB) kb_rr(StageArgs a) {
   __shared__ float sp[RPY][RPX];
   __shared__ float sr[RFY][RFX + 1];
-  for (int t = first_tile(); t < a.ntiles; t += gridDim.x) {
-    const Tile T = a.tiles[t];
+  TileWalk tw;
+  if (!tw.init(a)) return;
+  do {
+    const Tile T{tw.d, 1 + tw.tx() * RCX, 1 + tw.ty() * RCY, tw.j};
     const BDom &D = a.dom[T.d];
     if (!dom_active(D)) continue;
     const BLev &F = a.lev[T.d * BMAXL + a.level];
@@ -165,7 +209,7 @@ __global__ void __launch_bounds__(TPB) kb_rr(StageArgs a) {
       C.f[ci] = vv * (1.0f / 16.0f);
       C.p[a.ib[T.d]][ci] = 0.0f;
     }
-  }
+  } while (tw.next(a));
 }
 
 // prolongate + correct (pressure_solver.cpp:134-181) for a tile of 256 x 8
@@ -173,8 +217,10 @@ __global__ void __launch_bounds__(TPB) kb_rr(StageArgs a) {
 // solution is p[ib] of the next level.
 constexpr int PCW = 256, PCH = 8;
 __global__ void __launch_bounds__(TPB) kb_pc(StageArgs a) {
-  for (int t = first_tile(); t < a.ntiles; t += gridDim.x) {
-    const Tile T = a.tiles[t];
+  TileWalk tw;
+  if (!tw.init(a)) return;
+  do {
+    const Tile T{tw.d, 1 + tw.tx() * PCW, 1 + tw.ty() * PCH, tw.j};
     const BDom &D = a.dom[T.d];
     if (!dom_active(D)) continue;
     const BLev &F = a.lev[T.d * BMAXL + a.level];
@@ -209,7 +255,7 @@ __global__ void __launch_bounds__(TPB) kb_pc(StageArgs a) {
       }
       p[i] += e;
     }
-  }
+  } while (tw.next(a));
 }
 
 // the coarse levels of each domain in LDS, one workgroup per domain slot;
@@ -233,8 +279,10 @@ __device__ __forceinline__ double wsum(double v) {
 // every domain: ia = e index of level 0; ib bit 0 = x index, bit 1 = add e.
 __global__ void __launch_bounds__(B_UPD_W) kb_update(StageArgs a) {
   __shared__ double red[B_UPD_W / 64];
-  for (int t = first_tile(); t < a.ntiles; t += gridDim.x) {
-    const Tile T = a.tiles[t];
+  TileWalk tw;
+  if (!tw.init(a)) return;
+  do {
+    const Tile T{tw.d, tw.tx() * B_UPD_W, tw.ty() * B_UPD_H, tw.j};
     const BDom &D = a.dom[T.d];
     if (!dom_active(D)) continue;
     const BLev &L0 = a.lev[T.d * BMAXL];
@@ -286,7 +334,7 @@ __global__ void __launch_bounds__(B_UPD_W) kb_update(StageArgs a) {
       for (int i = 0; i < B_UPD_W / 64; i++) s += red[i];
       D.partial[T.i] = s;
     }
-  }
+  } while (tw.next(a));
 }
 
 // per-domain norm (the order of refine.hip's k_sum_partials) and the
@@ -310,19 +358,20 @@ __global__ void __launch_bounds__(1024) kb_norm(StageArgs a, int nslots) {
   DState &S = *D.st;
   S.xcur = a.ib[d];
   if (a.ia[d]) { // first update: the initial residual
-    const double r0 = sqrt(n2);
-    S.r0 = r0;
+    const double r = sqrt(n2);
+    S.r0 = S.ref > 0.0 ? S.ref : r;
     S.iters = 0;
     S.vcycles = 0;
     S.stall = 0;
     S.best = 1.0;
-    S.rel = 1.0;
     S.converged = 0;
-    if (!isfinite(r0)) {
+    if (!isfinite(r)) {
       S.nonfinite = 1;
       S.active = 0;
-    } else if (r0 == 0.0) {
-      S.rel = 0.0;
+      return;
+    }
+    S.rel = r == 0.0 ? 0.0 : r / S.r0;
+    if (r == 0.0 || S.rel <= S.tol) { // nothing to do (a warm start may already meet tol)
       S.converged = 1;
       S.active = 0;
     }
@@ -497,7 +546,7 @@ BatchEngine::~BatchEngine() {
   if (d_lev_) (void)hipFree(d_lev_);
   if (d_dom_) (void)hipFree(d_dom_);
   if (d_cargs_) (void)hipFree(d_cargs_);
-  if (d_tiles_) (void)hipFree(d_tiles_);
+  if (d_off_) (void)hipFree(d_off_);
   if (d_idx_) (void)hipFree(d_idx_);
   if (d_count_) (void)hipFree(d_count_);
   if (d_active_) (void)hipFree(d_active_);
@@ -648,10 +697,11 @@ int BatchEngine::domain_set_guess(BDomain *d, const double *x) {
   return UBG_OK;
 }
 
-int BatchEngine::domain_start(BDomain *d, double tol, int max_iter) {
+int BatchEngine::domain_start(BDomain *d, double tol, int max_iter, double ref) {
   DState s{};
   s.active = 1;
   s.tol = tol;
+  s.ref = ref;
   s.max_iter = max_iter;
   s.k = k_;
   s.best = 1.0;
@@ -783,59 +833,63 @@ int BatchEngine::build_tables() {
   return UBG_OK;
 }
 
+// Per stage kind and level: the prefix of tile counts over the member slots
+// and each domain's tiles per row; kernels locate their tile from these, so
+// a member change rebuilds O(domains x levels) integers, not tile lists.
 int BatchEngine::build_tiles() {
-  h_tiles_.clear();
-  auto mark = [&](Span &sp) {
-    sp.off = h_tiles_.size();
-    sp.n = 0;
+  h_off_.clear();
+  auto span = [&](Span &sp, int nd, auto tiles_of) {
+    sp.off = h_off_.size();
+    sp.nd = nd;
+    h_off_.resize(sp.off + 2 * (size_t)nd + 1);
+    int *pre = h_off_.data() + sp.off, *ntx = pre + nd + 1;
+    long tot = 0;
+    for (int s = 0; s < nd; s++) {
+      int nx = 0, ny = 0;
+      tiles_of(s, nx, ny);
+      pre[s] = (int)tot;
+      ntx[s] = nx > 0 ? nx : 1;
+      tot += (long)nx * ny;
+    }
+    pre[nd] = (int)tot;
+    sp.n = (int)tot;
   };
-  auto close = [&](Span &sp) { sp.n = (int)(h_tiles_.size() - sp.off); };
+  auto cdiv = [](int a, int b) { return a > 0 ? (a + b - 1) / b : 0; };
   const int tx = tune_.tx, ty = tune_.ty;
+  const int nr = (int)run_.size();
   for (int l = 0; l < BMAXL; l++) {
-    mark(sw_[l]);
-    for (size_t s = 0; s < run_.size(); s++) {
+    span(sw_[l], nr, [&](int s, int &nx, int &ny) {
       const BGeom &G = *run_[s]->g;
-      if (l >= G.cfirst) continue;
-      for (int y = 0; y < G.h[l]; y += ty)
-        for (int x = 0; x < G.w[l]; x += tx) h_tiles_.push_back(Tile{(int)s, x, y, 0});
-    }
-    close(sw_[l]);
-    mark(rr_[l]);
-    for (size_t s = 0; s < run_.size(); s++) {
+      if (l >= G.cfirst) return;
+      nx = cdiv(G.w[l], tx);
+      ny = cdiv(G.h[l], ty);
+    });
+    span(rr_[l], nr, [&](int s, int &nx, int &ny) {
       const BGeom &G = *run_[s]->g;
-      if (l >= G.cfirst) continue;
-      for (int y = 1; y < G.h[l + 1] - 1; y += RCY)
-        for (int x = 1; x < G.w[l + 1] - 1; x += RCX) h_tiles_.push_back(Tile{(int)s, x, y, 0});
-    }
-    close(rr_[l]);
-    mark(pc_[l]);
-    for (size_t s = 0; s < run_.size(); s++) {
+      if (l >= G.cfirst) return;
+      nx = cdiv(G.w[l + 1] - 2, RCX);
+      ny = cdiv(G.h[l + 1] - 2, RCY);
+    });
+    span(pc_[l], nr, [&](int s, int &nx, int &ny) {
       const BGeom &G = *run_[s]->g;
-      if (l >= G.cfirst) continue;
-      for (int y = 1; y < G.h[l] - 1; y += PCH)
-        for (int x = 1; x < G.w[l] - 1; x += PCW) h_tiles_.push_back(Tile{(int)s, x, y, 0});
-    }
-    close(pc_[l]);
+      if (l >= G.cfirst) return;
+      nx = cdiv(G.w[l] - 2, PCW);
+      ny = cdiv(G.h[l] - 2, PCH);
+    });
   }
-  mark(upd_);
-  for (size_t s = 0; s < all_.size(); s++) {
-    const BDomain &D = *all_[s];
-    for (int by = 0; by < D.gy; by++)
-      for (int bx = 0; bx < D.gx; bx++)
-        h_tiles_.push_back(Tile{(int)s, bx * B_UPD_W, by * B_UPD_H, by * D.gx + bx});
+  span(upd_, (int)all_.size(), [&](int s, int &nx, int &ny) {
+    nx = all_[s]->gx;
+    ny = all_[s]->gy;
+  });
+  if (h_off_.size() > off_cap_) {
+    if (d_off_) (void)hipFree(d_off_);
+    d_off_ = nullptr;
+    off_cap_ = h_off_.size() + h_off_.size() / 4 + 64;
+    UBG_HIP(hipMalloc((void **)&d_off_, off_cap_ * sizeof(int)));
   }
-  close(upd_);
-  if (h_tiles_.size() > tiles_cap_) {
-    if (d_tiles_) (void)hipFree(d_tiles_);
-    d_tiles_ = nullptr;
-    tiles_cap_ = h_tiles_.size() + h_tiles_.size() / 4 + 64;
-    UBG_HIP(hipMalloc((void **)&d_tiles_, tiles_cap_ * sizeof(Tile)));
-  }
-  if (!h_tiles_.empty()) {
-    UBG_HIP(hipMemcpyAsync(d_tiles_, h_tiles_.data(), h_tiles_.size() * sizeof(Tile),
-                           hipMemcpyHostToDevice, stream_));
-    UBG_HIP(hipStreamSynchronize(stream_));
-  }
+  UBG_HIP(hipMemcpyAsync(d_off_, h_off_.data(), h_off_.size() * sizeof(int), hipMemcpyHostToDevice,
+                         stream_));
+  UBG_HIP(hipStreamSynchronize(stream_));
   return UBG_OK;
 }
 
@@ -924,6 +978,11 @@ int BatchEngine::launch_stages(const std::vector<StageRec> &st, const uint8_t *d
     int b = tune_.blocks > 0 ? std::min(ntiles, tune_.blocks) : ntiles;
     return std::max(b, 1);
   };
+  auto use = [&](StageArgs &a, const Span &sp) {
+    a.off = d_off_ + sp.off;
+    a.ndom = sp.nd;
+    a.ntiles = sp.n;
+  };
   for (const StageRec &r : st) {
     StageArgs a{};
     a.lev = d_lev_;
@@ -934,18 +993,15 @@ int BatchEngine::launch_stages(const std::vector<StageRec> &st, const uint8_t *d
     a.alpha = prm_.alpha;
     switch (r.kind) {
     case ST_SWEEP:
-      a.tiles = d_tiles_ + sw_[r.level].off;
-      a.ntiles = sw_[r.level].n;
+      use(a, sw_[r.level]);
       if (a.ntiles) launch_sweep(tune_, grid(a.ntiles), stream_, a);
       break;
     case ST_RR:
-      a.tiles = d_tiles_ + rr_[r.level].off;
-      a.ntiles = rr_[r.level].n;
+      use(a, rr_[r.level]);
       if (a.ntiles) hipLaunchKernelGGL(kb_rr, dim3(grid(a.ntiles)), dim3(TPB), 0, stream_, a);
       break;
     case ST_PC:
-      a.tiles = d_tiles_ + pc_[r.level].off;
-      a.ntiles = pc_[r.level].n;
+      use(a, pc_[r.level]);
       if (a.ntiles) hipLaunchKernelGGL(kb_pc, dim3(grid(a.ntiles)), dim3(TPB), 0, stream_, a);
       break;
     case ST_COARSE:
@@ -954,8 +1010,7 @@ int BatchEngine::launch_stages(const std::vector<StageRec> &st, const uint8_t *d
                            stream_, a, (const CoarseArgs *)d_cargs_, (int)run_.size());
       break;
     case ST_UPDATE:
-      a.tiles = d_tiles_ + upd_.off;
-      a.ntiles = upd_.n;
+      use(a, upd_);
       if (a.ntiles) hipLaunchKernelGGL(kb_update, dim3(grid(a.ntiles)), dim3(B_UPD_W), 0, stream_, a);
       break;
     case ST_NORM:
@@ -1118,7 +1173,8 @@ int BatchEngine::launch_sweeps_only(int reps) {
       a.ia = d_idx;
       a.ib = d_idx + n;
       a.alpha = prm_.alpha;
-      a.tiles = d_tiles_ + sw_[l].off;
+      a.off = d_off_ + sw_[l].off;
+      a.ndom = sw_[l].nd;
       a.ntiles = sw_[l].n;
       int b = tune_.blocks > 0 ? std::min(a.ntiles, tune_.blocks) : a.ntiles;
       launch_sweep(tune_, std::max(b, 1), stream_, a);

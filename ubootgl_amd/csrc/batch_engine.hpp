@@ -26,6 +26,7 @@ constexpr int B_MIN_SIZE = 16;
 // solve state of one domain (device; written by the norm kernel)
 struct DState {
   double r0, rel, best, tol;
+  double ref; // > 0: residuals are relative to ref, not to the first one
   int active, converged, iters, vcycles, stall, max_iter, xcur, k;
   int nonfinite, pad;
   double hist[UBG_IR_HISTORY];
@@ -130,7 +131,7 @@ public:
   int domain_set_guess(BDomain *d, const double *x_host);
   // reset a domain's solve state: active, tol/max_iter, fresh (next update
   // computes r0 and does not add e)
-  int domain_start(BDomain *d, double tol, int max_iter);
+  int domain_start(BDomain *d, double tol, int max_iter, double ref = 0.0);
   int domain_read_state(BDomain *d, DState *out);
   int domain_get_solution(BDomain *d, double *x_host);
 
@@ -182,13 +183,15 @@ private:
   CoarseArgs *d_cargs_ = nullptr;
   size_t tab_cap_ = 0;
   size_t coarse_lds_ = 0;
-  // tile lists: per level sweep / rr / pc (running members), update (all)
-  std::vector<Tile> h_tiles_;
-  Tile *d_tiles_ = nullptr;
-  size_t tiles_cap_ = 0;
+  // tile maps: per level sweep / rr / pc (running members), update (all):
+  // prefix of tile counts over slots and tiles per row, in one int array
+  std::vector<int> h_off_;
+  int *d_off_ = nullptr;
+  size_t off_cap_ = 0;
   struct Span {
-    size_t off = 0;
-    int n = 0;
+    size_t off = 0; // into the int array
+    int n = 0;      // tiles
+    int nd = 0;     // domain slots covered
   };
   Span sw_[BMAXL], rr_[BMAXL], pc_[BMAXL], upd_;
   // stage index rows

@@ -273,3 +273,72 @@ def config2(n=256, seed=2):
         solid = float(g.uniform(0.0, 0.08))
         probs.append(poisson_problem(W, H, seed=10_000 * seed + i, solid=solid, nmodes=8))
     return probs
+
+
+def serving_sizes(n, seed, lo=64, wmax=4096, hmax=2048):
+    """W log-uniform in [lo, wmax], H log-uniform in [lo, hmax], independent,
+    rounded to multiples of 32."""
+    g = rng(seed)
+    out = []
+    for _ in range(n):
+        W = int(32 * round(math.exp(g.uniform(math.log(lo), math.log(wmax))) / 32))
+        H = int(32 * round(math.exp(g.uniform(math.log(lo), math.log(hmax))) / 32))
+        out.append((max(lo, W), max(lo, H)))
+    return out
+
+
+class ServingTemplate:
+    """A served domain: flag, h and the rhs of `timesteps` consecutive
+    timesteps (the velocity's mode phases advance by `dphase` per step)."""
+
+    def __init__(self, flag, rhs, h):
+        self.flag = np.ascontiguousarray(flag, np.float32)
+        self.H, self.W = self.flag.shape
+        self.rhs = [np.ascontiguousarray(r, np.float32) for r in rhs]
+        self.h = np.float32(h)
+        inner = self.flag[1:-1, 1:-1] > 0.5
+        self.fnorm = [float(np.sqrt((r[1:-1, 1:-1][inner].astype(np.float64) ** 2).sum()))
+                      for r in self.rhs]
+
+    @property
+    def cells(self):
+        return self.W * self.H
+
+
+def serving_template(W, H, seed, timesteps=3, dphase=0.25, nmodes=8):
+    g = rng(seed)
+    solid = float(g.uniform(0.0, 0.08))
+    P = poisson_problem(W, H, seed=seed, solid=solid, nmodes=nmodes)
+    flag = P.flag.astype(np.float64)
+    g2 = rng(seed + 1)
+    mx = draw_modes(g2, nmodes, 6)
+    my = draw_modes(g2, nmodes, 6)
+    Xx = (np.arange(W - 1, dtype=np.float64)[None, :] + 0.5) / W
+    Yx = np.arange(H, dtype=np.float64)[:, None] / H
+    Xy = np.arange(W, dtype=np.float64)[None, :] / W
+    Yy = (np.arange(H - 1, dtype=np.float64)[:, None] + 0.5) / H
+    rhs = []
+    for t in range(timesteps):
+        sx = [(kx, ky, ph + t * dphase, a) for kx, ky, ph, a in mx]
+        sy = [(kx, ky, ph + t * dphase, a) for kx, ky, ph, a in my]
+        vx = mode_field((H, W - 1), Xx, Yx, sx) * flag[:, :-1] * flag[:, 1:]
+        vy = mode_field((H - 1, W), Xy, Yy, sy) * flag[:-1, :] * flag[1:, :]
+        rhs.append(divergence_rhs(flag, vx, vy, float(P.h)))
+    return ServingTemplate(P.flag, rhs, P.h)
+
+
+def serving_templates(n, seed=7, timesteps=3):
+    """n served domains with sizes from serving_sizes (64x64 .. 4096x2048)."""
+    return [serving_template(W, H, seed * 1000 + i, timesteps)
+            for i, (W, H) in enumerate(serving_sizes(n, seed))]
+
+
+def arrivals(rate, duration, seed=7):
+    """Poisson arrival times (s) at `rate` per second over [0, duration)."""
+    g = rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += g.exponential(1.0 / rate)
+        if t >= duration:
+            return np.array(out)
+        out.append(t)
