@@ -48,7 +48,7 @@ def main():
     ap.add_argument("--height", type=int, default=2048)
     ap.add_argument("--tracers", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=120)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--workload", choices=["step", "ir", "batch", "serve"], default="step",
                     help="step: the 4096x2048 fluid step (default); ir: the 8192x4096 "
@@ -143,11 +143,14 @@ def main():
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
     K = kern[dom]
     achieved = K["pass_GBps"]
+    # HBM bytes per launch of the same kernel from two rocprofv3 --pmc runs
+    # (FETCH_SIZE, WRITE_SIZE), scaled by what the counters read for the
+    # step's plain 33.5 MB device copy
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"mg_{dom}_l0_4096x2048_traffic.json")
-    if os.path.exists(tpath):
+    tpath = os.path.join(ROOT, "profiles", "mg_l0_4096x2048_traffic.json")
+    if (d.W, d.H) == (4096, 2048) and os.path.exists(tpath):
         with open(tpath) as fh:
-            traffic = json.load(fh).get("bytes_per_launch")
+            traffic = json.load(fh).get(dom, {}).get("bytes_per_launch")
 
     vxg = sim.get("vx")
     finite = bool(np.isfinite(vxg).all())

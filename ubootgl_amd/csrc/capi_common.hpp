@@ -145,8 +145,8 @@ struct ubg_mg {
   Timer *timers = nullptr; // borrowed from the owning simulation (UBG_TIMER_COUNT)
   std::recursive_mutex mu;
 
-  // run `fn` (one kernel launch) bracketed by HIP events on this stream when
-  // timer `id` is enabled and the work is on level 0
+  // run `fn` (one kernel launch) with timer `id`'s next event pair attached
+  // to its dispatch when the timer is enabled and the work is on level 0
   template <typename Fn> void timed(int id, int l, Fn fn) {
     Timer *t = timers ? &timers[id] : nullptr;
     bool on = t && t->on && l == 0;
@@ -156,12 +156,12 @@ struct ubg_mg {
         t->ev.resize(n + 512);
         for (size_t i = n; i < t->ev.size(); i++) (void)hipEventCreate(&t->ev[i]);
       }
-      (void)hipEventRecord(t->ev[t->used], stream);
+      g_launch_ev = LaunchEvents{t->ev[t->used], t->ev[t->used + 1]};
     }
     fn();
     if (on) {
-      (void)hipEventRecord(t->ev[t->used + 1], stream);
-      t->used += 2;
+      if (!g_launch_ev.start) t->used += 2; // the launch took the pair
+      g_launch_ev = LaunchEvents{};
     }
   }
 

@@ -188,7 +188,7 @@ void launch_rbgs_fused(Level &L, float h, float alpha, bool zero_gradient, hipSt
   const bool al = (L.w % 2) == 0;
   auto k = zero_gradient ? (al ? k_rbgs_pair<true, true> : k_rbgs_pair<true, false>)
                          : (al ? k_rbgs_pair<false, true> : k_rbgs_pair<false, false>);
-  hipLaunchKernelGGL(k, dim3(nblocks), dim3(TPB_T), 0, s, L.p, L.p_alt, L.f, L.mask, L.w, L.h, h,
+  launch_k(k, dim3(nblocks), dim3(TPB_T), 0, s, L.p, L.p_alt, L.f, L.mask, L.w, L.h, h,
                      alpha, nstrips, R, nunits);
   float *t = L.p;
   L.p = L.p_alt;

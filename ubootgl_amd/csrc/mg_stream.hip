@@ -305,11 +305,11 @@ __global__ void __launch_bounds__(TPB) k_mg_stream(const StreamArgs A) {
 template <int NS, int MODE>
 void launch_ns(const StreamArgs &a, bool zg, bool al, int nblocks, hipStream_t s) {
   if (zg) {
-    if (al) hipLaunchKernelGGL((k_mg_stream<NS, MODE, true, true>), dim3(nblocks), dim3(TPB), 0, s, a);
-    else hipLaunchKernelGGL((k_mg_stream<NS, MODE, true, false>), dim3(nblocks), dim3(TPB), 0, s, a);
+    if (al) launch_k((k_mg_stream<NS, MODE, true, true>), dim3(nblocks), dim3(TPB), 0, s, a);
+    else launch_k((k_mg_stream<NS, MODE, true, false>), dim3(nblocks), dim3(TPB), 0, s, a);
   } else {
-    if (al) hipLaunchKernelGGL((k_mg_stream<NS, MODE, false, true>), dim3(nblocks), dim3(TPB), 0, s, a);
-    else hipLaunchKernelGGL((k_mg_stream<NS, MODE, false, false>), dim3(nblocks), dim3(TPB), 0, s, a);
+    if (al) launch_k((k_mg_stream<NS, MODE, false, true>), dim3(nblocks), dim3(TPB), 0, s, a);
+    else launch_k((k_mg_stream<NS, MODE, false, false>), dim3(nblocks), dim3(TPB), 0, s, a);
   }
 }
 

@@ -11,10 +11,32 @@
 //   * an MG hierarchy is one allocation carved into levels.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
 namespace ubg {
+
+// Kernel timing: an event pair set here is attached to the next launch made
+// through launch_k, on the dispatch itself (hipExtLaunchKernelGGL), so the
+// elapsed time is the kernel's own begin-to-end -- the interval a kernel
+// trace reports.  Used from the thread that owns the handle being timed.
+struct LaunchEvents {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+inline LaunchEvents g_launch_ev;
+
+template <typename F, typename... Args>
+inline void launch_k(F kernel, dim3 grid, dim3 block, uint32_t shmem, hipStream_t s,
+                     Args... args) {
+  if (g_launch_ev.start) {
+    hipExtLaunchKernelGGL(kernel, grid, block, shmem, s, g_launch_ev.start, g_launch_ev.stop, 0,
+                          args...);
+    g_launch_ev = LaunchEvents{};
+  } else {
+    hipLaunchKernelGGL(kernel, grid, block, shmem, s, args...);
+  }
+}
 
 enum : uint8_t {
   MK_SELF = 1u, // F(x, y)
