@@ -316,6 +316,26 @@ UBG_API int ubg_sim_get_mg(ubg_sim *sim, ubg_mg **mg);
 /* bilinearVel(c) (advect_floating_items.cpp:11-14) for n grid-space points
  * c = (x0,y0,x1,y1,...); host arrays. */
 UBG_API int ubg_sim_bilinear_vel(ubg_sim *sim, const float *c, float *out, int n);
+/* The simulation part of UbootGlApp::shiftMap (ubootgl_app.cpp:248-296),
+ * called after Terrain::shiftMap: vx (both buffers, from the front one), vy
+ * and p shift one cell left, setGrids(c, new_flag(c)) for every cell
+ * (simulation.hpp:82-98), the inlet velocity 0.07 H / (1 + sum flag(0, y))
+ * on vx(0, y), saveCurrentVelocityFields, mg.updateFields.  W <= 16384. */
+UBG_API int ubg_sim_shift_map(ubg_sim *sim, const float *new_flag);
+/* Terrain flags from a decoded RGBA8 image (lodepng's layout, row 0 at the
+ * top): flag_full (W x H, may be NULL) = red / 255 with rows flipped
+ * (terrain.cpp:34-39); flag_sim ((W/scale) x (H/scale)) = Terrain::subSample,
+ * block mean > 0.99 (terrain.cpp:3-11).  Host arrays. */
+UBG_API int ubg_terrain_flags(const uint8_t *rgba, int width, int height, int scale,
+                              float *flag_full, float *flag_sim);
+/* Collocated velocity for rendering: VelocityTextures::updateFromStaggered
+ * (velocity_textures.cpp:63-93) + interp_shader.cs:14-33 on vx_current /
+ * vy_current (ubootgl_app.cpp:114).  vxy: (2W-1) x (2H-1) interleaved
+ * (vx, vy) pairs (the RG32F image), mag: their lengths (the R32F image);
+ * linear texel filtering with GL_REPEAT as the default sampler does.
+ * on_device != 0: vxy / mag are device addresses, written asynchronously on
+ * the simulation's stream; else host arrays (synchronous). */
+UBG_API int ubg_sim_collocate(ubg_sim *sim, float *vxy, float *mag, int on_device);
 /* Tracers: n points in physical coordinates (x,y) and ages, resident on the
  * device.  Each ubg_sim_run step advects them with RK2 midpoint integration
  * of bilinearVel, the freeze/age rule and Wang-hash respawn of

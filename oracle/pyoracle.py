@@ -288,3 +288,90 @@ def solve_poisson_fp64(flag, f, h, tol=1e-13, nu_pre=3, nu_post=3, nu_coarse=5, 
         if hist[-1] <= tol:
             break
     return x, hist
+
+
+def collocate(vx, vy):
+    """Collocated velocity image of interp_shader.cs:14-33 (dispatched by
+    VelocityTextures::updateFromStaggered, velocity_textures.cpp:63-93) with
+    the default sampler (linear filtering between texel centres, GL_REPEAT),
+    in float32 in the kernel's operation order: pixel (gx, gy) samples vx at
+    texel point ((gx-1)/2, gy/2) and vy at (gx/2, (gy-1)/2).  Returns
+    (vxy (2H-1, 2W-1, 2), mag (2H-1, 2W-1))."""
+    vx = np.asarray(vx, np.float32)
+    vy = np.asarray(vy, np.float32)
+    H = vx.shape[0]
+    W = vx.shape[1] + 1
+    gx = np.arange(2 * W - 1)
+    gy = np.arange(2 * H - 1)
+    half, one = np.float32(0.5), np.float32(1.0)
+
+    def linear(g, twx, twy):
+        gh, gw = g.shape
+        i0, j0 = np.floor_divide(twx, 2), np.floor_divide(twy, 2)
+        wx = np.where(twx % 2 == 1, half, np.float32(0))[None, :]
+        wy = np.where(twy % 2 == 1, half, np.float32(0))[:, None]
+        xa, xb = np.mod(i0, gw), np.mod(i0 + 1, gw)
+        ya, yb = np.mod(j0, gh), np.mod(j0 + 1, gh)
+        t00, t10 = g[np.ix_(ya, xa)], g[np.ix_(ya, xb)]
+        t01, t11 = g[np.ix_(yb, xa)], g[np.ix_(yb, xb)]
+        r0 = (one - wx) * t00 + wx * t10
+        r1 = (one - wx) * t01 + wx * t11
+        return (one - wy) * r0 + wy * r1
+
+    a = linear(vx, gx - 1, gy)
+    b = linear(vy, gx, gy - 1)
+    mag = np.sqrt(a * a + b * b)
+    return np.stack([a, b], -1).astype(np.float32), mag.astype(np.float32)
+
+
+def shift_map(fields, new_flag):
+    """The simulation part of UbootGlApp::shiftMap (ubootgl_app.cpp:248-296)
+    on a dict of float32 fields (vx, vx_back, vy, vy_back, p): the left shifts
+    (:253-269), setGrids(c, new_flag(c)) for every cell (:271-275,
+    simulation.hpp:82-98), the inlet velocity (:278-287), the current copies
+    (:292).  Returns the new dict incl. flag, vx_current, vy_current."""
+    f = {k: np.array(v, np.float32, copy=True) for k, v in fields.items()}
+    flag = np.asarray(new_flag, np.float32)
+    H, W = flag.shape
+    # for x in 2 .. w-1: f(x-1) = f(x), b(x-1) = f(x)  (f(x) not yet overwritten)
+    for a, b in (("vx", "vx_back"), ("vy", "vy_back")):
+        w = f[a].shape[1]
+        moved = f[a][:, 2:w].copy()
+        f[a][:, 1:w - 1] = moved
+        f[b][:, 1:w - 1] = moved
+    f["p"][:, 0:W - 1] = f["p"][:, 1:W].copy()
+    solid = flag == 0
+    f["vx"][solid[:, :W - 1]] = 0.0
+    f["vx"][solid[:, 1:]] = 0.0
+    f["vy"][solid[:H - 1, :]] = 0.0
+    f["vy"][solid[1:, :]] = 0.0
+    f["p"][solid] = 0.0
+    area = np.float32(1.0)
+    for y in range(H - 1):
+        area = np.float32(area + flag[y, 0])
+    vin = np.float32(np.float32(0.07) * np.float32(H)) / area
+    col = (vin * flag[:, 0]).astype(np.float32)
+    f["vx"][:, 0] = col
+    f["vx_back"][:, 0] = col
+    f["flag"] = flag.copy()
+    f["vx_current"] = f["vx"].copy()
+    f["vy_current"] = f["vy"].copy()
+    return f
+
+
+def terrain_flags(rgba, scale):
+    """Terrain's flags (terrain.cpp:3-45): full = red / 255 with rows flipped;
+    sim(x, y) = (block sum in row-major order / scale / scale) > 0.99, float32."""
+    rgba = np.asarray(rgba, np.uint8)
+    H, W = rgba.shape[:2]
+    full = (rgba[::-1, :, 0].astype(np.float32) / np.float32(255.0)).astype(np.float32)
+    sh, sw = H // scale, W // scale
+    sim = np.zeros((sh, sw), np.float32)
+    acc = np.zeros((sh, sw), np.float32)
+    blocks = full[:sh * scale, :sw * scale].reshape(sh, scale, sw, scale)
+    for sy in range(scale):
+        for sx in range(scale):
+            acc = (acc + blocks[:, sy, :, sx]).astype(np.float32)
+    mean = (acc / np.float32(scale) / np.float32(scale)).astype(np.float32)
+    sim[mean > np.float32(0.99)] = 1.0
+    return full, sim

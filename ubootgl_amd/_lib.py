@@ -160,6 +160,9 @@ SIGNATURES = {
     "ubg_sim_get_sinks": (C.c_int, [_vp, _fp, C.c_int, _ip]),
     "ubg_sim_get_mg": (C.c_int, [_vp, C.POINTER(_vp)]),
     "ubg_sim_bilinear_vel": (C.c_int, [_vp, _fp, _fp, C.c_int]),
+    "ubg_sim_collocate": (C.c_int, [_vp, _fp, _fp, C.c_int]),
+    "ubg_sim_shift_map": (C.c_int, [_vp, _fp]),
+    "ubg_terrain_flags": (C.c_int, [C.POINTER(C.c_uint8), C.c_int, C.c_int, C.c_int, _fp, _fp]),
     "ubg_sim_set_tracers": (C.c_int, [_vp, _fp, _fp, C.c_int]),
     "ubg_sim_get_tracers": (C.c_int, [_vp, _fp, _fp, C.c_int]),
     "ubg_sim_advect_tracers": (C.c_int, [_vp, C.c_float, C.c_uint32]),

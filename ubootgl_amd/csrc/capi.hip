@@ -866,6 +866,80 @@ UBG_API int ubg_sim_bilinear_vel(ubg_sim *s, const float *c, float *out, int n) 
   return UBG_OK;
 }
 
+UBG_API int ubg_sim_shift_map(ubg_sim *s, const float *new_flag) {
+  if (!s || !new_flag) return fail(UBG_E_INVALID, "null argument");
+  if (s->W > 16384) return fail(UBG_E_INVALID, "map shift needs W <= 16384");
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  std::vector<uint8_t> f8;
+  UBG_TRY(validate_flag(new_flag, (size_t)s->W * s->H, f8));
+  uint8_t *d = nullptr;
+  UBG_HIP(hipMalloc((void **)&d, f8.size()));
+  hipMemcpyAsync(d, f8.data(), f8.size(), hipMemcpyHostToDevice, s->stream);
+  launch_shift_map(s->vel(), s->P(), d, s->flag, s->stream);
+  s->save_current();              // ubootgl_app.cpp:292
+  s->mg->classify();              // ubootgl_app.cpp:295, mg.updateFields
+  int rc = check_launch();
+  hipError_t e = hipStreamSynchronize(s->stream);
+  (void)hipFree(d);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(UBG_E_HIP, hipGetErrorString(e));
+  return UBG_OK;
+}
+
+UBG_API int ubg_terrain_flags(const uint8_t *rgba, int width, int height, int scale,
+                              float *flag_full, float *flag_sim) {
+  if (!rgba || !flag_sim || width <= 0 || height <= 0 || scale <= 0)
+    return fail(UBG_E_INVALID, "bad argument");
+  int nd = 0;
+  ubg_device_count(&nd);
+  if (nd <= 0) return fail(UBG_E_NO_DEVICE, "no HIP device visible");
+  const size_t npx = (size_t)width * height;
+  const int sw = width / scale, sh = height / scale;
+  const size_t nsim = (size_t)sw * sh;
+  char *d = nullptr;
+  UBG_HIP(hipMalloc((void **)&d, 4 * npx + 4 * npx + 4 * nsim + 256));
+  uint8_t *drgba = (uint8_t *)d;
+  float *dfull = (float *)(d + 4 * npx);
+  float *dsim = (float *)(d + 8 * npx);
+  hipStream_t st = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) {
+    hipMemcpyAsync(drgba, rgba, 4 * npx, hipMemcpyHostToDevice, st);
+    launch_terrain_flags(drgba, width, height, scale, dfull, dsim, st);
+    e = hipGetLastError();
+    if (e == hipSuccess && flag_full)
+      e = hipMemcpyAsync(flag_full, dfull, 4 * npx, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && nsim) e = hipMemcpyAsync(flag_sim, dsim, 4 * nsim, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+  }
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(UBG_E_HIP, hipGetErrorString(e));
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_collocate(ubg_sim *s, float *vxy, float *mag, int on_device) {
+  if (!s || !vxy || !mag) return fail(UBG_E_INVALID, "null argument");
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  const size_t n = (size_t)(2 * s->W - 1) * (2 * s->H - 1);
+  if (on_device) {
+    launch_collocate(s->vx_cur, s->vy_cur, vxy, mag, s->W, s->H, s->stream);
+    UBG_TRY(check_launch());
+    return UBG_OK;
+  }
+  float *d = nullptr;
+  UBG_HIP(hipMalloc((void **)&d, sizeof(float) * 3 * n));
+  launch_collocate(s->vx_cur, s->vy_cur, d, d + 2 * n, s->W, s->H, s->stream);
+  int rc = check_launch();
+  hipMemcpyAsync(vxy, d, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, s->stream);
+  hipMemcpyAsync(mag, d + 2 * n, sizeof(float) * n, hipMemcpyDeviceToHost, s->stream);
+  hipError_t e = hipStreamSynchronize(s->stream);
+  (void)hipFree(d);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(UBG_E_HIP, hipGetErrorString(e));
+  return UBG_OK;
+}
+
 UBG_API int ubg_sim_set_tracers(ubg_sim *s, const float *xy, const float *age, int n) {
   if (!s || n < 0 || (n > 0 && !xy)) return fail(UBG_E_INVALID, "bad argument");
   std::lock_guard<std::recursive_mutex> g(s->mu_);

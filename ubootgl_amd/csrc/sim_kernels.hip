@@ -257,6 +257,129 @@ __global__ void k_advect(const float *__restrict__ xf, const float *__restrict__
   }
 }
 
+// Collocated velocity for rendering (interp_shader.cs:14-33, dispatched by
+// VelocityTextures::updateFromStaggered, velocity_textures.cpp:63-93): pixel
+// (gx, gy) of the (2W-1) x (2H-1) image samples vx at normalized
+// (gx / (2W-2), (gy+1) / 2H) and vy at ((gx+1) / 2W, gy / (2H-2)) with
+// texture(): linear filtering between texel centres and GL_REPEAT wrapping
+// (the default sampler state).  In texel space these are the points
+// ((gx-1)/2, gy/2) of vx and (gx/2, (gy-1)/2) of vy, so every weight is 0 or
+// 1/2; the filter is evaluated as lerp(lerp(t00, t10, wx), lerp(t01, t11,
+// wx), wy) written with (1-w) and w.  mag = length(vx, vy).
+__device__ __forceinline__ float texel_linear(const float *__restrict__ g, int gw, int gh,
+                                              int twx, int twy) {
+  // twx, twy: texel-space coordinate times 2 (exact), wrapped with REPEAT
+  const int i0 = twx >> 1, j0 = twy >> 1; // floor for negatives too
+  const float wx = (twx & 1) ? 0.5f : 0.0f, wy = (twy & 1) ? 0.5f : 0.0f;
+  auto wrap = [](int i, int n) { return ((i % n) + n) % n; };
+  const int xa = wrap(i0, gw), xb = wrap(i0 + 1, gw);
+  const int ya = wrap(j0, gh), yb = wrap(j0 + 1, gh);
+  const float t00 = g[(size_t)ya * gw + xa], t10 = g[(size_t)ya * gw + xb];
+  const float t01 = g[(size_t)yb * gw + xa], t11 = g[(size_t)yb * gw + xb];
+  const float r0 = (1.0f - wx) * t00 + wx * t10;
+  const float r1 = (1.0f - wx) * t01 + wx * t11;
+  return (1.0f - wy) * r0 + wy * r1;
+}
+
+__global__ void k_collocate(const float *__restrict__ vx, const float *__restrict__ vy,
+                            float *__restrict__ vxy, float *__restrict__ mag, int W, int H) {
+  const int gx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gy = blockIdx.y;
+  const int ow = 2 * W - 1;
+  if (gx >= ow) return;
+  const float a = texel_linear(vx, W - 1, H, gx - 1, gy);
+  const float b = texel_linear(vy, W, H - 1, gx, gy - 1);
+  const size_t o = (size_t)gy * ow + gx;
+  vxy[2 * o] = a;
+  vxy[2 * o + 1] = b;
+  mag[o] = sqrtf(a * a + b * b);
+}
+
+// UbootGlApp::shiftMap's simulation part (ubootgl_app.cpp:248-296).  One
+// workgroup per row stages the row in LDS, so the sequential left-to-right
+// copies (each reading a value not yet overwritten) become: new(x-1) =
+// old(x).  field 0: vx (W-1 wide, both buffers from the front one, x-1 in
+// [1, W-3]); 1: vy (W wide, x-1 in [1, W-2]); 2: p (x-1 in [0, W-2]).
+__global__ void k_shift_rows(float *__restrict__ vxf, float *__restrict__ vxb,
+                             float *__restrict__ vyf, float *__restrict__ vyb,
+                             float *__restrict__ p, int W, int H) {
+  extern __shared__ float row[];
+  const int field = blockIdx.y, y = blockIdx.x;
+  const int w = field == 0 ? W - 1 : W;
+  const int rows = field == 1 ? H - 1 : H;
+  if (y >= rows) return;
+  float *f = field == 0 ? vxf : (field == 1 ? vyf : p);
+  float *b = field == 0 ? vxb : (field == 1 ? vyb : nullptr);
+  float *fr = f + (size_t)y * w;
+  for (int x = threadIdx.x; x < w; x += blockDim.x) row[x] = fr[x];
+  __syncthreads();
+  const int lo = field == 2 ? 0 : 1, hi = w - 2; // targets x-1
+  for (int t = lo + threadIdx.x; t <= hi; t += blockDim.x) {
+    fr[t] = row[t + 1];
+    if (b) b[(size_t)y * w + t] = row[t + 1];
+  }
+}
+
+// setGrids(c, flag(c)) for every cell (simulation.hpp:82-98): the flag, and
+// for a solid cell zero velocities on its four faces and its pressure (front
+// buffers).  Only zeros are written, so the order of the cells is immaterial.
+__global__ void k_apply_flags(const uint8_t *__restrict__ nf, uint8_t *__restrict__ fl,
+                              float *__restrict__ vxf, float *__restrict__ vyf,
+                              float *__restrict__ p, int W, int H) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= W) return;
+  const size_t c = (size_t)y * W + x;
+  const uint8_t v = nf[c];
+  fl[c] = v;
+  if (v) return;
+  if (x < W - 1) vxf[(size_t)y * (W - 1) + x] = 0.0f;
+  if (x > 0) vxf[(size_t)y * (W - 1) + x - 1] = 0.0f;
+  if (y < H - 1) vyf[c] = 0.0f;
+  if (y > 0) vyf[c - W] = 0.0f;
+  p[c] = 0.0f;
+}
+
+// inlet velocity after a map shift (ubootgl_app.cpp:283-291): inletArea =
+// 1 + sum of flag(0, y) over the H-1 rows of vy (integers, exact in float);
+// vx(0, y) = vx.b(0, y) = 0.07 H / inletArea * flag(0, y).
+__global__ void k_inlet(const uint8_t *__restrict__ fl, float *__restrict__ vxf,
+                        float *__restrict__ vxb, int W, int H) {
+  __shared__ int red[4];
+  int c = 0;
+  for (int y = threadIdx.x; y < H - 1; y += 256) c += fl[(size_t)y * W];
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  const float area = 1.0f + (float)(red[0] + red[1] + red[2] + red[3]);
+  const float vin = 0.07f * (float)H / area;
+  for (int y = threadIdx.x; y < H; y += 256) {
+    const float v = vin * (float)fl[(size_t)y * W];
+    vxf[(size_t)y * (W - 1)] = v;
+    vxb[(size_t)y * (W - 1)] = v;
+  }
+}
+
+// Terrain's flags (terrain.cpp:34-45): full resolution = the image's red
+// channel / 255 with rows flipped; the simulation flag (Terrain::subSample,
+// terrain.cpp:3-11) = the mean of a scale x scale block, summed in row-major
+// order, above 0.99.
+__global__ void k_terrain_full(const uint8_t *__restrict__ rgba, int iw, int ih,
+                               float *__restrict__ full) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= iw) return;
+  full[(size_t)y * iw + x] = (float)rgba[4 * ((size_t)(ih - y - 1) * iw + x)] / 255.0f;
+}
+
+__global__ void k_terrain_subsample(const float *__restrict__ full, int iw, int scale,
+                                    float *__restrict__ sim, int sw) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= sw) return;
+  float sum = 0.0f;
+  for (int sy = 0; sy < scale; sy++)
+    for (int sx = 0; sx < scale; sx++) sum += full[(size_t)(y * scale + sy) * iw + x * scale + sx];
+  sim[(size_t)y * sw + x] = (sum / (float)scale / (float)scale) > 0.99f ? 1.0f : 0.0f;
+}
+
 // project's divergence (simulation.cpp:166-171), interior, not flag-masked.
 __global__ void k_divergence(const float *__restrict__ xf, const float *__restrict__ yf,
                              float *__restrict__ f, int W, int H, float ih) {
@@ -440,6 +563,28 @@ void launch_correct_velocity(const VelView &v, const float *p, const uint8_t *fl
                              hipStream_t s) {
   hipLaunchKernelGGL(k_correct_velocity, dim3(ceil_div(v.W - 2, TPB), v.H - 2), dim3(TPB), 0, s,
                      v.vxf, v.vyf, p, flag, v.W, v.H, ih);
+}
+void launch_shift_map(const VelView &v, float *p, const uint8_t *new_flag, uint8_t *flag,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_shift_rows, dim3(v.H, 3), dim3(TPB), sizeof(float) * v.W, s, v.vxf, v.vxb,
+                     v.vyf, v.vyb, p, v.W, v.H);
+  hipLaunchKernelGGL(k_apply_flags, dim3(ceil_div(v.W, TPB), v.H), dim3(TPB), 0, s, new_flag, flag,
+                     v.vxf, v.vyf, p, v.W, v.H);
+  hipLaunchKernelGGL(k_inlet, dim3(1), dim3(256), 0, s, flag, v.vxf, v.vxb, v.W, v.H);
+}
+void launch_terrain_flags(const uint8_t *rgba, int iw, int ih, int scale, float *full, float *sim,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_terrain_full, dim3(ceil_div(iw, TPB), ih), dim3(TPB), 0, s, rgba, iw, ih,
+                     full);
+  const int sw = iw / scale, sh = ih / scale;
+  if (sw <= 0 || sh <= 0) return;
+  hipLaunchKernelGGL(k_terrain_subsample, dim3(ceil_div(sw, TPB), sh), dim3(TPB), 0, s, full, iw,
+                     scale, sim, sw);
+}
+void launch_collocate(const float *vx, const float *vy, float *vxy, float *mag, int W, int H,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_collocate, dim3(ceil_div(2 * W - 1, TPB), 2 * H - 1), dim3(TPB), 0, s, vx,
+                     vy, vxy, mag, W, H);
 }
 void launch_bilinear_vel(const VelView &v, const float *c, float *out, int n, hipStream_t s) {
   if (n <= 0) return;

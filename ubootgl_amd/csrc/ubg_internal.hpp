@@ -111,6 +111,17 @@ void launch_stamp_sinks(float *f, int W, const int *xy, const float *val, int n,
 void launch_correct_velocity(const VelView &v, const float *p, const uint8_t *flag, float ih,
                              hipStream_t s);
 void launch_bilinear_vel(const VelView &v, const float *c, float *out, int n, hipStream_t s);
+// the simulation part of a map shift: fields shifted one cell left, new
+// flags with setGrids zeroing, inlet velocity (front and back vx)
+void launch_shift_map(const VelView &v, float *p, const uint8_t *new_flag, uint8_t *flag,
+                      hipStream_t s);
+// terrain flags from an RGBA image's red channel: full resolution (iw x ih)
+// and the simulation flag ((iw/scale) x (ih/scale))
+void launch_terrain_flags(const uint8_t *rgba, int iw, int ih, int scale, float *full, float *sim,
+                          hipStream_t s);
+// (2W-1) x (2H-1) collocated (vx, vy) pairs and magnitudes from vx, vy
+void launch_collocate(const float *vx, const float *vy, float *vxy, float *mag, int W, int H,
+                      hipStream_t s);
 void launch_tracers(const VelView &v, const uint8_t *flag, float h, float pwidth, float dt,
                     uint32_t seed, float *xy, float *age, const int *perm, int n,
                     hipStream_t s);

@@ -263,6 +263,23 @@ class Simulation:
             check(lib().ubg_sim_get_sinks(self._h, fptr(out), n.value, C.byref(n)))
         return out
 
+    def shiftMap(self, new_flag):
+        """The simulation part of UbootGlApp::shiftMap (ubootgl_app.cpp:248-296):
+        fields shift one cell left, the new flag is applied with setGrids,
+        inlet velocity, current fields saved, MG reclassified."""
+        check(lib().ubg_sim_shift_map(self._h, fptr(_f32(new_flag))))
+
+    def collocated(self):
+        """(vxy, mag): the (2H-1) x (2W-1) x 2 collocated velocity image and its
+        magnitude from vx_current / vy_current (VelocityTextures::
+        updateFromStaggered, velocity_textures.cpp:63-93)."""
+        W, H = self.width, self.height
+        vxy = np.zeros((2 * H - 1, 2 * W - 1, 2), np.float32)
+        mag = np.zeros((2 * H - 1, 2 * W - 1), np.float32)
+        check(lib().ubg_sim_collocate(self._h, vxy.ctypes.data_as(C.POINTER(C.c_float)),
+                                      fptr(mag), 0))
+        return vxy, mag
+
     def bilinearVel(self, c):
         c = _f32(c).reshape(-1, 2)
         out = np.zeros_like(c)
@@ -308,3 +325,15 @@ class Simulation:
             self.close()
         except Exception:
             pass
+
+
+def terrain_flags(rgba, scale):
+    """Terrain flags from a decoded RGBA8 image (H x W x 4, row 0 at the top):
+    (flag_full (H x W), flag_sim (H/scale x W/scale)) -- terrain.cpp:3-45."""
+    rgba = np.ascontiguousarray(rgba, np.uint8)
+    H, W = rgba.shape[:2]
+    full = np.zeros((H, W), np.float32)
+    sim = np.zeros((H // scale, W // scale), np.float32)
+    check(lib().ubg_terrain_flags(rgba.ctypes.data_as(C.POINTER(C.c_uint8)), W, H, int(scale),
+                                  fptr(full), fptr(sim)))
+    return full, sim
