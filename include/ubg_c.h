@@ -316,6 +316,26 @@ UBG_API int ubg_sim_get_mg(ubg_sim *sim, ubg_mg **mg);
 /* bilinearVel(c) (advect_floating_items.cpp:11-14) for n grid-space points
  * c = (x0,y0,x1,y1,...); host arrays. */
 UBG_API int ubg_sim_bilinear_vel(ubg_sim *sim, const float *c, float *out, int n);
+/* Floating items (components.hpp:6-42: CoItem + CoKinematics /
+ * CoKinematicsSimple), physical coordinates. */
+typedef struct ubg_item {
+  float size[2];
+  float pos[2];
+  float rotation;
+  float mass;
+  float vel[2];
+  float force[2];
+  float ang_vel;
+  float ang_force;
+  int bump_count;
+} ubg_item;
+/* Simulation::advectFloatingItems (simple = 0) or advectFloatingItemsSimple
+ * (simple = 1) (advect_floating_items.cpp:16-274) for n items in the order
+ * the reference's entity view visits them: the items are integrated on the
+ * device (one thread each, items updated in place) and their reaction is
+ * scattered into vx_accum / vy_accum (float atomics: the sums' order differs
+ * from the reference's sequential loop), to be applied by the next step. */
+UBG_API int ubg_sim_advect_items(ubg_sim *sim, ubg_item *items, int n, float game_dt, int simple);
 /* The simulation part of UbootGlApp::shiftMap (ubootgl_app.cpp:248-296),
  * called after Terrain::shiftMap: vx (both buffers, from the front one), vy
  * and p shift one cell left, setGrids(c, new_flag(c)) for every cell

@@ -74,6 +74,9 @@ def lib():
         "oracle_sim_bilinear_vel": (None, [C.c_void_p, _f32p, _f32p, C.c_int]),
         "oracle_flag_linear": (C.c_float, [_f32p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float]),
         "oracle_sim_advect_tracers": (None, [C.c_void_p, _f32p, _f32p, C.c_int, C.c_float, C.c_uint32]),
+        "oracle_sim_advect_items": (None, [C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_int]),
+        "oracle_cephes_sin": (C.c_float, [C.c_float]),
+        "oracle_cephes_cos": (C.c_float, [C.c_float]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -253,6 +256,13 @@ class OracleSim:
         lib().oracle_sim_bilinear_vel(self.h_, c, out, c.shape[0])
         return out
 
+    def advect_items(self, items, game_dt, simple=False):
+        """advect_floating_items.cpp:16-274 on this simulation; items is a
+        numpy array with the ubg_item layout (updated in place)."""
+        lib().oracle_sim_advect_items(self.h_, items.ctypes.data, len(items), float(game_dt),
+                                      int(bool(simple)))
+        return items
+
     def advect_tracers(self, xy, age, dt, seed):
         """xy (n,2) float32 and age (n,) float32 are updated in place."""
         assert xy.dtype == np.float32 and xy.flags.c_contiguous
@@ -375,3 +385,11 @@ def terrain_flags(rgba, scale):
     mean = (acc / np.float32(scale) / np.float32(scale)).astype(np.float32)
     sim[mean > np.float32(0.99)] = 1.0
     return full, sim
+
+
+def cephes_sin(x):
+    return lib().oracle_cephes_sin(float(x))
+
+
+def cephes_cos(x):
+    return lib().oracle_cephes_cos(float(x))

@@ -673,6 +673,266 @@ inline float flag_linear(const float *F, int W, int H, float pwidth, float px,
   return mix(mix(p00, p10, sx), mix(p01, p11, sx), sy);
 }
 
+// ---------------------------------------------------------------------------
+// Floating items (advect_floating_items.cpp:16-274)
+// ---------------------------------------------------------------------------
+
+// glm's vec2 as used there: component-wise ops evaluated left to right;
+// glm::min/max are comparison based, normalize = v * (1 / sqrt(dot)),
+// reflect = I - N * dot(N, I) * 2, rotate = (x c - y s, x s + y c).
+struct Vec2 {
+  float x, y;
+};
+inline Vec2 operator+(Vec2 a, Vec2 b) { return {a.x + b.x, a.y + b.y}; }
+inline Vec2 operator-(Vec2 a, Vec2 b) { return {a.x - b.x, a.y - b.y}; }
+inline Vec2 operator*(Vec2 a, Vec2 b) { return {a.x * b.x, a.y * b.y}; }
+inline Vec2 operator*(Vec2 a, float k) { return {a.x * k, a.y * k}; }
+inline Vec2 operator*(float k, Vec2 a) { return {k * a.x, k * a.y}; }
+inline Vec2 operator/(Vec2 a, float k) { return {a.x / k, a.y / k}; }
+inline float vdot(Vec2 a, Vec2 b) { return a.x * b.x + a.y * b.y; }
+inline float vlength(Vec2 a) { return std::sqrt(vdot(a, a)); }
+inline Vec2 vnormalize(Vec2 a) { return a * (1.0f / std::sqrt(vdot(a, a))); }
+inline Vec2 vreflect(Vec2 i, Vec2 n) { return i - n * vdot(n, i) * 2.0f; }
+inline float gmin(float a, float b) { return (b < a) ? b : a; }
+inline float gmax(float a, float b) { return (a < b) ? b : a; }
+
+// sin / cos by the Cephes single-precision method (S. L. Moshier, Cephes
+// Math Library sinf.c / cosf.c): reduction by pi/4 in three parts, then the
+// degree-7 sine or degree-8 cosine polynomial of the reduced argument.
+const float kFOPI = 1.27323954473516f;
+const float kDP1 = 0.78515625f, kDP2 = 2.4187564849853515625e-4f, kDP3 = 3.77489497744594108e-8f;
+inline float cephes_sin_poly(float x, float z) {
+  return ((-1.9515295891e-4f * z + 8.3321608736e-3f) * z - 1.6666654611e-1f) * z * x + x;
+}
+inline float cephes_cos_poly(float z) {
+  return ((2.443315711809948e-5f * z - 1.388731625493765e-3f) * z + 4.166664568298827e-2f) * z *
+             z - 0.5f * z + 1.0f;
+}
+inline float cephes_reduce(float ax, int &j) {
+  j = (int)(kFOPI * ax);
+  float y = (float)j;
+  if (j & 1) {
+    j += 1;
+    y += 1.0f;
+  }
+  j &= 7;
+  return ((ax - y * kDP1) - y * kDP2) - y * kDP3;
+}
+inline float cephes_sin(float x) {
+  int j;
+  const float r = cephes_reduce(std::fabs(x), j);
+  bool neg = x < 0.0f;
+  if (j > 3) {
+    neg = !neg;
+    j -= 4;
+  }
+  const float z = r * r;
+  const float v = (j == 1 || j == 2) ? cephes_cos_poly(z) : cephes_sin_poly(r, z);
+  return neg ? -v : v;
+}
+inline float cephes_cos(float x) {
+  int j;
+  const float r = cephes_reduce(std::fabs(x), j);
+  bool neg = false;
+  if (j > 3) {
+    j -= 4;
+    neg = !neg;
+  }
+  if (j > 1) neg = !neg;
+  const float z = r * r;
+  const float v = (j == 1 || j == 2) ? cephes_sin_poly(r, z) : cephes_cos_poly(z);
+  return neg ? -v : v;
+}
+inline Vec2 vrotate(Vec2 v, float a) {
+  const float c = cephes_cos(a), sn = cephes_sin(a);
+  return {v.x * c - v.y * sn, v.x * sn + v.y * c};
+}
+
+// components.hpp:6-42 (CoItem + CoKinematics / CoKinematicsSimple), the
+// layout of ubg_item
+struct OItem {
+  float size[2], pos[2], rotation, mass, vel[2], force[2], ang_vel, ang_force;
+  int bump_count;
+};
+
+struct ItemFluid {
+  Sim *s;
+  Vec2 vel(Vec2 c) const { // bilinearVel (advect_floating_items.cpp:11-14)
+    return {bilinear(s->vxf(), s->Wx, s->H, c.x - 0.5f, c.y),
+            bilinear(s->vyf(), s->W, s->Hy, c.x, c.y - 0.5f)};
+  }
+  float flag(Vec2 pc) const { return flag_linear(s->flag.data(), s->W, s->H, s->pwidth, pc.x, pc.y); }
+  Vec2 normal(Vec2 pc) const { // psampleFlagNormal (simulation.cpp:413-420)
+    const float h = s->h;
+    const float p01 = flag({pc.x - h, pc.y + h}), p11 = flag({pc.x + h, pc.y + h});
+    const float p00 = flag({pc.x - h, pc.y - h}), p10 = flag({pc.x + h, pc.y - h});
+    return {p11 + p10 - p01 - p00, p01 + p11 - p00 - p10};
+  }
+  // bilinearScatter (interpolators.hpp:29-40)
+  static void scatter(std::vector<float> &g, int gw, int gh, Vec2 c, float v) {
+    c.x = gmin(gmax(c.x, 0.0f), gw - 1.1f);
+    c.y = gmin(gmax(c.y, 0.0f), gh - 1.1f);
+    const int ix = (int)c.x, iy = (int)c.y;
+    const float sx = c.x - std::floor(c.x), sy = c.y - std::floor(c.y);
+    const float ax = 1.0f - sx, ay = 1.0f - sy;
+    g[(size_t)(iy + 1) * gw + ix + 1] += sx * sy * v;
+    g[(size_t)(iy + 1) * gw + ix] += ax * sy * v;
+    g[(size_t)iy * gw + ix + 1] += sx * ay * v;
+    g[(size_t)iy * gw + ix] += ax * ay * v;
+  }
+  bool skip(Vec2 pos) const { // advect_floating_items.cpp:38-44
+    const Vec2 gp = pos / (s->pwidth / (float)s->W);
+    // a non-finite position counts as outside (the reference indexes the
+    // grid at (int)NaN for such an item)
+    return !(gp.x < s->W - 2 && gp.x > 1.0 && gp.y < s->H - 2 && gp.y > 1.0);
+  }
+};
+
+inline int item_substeps(Vec2 vel, float dt, float h) { // :24-26 (2.5 is a double)
+  return (int)std::fmin(15.0f, std::fmax(1.0f, gmax(std::fabs(vel.x), std::fabs(vel.y)) * dt / h * 2.5));
+}
+
+// Simulation::advectFloatingItems (advect_floating_items.cpp:16-138)
+void advect_item(const ItemFluid &F, OItem &o, float gameDT) {
+  Sim &S = *F.s;
+  const float h = S.h;
+  Vec2 size{o.size[0], o.size[1]}, pos{o.pos[0], o.pos[1]}, vel{o.vel[0], o.vel[1]},
+      force{o.force[0], o.force[1]};
+  const int steps = item_substeps(vel, gameDT, h);
+  const float subDT = gameDT / steps;
+  for (int step = 0; step < steps; step++) {
+    const Vec2 before = pos;
+    pos = pos + subDT * vel;
+    o.rotation = (float)std::fmod(o.rotation + subDT * o.ang_vel + 2 * M_PI, 2 * M_PI);
+    if (F.skip(pos)) continue;
+    force = force + Vec2{0.0f, -0.5f} * o.mass;
+    const Vec2 pts[5] = {{1, 1}, {-1, 1}, {1, -1}, {-1, -1}, {0, 0}};
+    for (const Vec2 &q : pts) {
+      const Vec2 sp = vrotate(q * 0.5f * size, o.rotation);
+      if (F.flag(pos + sp) < 0.5f) {
+        Vec2 n = vnormalize(F.normal(0.5f * (before + pos) + sp));
+        if (F.flag(before + sp) > 0.5f) pos = before;
+        if (vlength(n) > 0.0f) {
+          n = n / vlength(n);
+          if (vdot(vel, n) < 0.0) vel = vreflect(vel, n) * 0.7f;
+          vel = vel + n * 0.07f * F.flag(pos + sp);
+          if (vdot(force, n) < 0.0f) force = force + 1.1f * vdot(force, n) * n;
+        }
+        o.bump_count++;
+      }
+    }
+    const Vec2 external = Vec2{0.0f, -0.5f} * o.mass + force;
+    Vec2 central{0.0f, 0.0f};
+    const float angForce = o.ang_force;
+    const Vec2 surf[4] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+    const float side[4] = {size.y, size.y, size.x, size.x};
+    for (int i = 0; i < 4; i++) {
+      const int nSP = (int)std::fmax(2.0f, side[i] / h);
+      for (int k = 0; k < nSP; k++) {
+        const Vec2 sp = surf[i] + Vec2{std::fabs(surf[i].y), std::fabs(surf[i].x)} *
+                                      (1.0f - k * 2.0f / (nSP - 1));
+        Vec2 t = sp * size * 0.5f;
+        t = vrotate(t, o.rotation);
+        t = t + pos;
+        const Vec2 dv = F.vel(t / h) - (vel + 3.141f * vrotate(sp * size * 0.5f, o.rotation + 0.5f * 3.141f) * o.ang_vel);
+        const Vec2 nrm = vrotate(vnormalize(sp), o.rotation);
+        const Vec2 fc = nrm * gmin(0.0f, vdot(dv, vrotate(surf[i], o.rotation)));
+        central = central + fc * 400000.0f * (0.003f + side[i]) * side[i] / (float)nSP;
+        if (t.x / h < 1.0f || t.x / h > S.Wx - 2.0f || t.y / h < 1.0f || t.y / h > S.H - 2.0f)
+          continue;
+        const Vec2 dvec = fc * (0.003f + side[i]) * side[i] / (float)nSP * subDT * 18000000.0f;
+        ItemFluid::scatter(S.vx_accum, S.Wx, S.H, t / h - Vec2{0.5f, 0.0f}, -dvec.x);
+        ItemFluid::scatter(S.vy_accum, S.W, S.Hy, t / h - Vec2{0.0f, 0.5f}, -dvec.y);
+      }
+    }
+    central = central + 1000.0f * (size.x + size.y) * (F.vel(pos / h) - vel);
+    vel = vel + subDT * (external + central) / o.mass;
+    const float angMass = size.x * size.y * o.mass * (1.0f / 12.0f);
+    o.ang_vel += subDT * angForce / angMass;
+    o.ang_vel *= 0.98;
+  }
+  o.pos[0] = pos.x;
+  o.pos[1] = pos.y;
+  o.vel[0] = vel.x;
+  o.vel[1] = vel.y;
+  o.ang_force = 0.0f;
+  o.force[0] = o.force[1] = 0.0f;
+}
+
+// Simulation::advectFloatingItemsSimple (advect_floating_items.cpp:140-274);
+// snap: the positions the 100 bins hold (taken before any item moves)
+void advect_item_simple(const ItemFluid &F, OItem &o, const std::vector<Vec2> &snap, float gameDT) {
+  Sim &S = *F.s;
+  const float h = S.h;
+  Vec2 size{o.size[0], o.size[1]}, pos{o.pos[0], o.pos[1]}, vel{o.vel[0], o.vel[1]},
+      force{o.force[0], o.force[1]};
+  Vec2 rep{0.0f, 0.0f};
+  int contacts = 0;
+  const int bin = ((int)(pos.x * 100)) % 100;
+  for (const Vec2 &p2 : snap) {
+    if (((int)(p2.x * 100)) % 100 != bin) continue;
+    const Vec2 dis = pos - p2;
+    if (dis.x * dis.x + dis.y * dis.y < (size.x * size.y * 0.4f)) {
+      const float len = gmax(0.1f * size.x, vlength(dis));
+      rep = rep + 0.0001f * (dis / len / len);
+      contacts++;
+    }
+  }
+  rep = rep / (float)std::max(1, contacts);
+  force = force + rep * 10.0f;
+  const int steps = item_substeps(vel, gameDT, h);
+  const float subDT = gameDT / steps;
+  for (int step = 0; step < steps; step++) {
+    pos = pos + subDT * vel;
+    o.rotation = (float)std::fmod(o.rotation + subDT * o.ang_vel + 2 * M_PI, 2 * M_PI);
+    const Vec2 gridPos = pos / (S.pwidth / (float)S.W);
+    if (F.skip(pos)) continue;
+    if (F.flag(pos) < 1.0f) {
+      Vec2 n = F.normal(pos);
+      if (vlength(n) > 0.0f) {
+        n = n / vlength(n);
+        if (vdot(vel, n) < 0.0) vel = vreflect(vel, n) * 0.7f;
+        if (vdot(force, n) < 0.0f) force = vreflect(force, n) * 0.7f;
+        const Vec2 lat = vrotate(n, 0.5f * (float)M_PI);
+        const float latVel = vdot(lat, vel);
+        const float rotVel = o.ang_vel * (size.x + size.y) * 0.5f;
+        const float latDiff = latVel - rotVel;
+        force = force + latDiff * lat * 1.0f;
+      } else {
+        vel = Vec2{0.0f, 0.0f};
+      }
+      o.bump_count++;
+    }
+    Vec2 external = Vec2{0.0f, -0.5f} * o.mass + force;
+    const Vec2 dvel = F.vel(pos / h) - vel;
+    external = external + 2000.0f * (size.x + size.y) * dvel;
+    if (pos.x / h > 1.0f && pos.x / h < S.Wx - 2.0f && pos.y / h < 1.0f && pos.y / h < S.H - 2.0f) {
+      const Vec2 dvec = dvel * size.x * size.y;
+      ItemFluid::scatter(S.vx_accum, S.Wx, S.H, pos / h - Vec2{0.5f, 0.0f}, -dvec.x);
+      ItemFluid::scatter(S.vy_accum, S.W, S.Hy, pos / h - Vec2{0.0f, 0.5f}, -dvec.y);
+    }
+    vel = vel + subDT * external / o.mass;
+    const int gx = (int)gridPos.x, gy = (int)gridPos.y;
+    const float *vx = S.vxf(), *vy = S.vyf(), *p = S.p.data();
+    const float fluidAng = -((vx[(size_t)gy * S.Wx + gx] - vx[(size_t)(gy - 1) * S.Wx + gx]) -
+                             (vy[(size_t)gy * S.W + gx] - vy[(size_t)gy * S.W + gx - 1])) / h / 2;
+    const float angMass = size.x * size.y * o.mass * (1.0f / 12.0f);
+    const float k = gmin(1.0f, subDT / angMass * 0.0005f * (size.x + size.y) / 4.0f);
+    o.ang_vel += k * (fluidAng - o.ang_vel);
+    o.ang_vel += subDT / angMass * o.ang_force;
+    S.vx_accum[(size_t)gy * S.Wx + gx] -= o.ang_vel * k * 0.01f * p[(size_t)gy * S.W + gx];
+    S.vx_accum[(size_t)(gy - 1) * S.Wx + gx] += o.ang_vel * k * 0.01f * p[(size_t)(gy - 1) * S.W + gx];
+    S.vy_accum[(size_t)gy * S.W + gx] += o.ang_vel * k * 0.01f * p[(size_t)gy * S.W + gx];
+    S.vy_accum[(size_t)gy * S.W + gx - 1] -= o.ang_vel * k * 0.01f * p[(size_t)gy * S.W + gx - 1];
+  }
+  o.pos[0] = pos.x;
+  o.pos[1] = pos.y;
+  o.vel[0] = vel.x;
+  o.vel[1] = vel.y;
+  o.ang_force = 0.0f;
+  o.force[0] = o.force[1] = 0.0f;
+}
+
 } // namespace
 
 // ===========================================================================
@@ -950,3 +1210,20 @@ ORACLE_API void oracle_sim_advect_tracers(void *sp, float *xy, float *age, int n
     age[i] = a;
   }
 }
+
+// advectFloatingItems / advectFloatingItemsSimple on the oracle simulation:
+// items (ubg_item layout) in order, accumulation into vx_accum / vy_accum.
+ORACLE_API void oracle_sim_advect_items(void *sp, void *items, int n, float gameDT, int simple) {
+  Sim *s = (Sim *)sp;
+  OItem *it = (OItem *)items;
+  ItemFluid F{s};
+  std::vector<Vec2> snap;
+  for (int i = 0; i < n; i++) snap.push_back({it[i].pos[0], it[i].pos[1]});
+  for (int i = 0; i < n; i++) {
+    if (simple) advect_item_simple(F, it[i], snap, gameDT);
+    else advect_item(F, it[i], gameDT);
+  }
+}
+
+ORACLE_API float oracle_cephes_sin(float x) { return cephes_sin(x); }
+ORACLE_API float oracle_cephes_cos(float x) { return cephes_cos(x); }

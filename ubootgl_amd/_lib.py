@@ -37,6 +37,12 @@ LEVEL_P, LEVEL_F, LEVEL_R = range(3)
 TIMER_SMOOTH_L0, TIMER_L0_PRE, TIMER_L0_POST = range(3)
 
 
+# ubg_item (components.hpp:6-42): CoItem + CoKinematics
+ITEM_DTYPE = np.dtype([("size", "<f4", 2), ("pos", "<f4", 2), ("rotation", "<f4"), ("mass", "<f4"),
+                       ("vel", "<f4", 2), ("force", "<f4", 2), ("ang_vel", "<f4"),
+                       ("ang_force", "<f4"), ("bump_count", "<i4")])
+
+
 class UbgError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{ERROR_NAMES.get(code, code)}: {msg}")
@@ -162,6 +168,7 @@ SIGNATURES = {
     "ubg_sim_bilinear_vel": (C.c_int, [_vp, _fp, _fp, C.c_int]),
     "ubg_sim_collocate": (C.c_int, [_vp, _fp, _fp, C.c_int]),
     "ubg_sim_shift_map": (C.c_int, [_vp, _fp]),
+    "ubg_sim_advect_items": (C.c_int, [_vp, _vp, C.c_int, C.c_float, C.c_int]),
     "ubg_terrain_flags": (C.c_int, [C.POINTER(C.c_uint8), C.c_int, C.c_int, C.c_int, _fp, _fp]),
     "ubg_sim_set_tracers": (C.c_int, [_vp, _fp, _fp, C.c_int]),
     "ubg_sim_get_tracers": (C.c_int, [_vp, _fp, _fp, C.c_int]),

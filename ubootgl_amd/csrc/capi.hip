@@ -866,6 +866,35 @@ UBG_API int ubg_sim_bilinear_vel(ubg_sim *s, const float *c, float *out, int n) 
   return UBG_OK;
 }
 
+UBG_API int ubg_sim_advect_items(ubg_sim *s, ubg_item *items, int n, float game_dt, int simple) {
+  if (!s || n < 0 || (n > 0 && !items)) return fail(UBG_E_INVALID, "bad argument");
+  if (!std::isfinite(game_dt)) return fail(UBG_E_INVALID, "game_dt must be finite");
+  if (n == 0) return UBG_OK;
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  const size_t bytes = sizeof(ubg_item) * n + sizeof(float) * 2 * n;
+  char *d = nullptr;
+  UBG_HIP(hipMalloc((void **)&d, bytes));
+  ubg_item *di = (ubg_item *)d;
+  float *snap = (float *)(d + sizeof(ubg_item) * n);
+  std::vector<float> pos(2 * (size_t)n);
+  for (int i = 0; i < n; i++) {
+    pos[2 * i] = items[i].pos[0];
+    pos[2 * i + 1] = items[i].pos[1];
+  }
+  hipMemcpyAsync(di, items, sizeof(ubg_item) * n, hipMemcpyHostToDevice, s->stream);
+  hipMemcpyAsync(snap, pos.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice, s->stream);
+  launch_items(di, snap, n, s->vx[s->fx], s->vy[s->fy], s->P(), s->flag, s->vx_acc, s->vy_acc, s->W,
+               s->H, s->h, s->pwidth, game_dt, simple != 0, s->stream);
+  s->accum_dirty = true; // the next step adds the items' reaction to the velocity
+  int rc = check_launch();
+  hipMemcpyAsync(items, di, sizeof(ubg_item) * n, hipMemcpyDeviceToHost, s->stream);
+  hipError_t e = hipStreamSynchronize(s->stream);
+  (void)hipFree(d);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(UBG_E_HIP, hipGetErrorString(e));
+  return UBG_OK;
+}
+
 UBG_API int ubg_sim_shift_map(ubg_sim *s, const float *new_flag) {
   if (!s || !new_flag) return fail(UBG_E_INVALID, "null argument");
   if (s->W > 16384) return fail(UBG_E_INVALID, "map shift needs W <= 16384");

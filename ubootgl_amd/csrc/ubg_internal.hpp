@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+struct ubg_item;
+
 namespace ubg {
 
 // Kernel timing: an event pair set here is attached to the next launch made
@@ -119,6 +121,11 @@ void launch_shift_map(const VelView &v, float *p, const uint8_t *new_flag, uint8
 // and the simulation flag ((iw/scale) x (ih/scale))
 void launch_terrain_flags(const uint8_t *rgba, int iw, int ih, int scale, float *full, float *sim,
                           hipStream_t s);
+// floating items (one thread each), scatter into ax / ay; snap: 2n input
+// positions (bins of the simple kind)
+void launch_items(ubg_item *items, const float *snap, int n, const float *vx, const float *vy,
+                  const float *p, const uint8_t *flag, float *ax, float *ay, int W, int H, float h,
+                  float pwidth, float game_dt, bool simple, hipStream_t s);
 // (2W-1) x (2H-1) collocated (vx, vy) pairs and magnitudes from vx, vy
 void launch_collocate(const float *vx, const float *vy, float *vxy, float *mag, int W, int H,
                       hipStream_t s);

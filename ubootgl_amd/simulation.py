@@ -263,6 +263,16 @@ class Simulation:
             check(lib().ubg_sim_get_sinks(self._h, fptr(out), n.value, C.byref(n)))
         return out
 
+    def advectFloatingItems(self, items, game_dt, simple=False):
+        """Simulation::advectFloatingItems / advectFloatingItemsSimple
+        (advect_floating_items.cpp:16-274): items (numpy array of
+        ITEM_DTYPE, physical coordinates) are updated in place; their
+        reaction goes into vx_accum / vy_accum."""
+        assert items.dtype == L.ITEM_DTYPE and items.flags.c_contiguous
+        check(lib().ubg_sim_advect_items(self._h, items.ctypes.data_as(C.c_void_p), len(items),
+                                         float(game_dt), int(bool(simple))))
+        return items
+
     def shiftMap(self, new_flag):
         """The simulation part of UbootGlApp::shiftMap (ubootgl_app.cpp:248-296):
         fields shift one cell left, the new flag is applied with setGrids,
