@@ -21,6 +21,7 @@
 #include <array>
 #include <mutex>
 #include <sstream>
+#include <type_traits>
 #include <vector>
 
 class Simulation {
@@ -81,6 +82,65 @@ public:
     vy_accum.fill(0.0f);
   }
   void saveCurrentVelocityFields() { stage(UBG_STAGE_SAVE_CURRENT); }
+
+  // advect_floating_items.cpp:16-274.  The reference takes an entt::registry
+  // and views CoItem + CoKinematics (or CoKinematicsSimple); this header does
+  // not depend on entt or the game's components, so the game keeps two
+  // forwarders:
+  //   void advectFloatingItems(entt::registry &r, float dt)
+  //     { advectFloatingItemsView<CoItem, CoKinematics>(r, dt, false); }
+  //   void advectFloatingItemsSimple(entt::registry &r, float dt)
+  //     { advectFloatingItemsView<CoItem, CoKinematicsSimple>(r, dt, true); }
+  // The view's items are packed in visiting order, integrated on the device
+  // and written back; their reaction lands in the device-side accumulation
+  // fields, which the next step applies (the host mirrors vx_accum / vy_accum
+  // are not involved).
+  template <class ItemC, class KinC, class Registry>
+  void advectFloatingItemsView(Registry &registry, float gameDT, bool simple) {
+    auto view = registry.template view<ItemC, KinC>();
+    using Entity = std::decay_t<decltype(*view.begin())>;
+    std::vector<Entity> ents;
+    std::vector<ubg_item> items;
+    for (auto e : view) {
+      const ItemC &it = view.template get<ItemC>(e);
+      const KinC &k = view.template get<KinC>(e);
+      ubg_item u{};
+      u.size[0] = it.size.x;
+      u.size[1] = it.size.y;
+      u.pos[0] = it.pos.x;
+      u.pos[1] = it.pos.y;
+      u.rotation = it.rotation;
+      u.mass = k.mass;
+      u.vel[0] = k.vel.x;
+      u.vel[1] = k.vel.y;
+      u.force[0] = k.force.x;
+      u.force[1] = k.force.y;
+      u.ang_vel = k.angVel;
+      u.ang_force = k.angForce;
+      u.bump_count = k.bumpCount;
+      items.push_back(u);
+      ents.push_back(e);
+    }
+    {
+      std::scoped_lock lock(accum_mutex);
+      ubg_check(ubg_sim_advect_items(h_, items.data(), (int)items.size(), gameDT, simple ? 1 : 0));
+    }
+    for (size_t i = 0; i < ents.size(); i++) {
+      ItemC &it = view.template get<ItemC>(ents[i]);
+      KinC &k = view.template get<KinC>(ents[i]);
+      const ubg_item &u = items[i];
+      it.pos.x = u.pos[0];
+      it.pos.y = u.pos[1];
+      it.rotation = u.rotation;
+      k.vel.x = u.vel[0];
+      k.vel.y = u.vel[1];
+      k.force.x = u.force[0];
+      k.force.y = u.force[1];
+      k.angVel = u.ang_vel;
+      k.angForce = u.ang_force;
+      k.bumpCount = u.bump_count;
+    }
+  }
 
   // simulation.hpp:82-98
   template <typename V> void setGrids(const V &c, float val) {

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <type_traits>
 #include <vector>
 
 extern "C" {
@@ -15,6 +16,7 @@ void oracle_sim_destroy(void *s);
 int oracle_sim_set_field(void *s, int id, const float *src);
 int oracle_sim_get_field(void *s, int id, float *dst);
 void oracle_sim_step(void *s, float dt);
+void oracle_sim_advect_items(void *s, void *items, int n, float gameDT, int simple);
 void *oracle_mg_create(int W, int H, int fp64);
 void oracle_mg_update_fields(void *m, const float *flag);
 void oracle_mg_solve(void *m, float *p, const float *f, const float *flag, float h, int zg);
@@ -28,6 +30,46 @@ static double rel(const float *a, const float *b, size_t n) {
   }
   return std::sqrt(num / (den > 0 ? den : 1));
 }
+
+// A registry with the interface the drop-in's item forwarder uses
+// (view<A, B>(), iteration over entities, view.get<T>(entity)); component
+// layouts as components.hpp:6-27.
+struct V2 {
+  float x, y;
+};
+struct CoItem {
+  V2 size, pos;
+  float rotation;
+};
+struct CoKinematics {
+  float mass;
+  V2 vel, force;
+  float angVel, angForce;
+  int bumpCount;
+};
+struct MiniRegistry {
+  std::vector<CoItem> items;
+  std::vector<CoKinematics> kins;
+  struct View {
+    MiniRegistry *r;
+    struct It {
+      int i;
+      int operator*() const { return i; }
+      It &operator++() {
+        ++i;
+        return *this;
+      }
+      bool operator!=(const It &o) const { return i != o.i; }
+    };
+    It begin() const { return {0}; }
+    It end() const { return {(int)r->items.size()}; }
+    template <class T> T &get(int e) const {
+      if constexpr (std::is_same_v<T, CoItem>) return r->items[e];
+      else return r->kins[e];
+    }
+  };
+  template <class A, class B> View view() { return View{this}; }
+};
 
 static int fails = 0;
 static void expect(bool ok, const char *what, double v) {
@@ -100,6 +142,44 @@ int main() {
     double rp = rel(sim.p.data(), op.data(), op.size());
     expect(rv < 1e-5, "Simulation::step vx matches the oracle", rv);
     expect(rp < 1e-5, "Simulation::step p matches the oracle", rp);
+    // floating items through the registry forwarder, against the oracle on
+    // the same fields (advect_floating_items.cpp:16-138)
+    {
+      MiniRegistry reg;
+      std::vector<ubg_item> ref;
+      for (int i = 0; i < 24; i++) {
+        CoItem it{{0.009f, 0.0022f}, {(10.0f + 3.1f * i) / W, (8.0f + 1.3f * i) / W}, 0.3f * i};
+        CoKinematics k{1.0f, {0.01f * (i % 5) - 0.02f, 0.005f * (i % 3)}, {0.0f, 0.0f}, 0.1f * i, 0.0f, 0};
+        reg.items.push_back(it);
+        reg.kins.push_back(k);
+        ubg_item u{};
+        u.size[0] = it.size.x;
+        u.size[1] = it.size.y;
+        u.pos[0] = it.pos.x;
+        u.pos[1] = it.pos.y;
+        u.rotation = it.rotation;
+        u.mass = k.mass;
+        u.vel[0] = k.vel.x;
+        u.vel[1] = k.vel.y;
+        u.ang_vel = k.angVel;
+        ref.push_back(u);
+      }
+      for (int frame = 0; frame < 2; frame++) {
+        sim.advectFloatingItemsView<CoItem, CoKinematics>(reg, 1.0f / 60, false);
+        oracle_sim_advect_items(os, ref.data(), (int)ref.size(), 1.0f / 60, 0);
+      }
+      double worst = 0;
+      int bumps = 0;
+      for (size_t i = 0; i < ref.size(); i++) {
+        worst = std::fmax(worst, std::fabs(reg.items[i].pos.x - ref[i].pos[0]));
+        worst = std::fmax(worst, std::fabs(reg.items[i].pos.y - ref[i].pos[1]));
+        worst = std::fmax(worst, std::fabs(reg.kins[i].vel.x - ref[i].vel[0]));
+        worst = std::fmax(worst, std::fabs(reg.kins[i].vel.y - ref[i].vel[1]));
+        bumps += reg.kins[i].bumpCount;
+      }
+      expect(worst < 1e-6, "advectFloatingItems (registry view) matches the oracle", worst);
+      expect(bumps > 0, "items hit the disk or the walls", bumps);
+    }
     // the game's per-frame flag update (ubootgl_app.cpp:111-112)
     sim.flag(10, 10) = 0.0f;
     sim.mg.updateFields(sim.flag);

@@ -331,6 +331,22 @@ struct ubg_sim {
   int *d_sink_xy = nullptr;
   float *d_sink_val = nullptr;
   int sink_cap = 0;
+  // device staging for the entry points that take host arrays; each of them
+  // ends with a stream synchronize, so one buffer serves all (grows only)
+  char *stage = nullptr;
+  size_t stage_cap = 0;
+  int staging(size_t bytes, void **out) {
+    if (bytes > stage_cap) {
+      UBG_HIP(hipStreamSynchronize(stream));
+      if (stage) (void)hipFree(stage);
+      stage = nullptr;
+      stage_cap = 0;
+      UBG_HIP(hipMalloc((void **)&stage, bytes));
+      stage_cap = bytes;
+    }
+    *out = stage;
+    return UBG_OK;
+  }
   // tracers are stored sorted by 16x16-cell tile of their position (gathers
   // of one wave then touch few cache lines); tr_perm[slot] = tracer id.
   float *tr_xy = nullptr, *tr_age = nullptr;
@@ -632,6 +648,7 @@ UBG_API int ubg_sim_destroy(ubg_sim *s) {
   if (s->tr_xy) (void)hipFree(s->tr_xy);
   if (s->tr_age) (void)hipFree(s->tr_age);
   if (s->tr_perm) (void)hipFree(s->tr_perm);
+  if (s->stage) (void)hipFree(s->stage);
   (void)hipStreamDestroy(s->stream);
   s->pool.release();
   delete s;
@@ -854,13 +871,12 @@ UBG_API int ubg_sim_bilinear_vel(ubg_sim *s, const float *c, float *out, int n) 
   if (n == 0) return UBG_OK;
   std::lock_guard<std::recursive_mutex> g(s->mu_);
   float *d = nullptr;
-  UBG_HIP(hipMalloc((void **)&d, sizeof(float) * 4 * n));
+  UBG_TRY(s->staging(sizeof(float) * 4 * n, (void **)&d));
   hipMemcpyAsync(d, c, sizeof(float) * 2 * n, hipMemcpyHostToDevice, s->stream);
   launch_bilinear_vel(s->vel(), d, d + 2 * n, n, s->stream);
   int rc = check_launch();
   hipMemcpyAsync(out, d + 2 * n, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, s->stream);
   hipError_t e = hipStreamSynchronize(s->stream);
-  (void)hipFree(d);
   if (rc) return rc;
   if (e != hipSuccess) return fail(UBG_E_HIP, hipGetErrorString(e));
   return UBG_OK;
@@ -873,7 +889,7 @@ UBG_API int ubg_sim_advect_items(ubg_sim *s, ubg_item *items, int n, float game_
   std::lock_guard<std::recursive_mutex> g(s->mu_);
   const size_t bytes = sizeof(ubg_item) * n + sizeof(float) * 2 * n;
   char *d = nullptr;
-  UBG_HIP(hipMalloc((void **)&d, bytes));
+  UBG_TRY(s->staging(bytes, (void **)&d));
   ubg_item *di = (ubg_item *)d;
   float *snap = (float *)(d + sizeof(ubg_item) * n);
   std::vector<float> pos(2 * (size_t)n);
@@ -889,7 +905,6 @@ UBG_API int ubg_sim_advect_items(ubg_sim *s, ubg_item *items, int n, float game_
   int rc = check_launch();
   hipMemcpyAsync(items, di, sizeof(ubg_item) * n, hipMemcpyDeviceToHost, s->stream);
   hipError_t e = hipStreamSynchronize(s->stream);
-  (void)hipFree(d);
   if (rc) return rc;
   if (e != hipSuccess) return fail(UBG_E_HIP, hipGetErrorString(e));
   return UBG_OK;
@@ -902,14 +917,13 @@ UBG_API int ubg_sim_shift_map(ubg_sim *s, const float *new_flag) {
   std::vector<uint8_t> f8;
   UBG_TRY(validate_flag(new_flag, (size_t)s->W * s->H, f8));
   uint8_t *d = nullptr;
-  UBG_HIP(hipMalloc((void **)&d, f8.size()));
+  UBG_TRY(s->staging(f8.size(), (void **)&d));
   hipMemcpyAsync(d, f8.data(), f8.size(), hipMemcpyHostToDevice, s->stream);
   launch_shift_map(s->vel(), s->P(), d, s->flag, s->stream);
   s->save_current();              // ubootgl_app.cpp:292
   s->mg->classify();              // ubootgl_app.cpp:295, mg.updateFields
   int rc = check_launch();
   hipError_t e = hipStreamSynchronize(s->stream);
-  (void)hipFree(d);
   if (rc) return rc;
   if (e != hipSuccess) return fail(UBG_E_HIP, hipGetErrorString(e));
   return UBG_OK;
@@ -957,13 +971,12 @@ UBG_API int ubg_sim_collocate(ubg_sim *s, float *vxy, float *mag, int on_device)
     return UBG_OK;
   }
   float *d = nullptr;
-  UBG_HIP(hipMalloc((void **)&d, sizeof(float) * 3 * n));
+  UBG_TRY(s->staging(sizeof(float) * 3 * n, (void **)&d));
   launch_collocate(s->vx_cur, s->vy_cur, d, d + 2 * n, s->W, s->H, s->stream);
   int rc = check_launch();
   hipMemcpyAsync(vxy, d, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, s->stream);
   hipMemcpyAsync(mag, d + 2 * n, sizeof(float) * n, hipMemcpyDeviceToHost, s->stream);
   hipError_t e = hipStreamSynchronize(s->stream);
-  (void)hipFree(d);
   if (rc) return rc;
   if (e != hipSuccess) return fail(UBG_E_HIP, hipGetErrorString(e));
   return UBG_OK;
