@@ -47,8 +47,18 @@ inline int check_launch() {
 
 inline size_t align_up(size_t n, size_t a = 256) { return (n + a - 1) / a * a; }
 
+#ifndef UBG_FUSE_MIN_LOG2
+#define UBG_FUSE_MIN_LOG2 22
+#endif
+#ifndef UBG_COARSE_MAX_CELLS
+#define UBG_COARSE_MAX_CELLS 4096
+#endif
 // levels with at least this many cells run the fused two-pass level kernels
-constexpr size_t kFuseMinCells = (size_t)1 << 22;
+constexpr size_t kFuseMinCells = (size_t)1 << UBG_FUSE_MIN_LOG2;
+// the single-workgroup LDS V-cycle starts at the first level with at most
+// this many cells (one CU's lanes do all of its work, so larger levels run
+// faster as tiles spread over the chip)
+constexpr size_t kCoarseMaxCells = UBG_COARSE_MAX_CELLS;
 
 // A simple bump allocator over one hipMalloc.
 struct Pool {
@@ -181,17 +191,24 @@ struct ubg_mg {
       sweeps(l, prm.nu_coarse, h, false); // no ZG on the coarsest solved level
       return;
     }
-    if (l > 0 && coarse_fits_lds(lv.data(), l, levels)) {
+    if (l > 0 && (size_t)lv[l].w * lv[l].h <= kCoarseMaxCells &&
+        coarse_fits_lds(lv.data(), l, levels)) {
       launch_coarse_vcycle(lv.data(), l, levels, h, prm.nu_pre, prm.nu_post, prm.nu_coarse,
                            prm.alpha, stream);
       return;
     }
     Level &L = lv[l], &C = lv[l + 1];
     if ((size_t)L.w * L.h < kFuseMinCells) {
-      // small level: few waves, so short per-wave work wins over fewer passes
+      float hc = h * (L.w - 1.0f) / (C.w - 1.0f); // pressure_solver.cpp:229
+      if (!z && tile_pass_ok(prm.nu_pre) && tile_pass_ok(prm.nu_post)) {
+        // small level: two LDS-tiled launches instead of 2 + nu_pre + nu_post
+        launch_tile_pre(L, C, prm.nu_pre, h, prm.alpha, stream);
+        solve_level(l + 1, hc, false);
+        launch_tile_post(L, C, prm.nu_post, h, prm.alpha, stream);
+        return;
+      }
       sweeps(l, prm.nu_pre, h, z);
       launch_residual_restrict(L, C, h, stream);
-      float hc = h * (L.w - 1.0f) / (C.w - 1.0f); // pressure_solver.cpp:229
       solve_level(l + 1, hc, false);
       launch_prolong_correct(L, C, stream);
       if (z) launch_zero_gradient(L.p, L.w, L.h, stream);

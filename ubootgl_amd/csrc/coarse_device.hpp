@@ -17,6 +17,7 @@ struct CoarseArgs {
   int w[MAXL], h[MAXL];
   float hh[MAXL];
   int op[MAXL], of[MAXL], om[MAXL]; // LDS byte offsets of p, f, mask
+  int orr;                           // residual scratch (first level's size)
   const uint8_t *mask[MAXL];
   const float *f0;
   float *p0;
@@ -52,16 +53,36 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseArgs &a, const fl
                                                   float *p0) {
   extern __shared__ __align__(16) char smem[];
   const int tid = threadIdx.x;
-  // load: f of the first level, masks of all levels; p = 0 everywhere
+  // load: f of the first level, masks of all levels; p = 0 everywhere.
+  // Each round issues KC loads per thread before its LDS stores, so a round
+  // costs one memory latency instead of one per element.
+  constexpr int KC = 8;
   for (int l = 0; l < a.n; l++) {
     float *P = (float *)(smem + a.op[l]);
     float *Fv = (float *)(smem + a.of[l]);
     uint8_t *M = (uint8_t *)(smem + a.om[l]);
-    int n = a.w[l] * a.h[l];
-    for (int i = tid; i < n; i += TPB_C) {
-      P[i] = 0.0f;
-      Fv[i] = (l == 0) ? f0[i] : 0.0f;
-      M[i] = a.mask[l][i];
+    const uint8_t *mg = a.mask[l];
+    const int n = a.w[l] * a.h[l];
+    for (int base = 0; base < n; base += KC * TPB_C) {
+      float vf[KC];
+      uint8_t vm[KC];
+#pragma unroll
+      for (int k = 0; k < KC; k++) {
+        const int i = base + tid + k * TPB_C;
+        if (i < n) {
+          vf[k] = (l == 0) ? f0[i] : 0.0f;
+          vm[k] = mg[i];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < KC; k++) {
+        const int i = base + tid + k * TPB_C;
+        if (i < n) {
+          P[i] = 0.0f;
+          Fv[i] = vf[k];
+          M[i] = vm[k];
+        }
+      }
     }
   }
   __syncthreads();
@@ -73,15 +94,21 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseArgs &a, const fl
     const uint8_t *M = (const uint8_t *)(smem + a.om[l]);
     const int w = a.w[l], h = a.h[l];
     for (int s = 0; s < a.nu_pre; s++) lds_sweep(P, Fv, M, w, h, a.hh[l], a.alpha);
-    // residual + restrict into level l+1 (its p starts at 0)
+    // residual (pressure_solver.cpp:91-116) of every interior cell once,
+    // then restrict (:118-132) into level l+1, whose p starts at 0
+    float *Rr = (float *)(smem + a.orr);
     float *Fc = (float *)(smem + a.of[l + 1]);
     float *Pc = (float *)(smem + a.op[l + 1]);
     const int wc = a.w[l + 1], hc = a.h[l + 1];
     const float ihsq = 1.0f / a.hh[l] / a.hh[l];
+    for (int y = 1 + (tid >> 6); y < h - 1; y += 16)
+      for (int x = 1 + (tid & 63); x < w - 1; x += 64)
+        Rr[y * w + x] = lds_residual(P, Fv, M, y * w + x, w, ihsq);
+    __syncthreads();
     for (int y = 1 + (tid >> 6); y < hc - 1; y += 16)
     for (int x = 1 + (tid & 63); x < wc - 1; x += 64) {
       int c = (2 * y) * w + 2 * x;
-      auto R = [&](int dx, int dy) { return lds_residual(P, Fv, M, c + dy * w + dx, w, ihsq); };
+      auto R = [&](int dx, int dy) { return Rr[c + dy * w + dx]; };
       float v = 0.0f;
       v = R(-1, -1) * 1.0f + R(0, -1) * 2.0f + R(1, -1) * 1.0f;
       v += R(-1, 0) * 2.0f + R(0, 0) * 4.0f + R(1, 0) * 2.0f;
@@ -108,27 +135,36 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseArgs &a, const fl
     const float *Pc = (const float *)(smem + a.op[l + 1]);
     const uint8_t *Mc = (const uint8_t *)(smem + a.om[l + 1]);
     const int wc = a.w[l + 1];
-    for (int y = 1 + (tid >> 6); y < h - 1; y += 16)
-    for (int x = 1 + (tid & 63); x < w - 1; x += 64) {
-      int i = y * w + x;
-      float F = (M[i] & MK_SELF) ? 1.0f : 0.0f;
-      int ax = x >> 1, by = y >> 1;
-      bool xe = !(x & 1), ye = !(y & 1);
-      auto E = [&](int xx, int yy) { return Pc[yy * wc + xx]; };
-      auto FC = [&](int xx, int yy) { return (int)(Mc[yy * wc + xx] & MK_SELF); };
-      float e = 0.0f;
-      if (xe && ye) {
-        e = E(ax, by) * F;
-      } else if (!xe && ye) {
-        if (x <= w - 3) e = F * (E(ax, by) + E(ax + 1, by)) / eps_count(FC(ax, by) + FC(ax + 1, by));
-      } else if (xe && !ye) {
-        if (y <= h - 3) e = F * (E(ax, by) + E(ax, by + 1)) / eps_count(FC(ax, by) + FC(ax, by + 1));
-      } else {
-        if (x <= w - 3 && y <= h - 3)
-          e = F * (E(ax, by) + E(ax + 1, by + 1) + E(ax + 1, by) + E(ax, by + 1)) /
-              eps_count(FC(ax, by) + FC(ax + 1, by + 1) + FC(ax + 1, by) + FC(ax, by + 1));
-      }
-      P[i] += e;
+    // prolongate + correct (pressure_solver.cpp:134-181) by 2x2 blocks of
+    // fine cells over coarse cell (ax, by), each by its parity rule
+    auto E = [&](int xx, int yy) { return Pc[yy * wc + xx]; };
+    auto FC = [&](int xx, int yy) { return (int)(Mc[yy * wc + xx] & MK_SELF); };
+    for (int by = (tid >> 6); by <= (h - 2) >> 1; by += 16)
+    for (int ax = (tid & 63); ax <= (w - 2) >> 1; ax += 64) {
+      const float e00 = E(ax, by);
+      const int f00 = FC(ax, by);
+#pragma unroll
+      for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+        for (int dx = 0; dx < 2; dx++) {
+          const int x = 2 * ax + dx, y = 2 * by + dy;
+          if (x < 1 || x > w - 2 || y < 1 || y > h - 2) continue;
+          const int i = y * w + x;
+          const float F = (M[i] & MK_SELF) ? 1.0f : 0.0f;
+          float e = 0.0f;
+          if (dx == 0 && dy == 0) {
+            e = e00 * F;
+          } else if (dy == 0) {
+            if (x <= w - 3) e = F * (e00 + E(ax + 1, by)) / eps_count(f00 + FC(ax + 1, by));
+          } else if (dx == 0) {
+            if (y <= h - 3) e = F * (e00 + E(ax, by + 1)) / eps_count(f00 + FC(ax, by + 1));
+          } else {
+            if (x <= w - 3 && y <= h - 3)
+              e = F * (e00 + E(ax + 1, by + 1) + E(ax + 1, by) + E(ax, by + 1)) /
+                  eps_count(f00 + FC(ax + 1, by + 1) + FC(ax + 1, by) + FC(ax, by + 1));
+          }
+          P[i] += e;
+        }
     }
     __syncthreads();
     for (int s = 0; s < a.nu_post; s++) lds_sweep(P, Fv, M, w, h, a.hh[l], a.alpha);
@@ -158,6 +194,8 @@ inline size_t coarse_lds_layout(const int *ws, const int *hs, int first, int lev
     }
     off += 2 * ((n * 4 + 15) & ~size_t(15)) + ((n + 15) & ~size_t(15));
   }
+  if (a) a->orr = (int)off;
+  off += ((size_t)ws[first] * hs[first] * 4 + 15) & ~size_t(15);
   return off;
 }
 

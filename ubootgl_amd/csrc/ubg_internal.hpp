@@ -93,6 +93,12 @@ void launch_presmooth_restrict(Level &L, Level &C, int ns, float h, float alpha,
 //   prolongation of C.p + correction (+ZG), then ns (0 or 1) sweeps.
 void launch_prolong_postsmooth(Level &L, const Level &C, int ns, float h, float alpha, bool zg,
                                hipStream_t s);
+// LDS-tiled level passes (mg_tile.hip) for levels below the streaming size:
+// nu sweeps + residual + restriction, and prolongation + correction + nu
+// sweeps, one launch each; both swap L.p / L.p_alt.  No zero-gradient BC.
+bool tile_pass_ok(int nu);
+void launch_tile_pre(Level &L, Level &C, int nu, float h, float alpha, hipStream_t s);
+void launch_tile_post(Level &L, const Level &C, int nu, float h, float alpha, hipStream_t s);
 void launch_residual(const Level &L, float h, hipStream_t s);
 void launch_residual_restrict(const Level &F, const Level &C, float h, hipStream_t s);
 void launch_prolong_correct(const Level &F, const Level &C, hipStream_t s);
