@@ -70,6 +70,7 @@ def lib():
         "oracle_sim_add_sink": (None, [C.c_void_p, C.c_float, C.c_float, C.c_float]),
         "oracle_sim_num_sinks": (C.c_int, [C.c_void_p]),
         "oracle_sim_step": (None, [C.c_void_p, C.c_float]),
+        "oracle_sim_set_bcs": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
         "oracle_sim_stage": (C.c_int, [C.c_void_p, C.c_int, C.c_float]),
         "oracle_sim_bilinear_vel": (None, [C.c_void_p, _f32p, _f32p, C.c_int]),
         "oracle_flag_linear": (C.c_float, [_f32p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float]),
@@ -243,6 +244,9 @@ class OracleSim:
 
     def stage(self, st, dt):
         assert lib().oracle_sim_stage(self.h_, st, dt) == 0
+
+    def set_bcs(self, west, east, north, south):
+        lib().oracle_sim_set_bcs(self.h_, west, east, north, south)
 
     def add_sink(self, x, y, z):
         lib().oracle_sim_add_sink(self.h_, x, y, z)

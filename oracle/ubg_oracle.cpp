@@ -1132,6 +1132,14 @@ ORACLE_API void oracle_sim_add_sink(void *sp, float x, float y, float z) {
 }
 ORACLE_API int oracle_sim_num_sinks(void *sp) { return (int)((Sim *)sp)->sinks.size(); }
 ORACLE_API void oracle_sim_step(void *sp, float dt) { ((Sim *)sp)->step(dt); }
+// the public bcWest / bcEast / bcNorth / bcSouth members (simulation.hpp:127-128)
+ORACLE_API void oracle_sim_set_bcs(void *sp, int west, int east, int north, int south) {
+  Sim *s = (Sim *)sp;
+  s->bcW = west;
+  s->bcE = east;
+  s->bcN = north;
+  s->bcS = south;
+}
 
 enum { OS_ACCUM = 0, OS_DIFFUSE, OS_ADVECT, OS_SET_VBCS, OS_PROJECT, OS_SET_PBC,
        OS_SAVE_CURRENT, OS_DIVERGENCE, OS_CORRECT_VELOCITY, OS_MG_SOLVE };

@@ -226,6 +226,55 @@ def test_step_small(gpu, W, H, nsteps):
     compare(sim, ref, FIELDS + [("vx_current", O.VX_CURRENT), ("vy_current", O.VY_CURRENT)])
 
 
+# (west, east, north, south): every kind on every side at least once
+BC_SETS = [(0, 1, 3, 2), (1, 0, 1, 3), (3, 3, 0, 0), (2, 2, 2, 1)]
+
+
+@pytest.mark.parametrize("bcs", BC_SETS)
+def test_step_bc_kinds(gpu, bcs):
+    """Steps under every boundary-condition kind: the fused passes write the
+    boundary cells exactly as setVBCs / setPBC do."""
+    d = workloads.small_domain(72, 40, seed=17, ndisks=3)
+    sim, ref = make_pair(d)
+    sim.setBCs(*bcs)
+    ref.set_bcs(*bcs)
+    for _ in range(3):
+        sim.step(float(d.dt))
+        ref.step(float(d.dt))
+    compare(sim, ref, FIELDS + [("vx_current", O.VX_CURRENT), ("vy_current", O.VY_CURRENT)])
+
+
+def test_step_after_host_writes(gpu):
+    """Accumulated velocity written by the host, then a step; a velocity
+    written by the host, then the diffuse stage alone: vy's boundary is
+    refreshed first, as the reference's setVBCs after the first vx
+    iteration does."""
+    d = workloads.small_domain(64, 40, seed=18, ndisks=2)
+    sim, ref = make_pair(d)
+    sim.setBCs(1, 1, 1, 1)
+    ref.set_bcs(1, 1, 1, 1)
+    g = np.random.default_rng(5)
+    ax = g.normal(0, 0.2, (40, 63)).astype(np.float32)
+    ay = g.normal(0, 0.2, (39, 64)).astype(np.float32)
+    sim.set("vx_accum", ax)
+    sim.set("vy_accum", ay)
+    ref.set(O.VX_ACCUM, ax)
+    ref.set(O.VY_ACCUM, ay)
+    for _ in range(2):
+        sim.step(float(d.dt))
+        ref.step(float(d.dt))
+    compare(sim, ref)
+    vy = g.normal(0, 0.5, (39, 64)).astype(np.float32)
+    sim.set("vy", vy)
+    ref.set(O.VY, vy)
+    sim.stage(L.STAGE_DIFFUSE, float(d.dt))
+    ref.stage(L.STAGE_DIFFUSE, float(d.dt))
+    compare(sim, ref)
+    sim.step(float(d.dt))
+    ref.step(float(d.dt))
+    compare(sim, ref)
+
+
 def test_step_config0(gpu):
     """512x256 config: one step, 10 V-cycles, 3/3/5 smoothing."""
     d = workloads.config0()

@@ -323,6 +323,9 @@ struct ubg_sim {
   float *f = nullptr, *r = nullptr;
   uint8_t *flag = nullptr, *mask = nullptr;
   bool accum_dirty = false;
+  // vy's boundary cells (both buffers) hold what setVBCs computes from vy's
+  // front interior; anything else that writes velocities clears it
+  bool vy_bc_fresh = false;
   ubg_mg *mg = nullptr;
   Pool pool;
   hipStream_t stream = nullptr;
@@ -392,21 +395,34 @@ struct ubg_sim {
 
   VelView vel() const { return VelView{W, H, vx[fx], vx[1 - fx], vy[fy], vy[1 - fy]}; }
 
-  void set_vbcs() { launch_set_vbcs(vel(), bc, stream); }
+  void set_vbcs() {
+    launch_set_vbcs(vel(), bc, stream);
+    vy_bc_fresh = true;
+  }
 
-  // simulation.cpp:104-162
-  void diffuse() {
+  // The reference's setVBCs after the first vx iteration of diffuse leaves
+  // vy's boundary current for vy's interior before vy is diffused; here the
+  // vx and vy iterations are paired, so that refresh (vy only: vx must keep
+  // its old boundary for its first iteration) happens before, when needed.
+  void refresh_vy_bcs() {
+    if (!vy_bc_fresh) launch_set_vbcs_vy(vel(), bc, stream);
+    vy_bc_fresh = true;
+  }
+
+  // simulation.cpp:104-162: iteration k of the vx loop and of the vy loop
+  // in one pass, each with the setVBCs that follows it.  Needs vy_bc_fresh.
+  void diffuse_fused() {
     const float a = dt * mu * (W - 1.0f) / pwidth;
     for (int it = 1; it < 3; it++) {
-      launch_diffuse_vx(vel(), flag, a, stream);
+      launch_diffuse_both(vel(), flag, a, bc, stream);
       fx = 1 - fx;
-      set_vbcs();
-    }
-    for (int it = 1; it < 3; it++) {
-      launch_diffuse_vy(vel(), flag, a, stream);
       fy = 1 - fy;
-      set_vbcs();
     }
+    vy_bc_fresh = true;
+  }
+  void diffuse() {
+    refresh_vy_bcs();
+    diffuse_fused();
   }
 
   // simulation.cpp:241-354
@@ -415,6 +431,7 @@ struct ubg_sim {
     launch_advect(vel(), flag, 0.5f * dt * ih, dt * ih, stream);
     fx = 1 - fx;
     fy = 1 - fy;
+    vy_bc_fresh = false;
   }
 
   // sinks (simulation.cpp:173-187): stamps resolved on the host
@@ -452,7 +469,10 @@ struct ubg_sim {
 
   void divergence() { launch_divergence(vel(), f, 1.0f / h, stream); }
 
-  void correct_velocity() { launch_correct_velocity(vel(), P(), flag, 1.0f / h, stream); }
+  void correct_velocity() {
+    launch_correct_velocity(vel(), P(), flag, 1.0f / h, stream);
+    vy_bc_fresh = false;
+  }
 
   void mg_solve() { mg->solve_level(0, h, true); }
 
@@ -471,17 +491,20 @@ struct ubg_sim {
     (void)hipMemcpyAsync(vy_cur, vy[fy], sizeof(float) * W * (H - 1), hipMemcpyDeviceToDevice, stream);
   }
 
-  // the part of step() after applyAccumulatedVelocity, without sinks
-  void step_body() {
-    diffuse();
+  // The part of step() after applyAccumulatedVelocity (simulation.cpp:
+  // 356-374) in fused passes: diffuse (2 passes), advect, setVBCs +
+  // divergence, [sinks,] the V-cycles, setPBC + correction + setVBCs +
+  // saveCurrentVelocityFields.  Needs vy_bc_fresh.
+  int step_body(bool with_sinks) {
+    diffuse_fused();
     advect();
-    set_vbcs();
-    divergence();
+    launch_vbcs_divergence(vel(), bc, f, 1.0f / h, stream);
+    if (with_sinks) UBG_TRY(apply_sinks());
     for (int c = 0; c < prm.vcycles_per_step; c++) mg_solve();
-    launch_set_pbc(P(), W, H, bc, stream);
-    correct_velocity();
-    set_vbcs();
-    if (prm.save_current) save_current();
+    launch_correct_tail(vel(), P(), flag, 1.0f / h, bc, prm.save_current ? vx_cur : nullptr,
+                        prm.save_current ? vy_cur : nullptr, stream);
+    vy_bc_fresh = true;
+    return UBG_OK;
   }
 
   // simulation.cpp:356-374
@@ -490,7 +513,9 @@ struct ubg_sim {
     if (accum_dirty) {
       launch_accumulate(vel(), vx_acc, vy_acc, stream);
       accum_dirty = false;
+      vy_bc_fresh = false;
     }
+    refresh_vy_bcs(); // outside any captured graph: it depends on the history
     bool any_timer = false;
     for (auto &t : timers) any_timer = any_timer || t.on;
     bool graphable = prm.use_graph && sinks.empty() && !any_timer;
@@ -506,7 +531,7 @@ struct ubg_sim {
         ng.before = st;
         hipGraph_t graph;
         UBG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        step_body();
+        (void)step_body(false);
         hipError_t e = hipStreamEndCapture(stream, &graph);
         ng.after = state();
         set_state(st); // capture executed nothing: restore the buffer roles
@@ -519,13 +544,9 @@ struct ubg_sim {
       }
       UBG_HIP(hipGraphLaunch(g->exec, stream));
       set_state(g->after);
+      vy_bc_fresh = true;
     } else {
-      diffuse();
-      advect();
-      set_vbcs();
-      UBG_TRY(project());
-      set_vbcs();
-      if (prm.save_current) save_current();
+      UBG_TRY(step_body(true));
     }
     return check_launch();
   }
@@ -692,6 +713,7 @@ UBG_API int ubg_sim_set_field(ubg_sim *s, int field, const float *src) {
   if (!s || !src) return fail(UBG_E_INVALID, "null argument");
   if (field < 0 || field >= UBG_FIELD_COUNT) return fail(UBG_E_INVALID, "unknown field");
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->vy_bc_fresh = false;
   int w, h;
   s->shape(field, &w, &h);
   size_t n = (size_t)w * h;
@@ -766,6 +788,7 @@ UBG_API int ubg_sim_stage(ubg_sim *s, int stage, float dt) {
   case UBG_STAGE_ACCUM:
     launch_accumulate(s->vel(), s->vx_acc, s->vy_acc, s->stream);
     s->accum_dirty = false;
+    s->vy_bc_fresh = false;
     break;
   case UBG_STAGE_DIFFUSE: s->diffuse(); break;
   case UBG_STAGE_ADVECT: s->advect(); break;
@@ -792,6 +815,7 @@ UBG_API int ubg_sim_set_grids(ubg_sim *s, int x, int y, float val) {
   std::lock_guard<std::recursive_mutex> g(s->mu_);
   // simulation.hpp:82-98: out-of-range cells are ignored
   if (!(x >= 0 && y >= 0 && x < s->W && y < s->H)) return UBG_OK;
+  s->vy_bc_fresh = false;
   const int W = s->W;
   uint8_t v8 = val != 0.0f ? 1 : 0;
   UBG_HIP(hipMemcpyAsync(s->flag + (size_t)y * W + x, &v8, 1, hipMemcpyHostToDevice, s->stream));
@@ -821,6 +845,7 @@ UBG_API int ubg_sim_set_bcs(ubg_sim *s, int west, int east, int north, int south
   s->bc.east = east;
   s->bc.north = north;
   s->bc.south = south;
+  s->vy_bc_fresh = false;
   s->invalidate_graphs();
   return UBG_OK;
 }
@@ -920,6 +945,7 @@ UBG_API int ubg_sim_shift_map(ubg_sim *s, const float *new_flag) {
   UBG_TRY(s->staging(f8.size(), (void **)&d));
   hipMemcpyAsync(d, f8.data(), f8.size(), hipMemcpyHostToDevice, s->stream);
   launch_shift_map(s->vel(), s->P(), d, s->flag, s->stream);
+  s->vy_bc_fresh = false;
   s->save_current();              // ubootgl_app.cpp:292
   s->mg->classify();              // ubootgl_app.cpp:295, mg.updateFields
   int rc = check_launch();

@@ -84,6 +84,43 @@ __global__ void k_set_vbcs(BCGrid gx, BCGrid gy) {
   bc_thread(gx, t);
   bc_thread(gy, t);
 }
+__global__ void k_set_vbcs_one(BCGrid g) { bc_thread(g, blockIdx.x * blockDim.x + threadIdx.x); }
+
+// The same setVBCs writes, issued by the thread that produced interior cell
+// (x, y) of grid g (g.f is the front buffer; v its new value there): every
+// boundary cell is written by the thread owning the interior cell its value
+// is computed from -- the side neighbour, or (x, y) = (ix, iy) for a corner
+// group.  The boundary's old values are read from g.f before any write; no
+// other thread reads or writes those cells.  cur (optional) receives the
+// same values (saveCurrentVelocityFields after the setVBCs).
+__device__ __forceinline__ void bc_duty(const BCGrid &g, int x, int y, float v, float *cur) {
+  const int w = g.w, h = g.h;
+  auto Fo = [&](int xx, int yy) { return g.f[(size_t)yy * w + xx]; };
+  auto putc = [&](int xx, int yy, float val) {
+    put(g, xx, yy, val);
+    if (cur) cur[(size_t)yy * w + xx] = val;
+  };
+  if (y >= 2 && y <= h - 3) {
+    if (x == 1) putc(0, y, bc_we(g, g.pw, v, Fo(0, y)));
+    if (x == w - 2) putc(w - 1, y, bc_we(g, g.pe, v, Fo(w - 1, y)));
+  }
+  if (x >= 2 && x <= w - 3) {
+    if (y == 1) putc(x, 0, bc_sn(g, g.ps, v, Fo(x, 0)));
+    if (y == h - 2) putc(x, h - 1, bc_sn(g, g.pn, v, Fo(x, h - 1)));
+  }
+  if ((x == 1 || x == w - 2) && (y == 1 || y == h - 2)) {
+    const bool e = x == w - 2, n = y == h - 2;
+    const int cx = e ? w - 1 : 0, cy = n ? h - 1 : 0;
+    const int kx = e ? g.pe : g.pw, ky = n ? g.pn : g.ps;
+    const float side = bc_we(g, kx, v, Fo(cx, y));
+    const float cornA = bc_we(g, kx, Fo(x, cy), Fo(cx, cy));
+    const float edge = bc_sn(g, ky, v, Fo(x, cy));
+    const float corn = bc_sn(g, ky, side, cornA);
+    putc(cx, y, side);
+    putc(x, cy, edge);
+    putc(cx, cy, corn);
+  }
+}
 
 // setPBC (simulation.cpp:36-45) in one launch: W/E columns over all rows,
 // then S/N rows over all columns; only OUTFLOW_ZERO_PRESSURE negates.  Corner
@@ -157,6 +194,53 @@ __global__ void k_diffuse_vx(const float *__restrict__ s, float *__restrict__ d,
   float fvs = FL(fl, W, x, y - 1) * FL(fl, W, x - 1, y - 1);
   val += s[i - Wx] * fvs + (1.0f - fvs) * -vc;
   d[i] = FL(fl, W, x, y) * FL(fl, W, x + 1, y) * (vc + a * val) / (1.0f + 4.0f * a);
+}
+
+// One iteration of both diffuse loops (simulation.cpp:114-129 and 140-155)
+// with the setVBCs that follows each of them: vx and vy do not read each
+// other, and setVBCs on one component only reads that component.  Reads the
+// front buffers sx / sy, writes the back buffers dx / dy (the fronts after
+// the swap) and the boundary cells of both buffers.  The vy boundary must be
+// current for vy's interior when the first iteration starts (the reference
+// refreshes it in the setVBCs after the first vx iteration).
+__global__ void k_diffuse_both(const float *__restrict__ sx, float *__restrict__ dx,
+                               const float *__restrict__ sy, float *__restrict__ dy,
+                               const uint8_t *__restrict__ fl, int W, int H, float a, BCGrid gx,
+                               BCGrid gy) {
+  const int Wx = W - 1;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int y = blockIdx.y + 1;
+  if (x < Wx - 1 && y < H - 1) {
+    const float *s = sx;
+    size_t i = (size_t)y * Wx + x;
+    float vc = s[i];
+    float val = 0.0f;
+    val += s[i + 1] * FL(fl, W, x + 1, y) * FL(fl, W, x + 2, y);
+    val += s[i - 1] * FL(fl, W, x, y) * FL(fl, W, x - 1, y);
+    float fvn = FL(fl, W, x, y + 1) * FL(fl, W, x - 1, y + 1);
+    val += s[i + Wx] * fvn + (1.0f - fvn) * -vc;
+    float fvs = FL(fl, W, x, y - 1) * FL(fl, W, x - 1, y - 1);
+    val += s[i - Wx] * fvs + (1.0f - fvs) * -vc;
+    const float v = FL(fl, W, x, y) * FL(fl, W, x + 1, y) * (vc + a * val) / (1.0f + 4.0f * a);
+    dx[i] = v;
+    bc_duty(gx, x, y, v, nullptr);
+  }
+  const int Hy = H - 1;
+  if (x < W - 1 && y < Hy - 1) {
+    const float *s = sy;
+    size_t i = (size_t)y * W + x;
+    float vc = s[i];
+    float val = 0.0f;
+    val += s[i - W] * FL(fl, W, x, y) * FL(fl, W, x, y - 1);
+    val += s[i + W] * FL(fl, W, x, y + 1) * FL(fl, W, x, y + 2);
+    float fve = FL(fl, W, x + 1, y) * FL(fl, W, x + 1, y + 1);
+    val += s[i + 1] * fve + (1.0f - fve) * -vc;
+    float fvw = FL(fl, W, x - 1, y) * FL(fl, W, x - 1, y + 1);
+    val += s[i - 1] * fvw + (1.0f - fvw) * -vc;
+    const float v = FL(fl, W, x, y) * FL(fl, W, x, y + 1) * (vc + a * val) / (1.0f + 4.0f * a);
+    dy[i] = v;
+    bc_duty(gy, x, y, v, nullptr);
+  }
 }
 
 // diffuse, vy part (simulation.cpp:140-155)
@@ -419,6 +503,94 @@ __global__ void k_correct_velocity(float *__restrict__ xf, float *__restrict__ y
     yf[(size_t)y * W + x] -= Fc * (float)fl[c + W] * ih * (p[c + W] - pc);
 }
 
+// setPBC's writes (simulation.cpp:36-45, corner groups as k_set_pbc) issued
+// by the thread of interior cell (x, y); p's interior is only read.
+__device__ __forceinline__ void pbc_duty(float *__restrict__ p, int w, int h, int x, int y,
+                                         int bw, int be, int bn, int bs) {
+  auto P = [&](int xx, int yy) -> float & { return p[(size_t)yy * w + xx]; };
+  if (y >= 2 && y <= h - 3) {
+    if (x == 1) P(0, y) = pbc(bw, P(1, y));
+    if (x == w - 2) P(w - 1, y) = pbc(be, P(w - 2, y));
+  }
+  if (x >= 2 && x <= w - 3) {
+    if (y == 1) P(x, 0) = pbc(bs, P(x, 1));
+    if (y == h - 2) P(x, h - 1) = pbc(bn, P(x, h - 2));
+  }
+  if ((x == 1 || x == w - 2) && (y == 1 || y == h - 2)) {
+    const bool e = x == w - 2, n = y == h - 2;
+    const int cx = e ? w - 1 : 0, cy = n ? h - 1 : 0;
+    const int kx = e ? be : bw, ky = n ? bn : bs;
+    const float side = pbc(kx, P(x, y));
+    const float edge = pbc(ky, P(x, y));
+    const float corn = pbc(ky, side);
+    P(cx, y) = side;
+    P(x, cy) = edge;
+    P(cx, cy) = corn;
+  }
+}
+
+// The end of project() and step(): setPBC, the velocity correction
+// (simulation.cpp:196-207), setVBCs and saveCurrentVelocityFields in one
+// pass.  The correction reads p at interior cells only, so setPBC's writes
+// do not reach it; every vx / vy face is either corrected here or a boundary
+// face, so the copy into vx_cur / vy_cur is complete (cur may be null).
+__global__ void k_correct_tail(float *__restrict__ xf, float *__restrict__ yf,
+                               float *__restrict__ p, const uint8_t *__restrict__ fl, int W,
+                               int H, float ih, BCGrid gx, BCGrid gy, float *__restrict__ xcur,
+                               float *__restrict__ ycur, int bw, int be, int bn, int bs) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int y = blockIdx.y + 1;
+  if (x >= W - 1) return;
+  const int Wx = W - 1;
+  const size_t c = (size_t)y * W + x;
+  const float Fc = (float)fl[c];
+  const float pc = p[c];
+  if (x < W - 2) {
+    const size_t i = (size_t)y * Wx + x;
+    const float v = xf[i] - Fc * (float)fl[c + 1] * ih * (p[c + 1] - pc);
+    xf[i] = v;
+    if (xcur) xcur[i] = v;
+    bc_duty(gx, x, y, v, xcur);
+  }
+  if (y < H - 2) {
+    const size_t i = (size_t)y * W + x;
+    const float v = yf[i] - Fc * (float)fl[c + W] * ih * (p[c + W] - pc);
+    yf[i] = v;
+    if (ycur) ycur[i] = v;
+    bc_duty(gy, x, y, v, ycur);
+  }
+  pbc_duty(p, W, H, x, y, bw, be, bn, bs);
+}
+
+// setVBCs after advect (simulation.cpp:369) fused with project's divergence
+// (:166-171).  Thread (x, y) owns the vx face (x, y) and the vy face (x, y):
+// it issues their boundary writes and computes f at cell (x, y) from the
+// face values after setVBCs -- a boundary face it reads is the side / edge
+// value of the rule applied to its interior neighbour (never a corner).
+__global__ void k_vbcs_divergence(const float *__restrict__ xf, const float *__restrict__ yf,
+                                  float *__restrict__ f, int W, int H, float ih, BCGrid gx,
+                                  BCGrid gy) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int y = blockIdx.y + 1;
+  if (x >= W - 1) return;
+  const int Wx = W - 1, Hy = H - 1;
+  auto X = [&](int xx, int yy) { return xf[(size_t)yy * Wx + xx]; };
+  auto Y = [&](int xx, int yy) { return yf[(size_t)yy * W + xx]; };
+  // vx faces x - 1 and x of row y, vy faces y - 1 and y of column x, as
+  // they are after setVBCs.  A boundary face read here is written by the
+  // thread of its interior neighbour (possibly this one) in the same pass;
+  // its rule reads its own old value only for INFLOW, which keeps it, so old
+  // and new reads give the same value.
+  const float vx_w = (x >= 2) ? X(x - 1, y) : bc_we(gx, gx.pw, X(1, y), X(0, y));
+  const float vx_e = (x <= Wx - 2) ? X(x, y) : bc_we(gx, gx.pe, X(Wx - 2, y), X(Wx - 1, y));
+  const float vy_s = (y >= 2) ? Y(x, y - 1) : bc_sn(gy, gy.ps, Y(x, 1), Y(x, 0));
+  const float vy_n = (y <= Hy - 2) ? Y(x, y) : bc_sn(gy, gy.pn, Y(x, Hy - 2), Y(x, Hy - 1));
+  if (x <= Wx - 2) bc_duty(gx, x, y, vx_e, nullptr);
+  if (y <= Hy - 2) bc_duty(gy, x, y, vy_n, nullptr);
+  // same association as the reference's expression: ((e - w) + n) - s
+  f[(size_t)y * W + x] = -ih * (vx_e - vx_w + vy_n - vy_s);
+}
+
 // scalar bilinearSample (interpolators.hpp:11-27), clamp to [0, n-1.1]
 __device__ __forceinline__ float bilinear(const float *__restrict__ g, int gw, int gh, float cx,
                                           float cy) {
@@ -524,6 +696,34 @@ void launch_set_vbcs(const VelView &v, const BCs &bc, hipStream_t s) {
   BCGrid gy{v.vyf, v.vyb, v.W, v.H - 1, bc.west, bc.east, bc.south, bc.north, false};
   int n = v.W > v.H ? v.W : v.H;
   hipLaunchKernelGGL(k_set_vbcs, dim3(ceil_div(n, TPB)), dim3(TPB), 0, s, gx, gy);
+}
+void launch_set_vbcs_vy(const VelView &v, const BCs &bc, hipStream_t s) {
+  BCGrid gy{v.vyf, v.vyb, v.W, v.H - 1, bc.west, bc.east, bc.south, bc.north, false};
+  int n = v.W > v.H ? v.W : v.H;
+  hipLaunchKernelGGL(k_set_vbcs_one, dim3(ceil_div(n, TPB)), dim3(TPB), 0, s, gy);
+}
+// one iteration of both diffuse loops with their setVBCs; the back buffers
+// become the fronts (the caller swaps both)
+void launch_diffuse_both(const VelView &v, const uint8_t *flag, float a, const BCs &bc,
+                         hipStream_t s) {
+  BCGrid gx{v.vxb, v.vxf, v.W - 1, v.H, bc.west, bc.east, bc.south, bc.north, true};
+  BCGrid gy{v.vyb, v.vyf, v.W, v.H - 1, bc.west, bc.east, bc.south, bc.north, false};
+  hipLaunchKernelGGL(k_diffuse_both, dim3(ceil_div(v.W - 2, TPB), v.H - 2), dim3(TPB), 0, s, v.vxf,
+                     v.vxb, v.vyf, v.vyb, flag, v.W, v.H, a, gx, gy);
+}
+void launch_vbcs_divergence(const VelView &v, const BCs &bc, float *f, float ih, hipStream_t s) {
+  BCGrid gx{v.vxf, v.vxb, v.W - 1, v.H, bc.west, bc.east, bc.south, bc.north, true};
+  BCGrid gy{v.vyf, v.vyb, v.W, v.H - 1, bc.west, bc.east, bc.south, bc.north, false};
+  hipLaunchKernelGGL(k_vbcs_divergence, dim3(ceil_div(v.W - 2, TPB), v.H - 2), dim3(TPB), 0, s,
+                     v.vxf, v.vyf, f, v.W, v.H, ih, gx, gy);
+}
+void launch_correct_tail(const VelView &v, float *p, const uint8_t *flag, float ih, const BCs &bc,
+                         float *xcur, float *ycur, hipStream_t s) {
+  BCGrid gx{v.vxf, v.vxb, v.W - 1, v.H, bc.west, bc.east, bc.south, bc.north, true};
+  BCGrid gy{v.vyf, v.vyb, v.W, v.H - 1, bc.west, bc.east, bc.south, bc.north, false};
+  hipLaunchKernelGGL(k_correct_tail, dim3(ceil_div(v.W - 2, TPB), v.H - 2), dim3(TPB), 0, s, v.vxf,
+                     v.vyf, p, flag, v.W, v.H, ih, gx, gy, xcur, ycur, bc.west, bc.east, bc.north,
+                     bc.south);
 }
 void launch_set_pbc(float *p, int W, int H, const BCs &bc, hipStream_t s) {
   int n = W > H ? W : H;

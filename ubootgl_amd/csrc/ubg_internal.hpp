@@ -110,6 +110,15 @@ void launch_coarse_vcycle(const Level *lv, int first, int levels, float h, int n
 // ---- fluid step (simulation.cpp) -----------------------------------------
 void launch_set_vbcs(const VelView &v, const BCs &bc, hipStream_t s);
 void launch_set_pbc(float *p, int W, int H, const BCs &bc, hipStream_t s);
+// setVBCs of vy alone (vx untouched)
+void launch_set_vbcs_vy(const VelView &v, const BCs &bc, hipStream_t s);
+// fused passes of the step (sim_kernels.hip); each gives the same values as
+// the reference's sequence of loops it replaces
+void launch_diffuse_both(const VelView &v, const uint8_t *flag, float a, const BCs &bc,
+                         hipStream_t s);
+void launch_vbcs_divergence(const VelView &v, const BCs &bc, float *f, float ih, hipStream_t s);
+void launch_correct_tail(const VelView &v, float *p, const uint8_t *flag, float ih, const BCs &bc,
+                         float *xcur, float *ycur, hipStream_t s);
 void launch_accumulate(const VelView &v, float *vx_acc, float *vy_acc, hipStream_t s);
 void launch_diffuse_vx(const VelView &v, const uint8_t *flag, float a, hipStream_t s);
 void launch_diffuse_vy(const VelView &v, const uint8_t *flag, float a, hipStream_t s);

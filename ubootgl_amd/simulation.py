@@ -249,6 +249,11 @@ class Simulation:
     def saveCurrentVelocityFields(self):
         self.stage(L.STAGE_SAVE_CURRENT)
 
+    def setBCs(self, west, east, north, south):
+        """The bcWest / bcEast / bcNorth / bcSouth members (simulation.hpp:
+        127-128): INFLOW 0, OUTFLOW 1, OUTFLOW_ZERO_PRESSURE 2, NOSLIP 3."""
+        check(lib().ubg_sim_set_bcs(self._h, int(west), int(east), int(north), int(south)))
+
     def setGrids(self, x, y, val):
         check(lib().ubg_sim_set_grids(self._h, int(x), int(y), float(val)))
 
