@@ -207,9 +207,10 @@ __global__ void k_diffuse_both(const float *__restrict__ sx, float *__restrict__
                                const float *__restrict__ sy, float *__restrict__ dy,
                                const uint8_t *__restrict__ fl, int W, int H, float a, BCGrid gx,
                                BCGrid gy) {
-  const int Wx = W - 1;
+  const int Wx = W - 1, Hy = H - 1;
   const int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int y = blockIdx.y + 1;
+  float vxn = 0.0f, vyn = 0.0f;
   if (x < Wx - 1 && y < H - 1) {
     const float *s = sx;
     size_t i = (size_t)y * Wx + x;
@@ -221,11 +222,9 @@ __global__ void k_diffuse_both(const float *__restrict__ sx, float *__restrict__
     val += s[i + Wx] * fvn + (1.0f - fvn) * -vc;
     float fvs = FL(fl, W, x, y - 1) * FL(fl, W, x - 1, y - 1);
     val += s[i - Wx] * fvs + (1.0f - fvs) * -vc;
-    const float v = FL(fl, W, x, y) * FL(fl, W, x + 1, y) * (vc + a * val) / (1.0f + 4.0f * a);
-    dx[i] = v;
-    bc_duty(gx, x, y, v, nullptr);
+    vxn = FL(fl, W, x, y) * FL(fl, W, x + 1, y) * (vc + a * val) / (1.0f + 4.0f * a);
+    dx[i] = vxn;
   }
-  const int Hy = H - 1;
   if (x < W - 1 && y < Hy - 1) {
     const float *s = sy;
     size_t i = (size_t)y * W + x;
@@ -237,9 +236,12 @@ __global__ void k_diffuse_both(const float *__restrict__ sx, float *__restrict__
     val += s[i + 1] * fve + (1.0f - fve) * -vc;
     float fvw = FL(fl, W, x - 1, y) * FL(fl, W, x - 1, y + 1);
     val += s[i - 1] * fvw + (1.0f - fvw) * -vc;
-    const float v = FL(fl, W, x, y) * FL(fl, W, x, y + 1) * (vc + a * val) / (1.0f + 4.0f * a);
-    dy[i] = v;
-    bc_duty(gy, x, y, v, nullptr);
+    vyn = FL(fl, W, x, y) * FL(fl, W, x, y + 1) * (vc + a * val) / (1.0f + 4.0f * a);
+    dy[i] = vyn;
+  }
+  if (x == 1 || x >= W - 3 || y == 1 || y >= H - 3) { // boundary duties
+    if (x < Wx - 1 && y < H - 1) bc_duty(gx, x, y, vxn, nullptr);
+    if (x < W - 1 && y < Hy - 1) bc_duty(gy, x, y, vyn, nullptr);
   }
 }
 
@@ -545,21 +547,24 @@ __global__ void k_correct_tail(float *__restrict__ xf, float *__restrict__ yf,
   const size_t c = (size_t)y * W + x;
   const float Fc = (float)fl[c];
   const float pc = p[c];
+  float vx = 0.0f, vy = 0.0f;
   if (x < W - 2) {
     const size_t i = (size_t)y * Wx + x;
-    const float v = xf[i] - Fc * (float)fl[c + 1] * ih * (p[c + 1] - pc);
-    xf[i] = v;
-    if (xcur) xcur[i] = v;
-    bc_duty(gx, x, y, v, xcur);
+    vx = xf[i] - Fc * (float)fl[c + 1] * ih * (p[c + 1] - pc);
+    xf[i] = vx;
+    if (xcur) xcur[i] = vx;
   }
   if (y < H - 2) {
     const size_t i = (size_t)y * W + x;
-    const float v = yf[i] - Fc * (float)fl[c + W] * ih * (p[c + W] - pc);
-    yf[i] = v;
-    if (ycur) ycur[i] = v;
-    bc_duty(gy, x, y, v, ycur);
+    vy = yf[i] - Fc * (float)fl[c + W] * ih * (p[c + W] - pc);
+    yf[i] = vy;
+    if (ycur) ycur[i] = vy;
   }
-  pbc_duty(p, W, H, x, y, bw, be, bn, bs);
+  if (x == 1 || x >= W - 3 || y == 1 || y >= H - 3) { // boundary duties
+    if (x < W - 2) bc_duty(gx, x, y, vx, xcur);
+    if (y < H - 2) bc_duty(gy, x, y, vy, ycur);
+    pbc_duty(p, W, H, x, y, bw, be, bn, bs);
+  }
 }
 
 // setVBCs after advect (simulation.cpp:369) fused with project's divergence
@@ -574,21 +579,24 @@ __global__ void k_vbcs_divergence(const float *__restrict__ xf, const float *__r
   const int y = blockIdx.y + 1;
   if (x >= W - 1) return;
   const int Wx = W - 1, Hy = H - 1;
-  auto X = [&](int xx, int yy) { return xf[(size_t)yy * Wx + xx]; };
-  auto Y = [&](int xx, int yy) { return yf[(size_t)yy * W + xx]; };
-  // vx faces x - 1 and x of row y, vy faces y - 1 and y of column x, as
-  // they are after setVBCs.  A boundary face read here is written by the
-  // thread of its interior neighbour (possibly this one) in the same pass;
-  // its rule reads its own old value only for INFLOW, which keeps it, so old
-  // and new reads give the same value.
-  const float vx_w = (x >= 2) ? X(x - 1, y) : bc_we(gx, gx.pw, X(1, y), X(0, y));
-  const float vx_e = (x <= Wx - 2) ? X(x, y) : bc_we(gx, gx.pe, X(Wx - 2, y), X(Wx - 1, y));
-  const float vy_s = (y >= 2) ? Y(x, y - 1) : bc_sn(gy, gy.ps, Y(x, 1), Y(x, 0));
-  const float vy_n = (y <= Hy - 2) ? Y(x, y) : bc_sn(gy, gy.pn, Y(x, Hy - 2), Y(x, Hy - 1));
-  if (x <= Wx - 2) bc_duty(gx, x, y, vx_e, nullptr);
-  if (y <= Hy - 2) bc_duty(gy, x, y, vy_n, nullptr);
+  // vx faces x - 1 and x of row y, vy faces y - 1 and y of column x (all
+  // exist for x <= W - 2, y <= H - 2), then their values after setVBCs: a
+  // boundary face among them gets the rule applied to the other face, its
+  // interior neighbour.  That face is written by the thread of its interior
+  // neighbour (possibly this one) in the same pass; the rule reads the old
+  // value only for INFLOW, which keeps it, so old and new reads agree.
+  const float xw = xf[(size_t)y * Wx + x - 1], xe = xf[(size_t)y * Wx + x];
+  const float ys = yf[(size_t)(y - 1) * W + x], yn = yf[(size_t)y * W + x];
+  const float vx_w = (x == 1) ? bc_we(gx, gx.pw, xe, xw) : xw;
+  const float vx_e = (x == Wx - 1) ? bc_we(gx, gx.pe, xw, xe) : xe;
+  const float vy_s = (y == 1) ? bc_sn(gy, gy.ps, yn, ys) : ys;
+  const float vy_n = (y == Hy - 1) ? bc_sn(gy, gy.pn, ys, yn) : yn;
   // same association as the reference's expression: ((e - w) + n) - s
   f[(size_t)y * W + x] = -ih * (vx_e - vx_w + vy_n - vy_s);
+  if (x == 1 || x >= W - 3 || y == 1 || y >= H - 3) { // boundary duties
+    if (x <= Wx - 2) bc_duty(gx, x, y, xe, nullptr);
+    if (y <= Hy - 2) bc_duty(gy, x, y, yn, nullptr);
+  }
 }
 
 // scalar bilinearSample (interpolators.hpp:11-27), clamp to [0, n-1.1]
