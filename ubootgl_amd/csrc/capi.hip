@@ -322,6 +322,10 @@ struct ubg_sim {
                                         // current one is mg->lv[0].p
   float *f = nullptr, *r = nullptr;
   uint8_t *flag = nullptr, *mask = nullptr;
+  // per-cell face factors of the vx / vy stencils (flag products, see
+  // launch_face_masks), rebuilt when the MG's flag generation moves
+  uint16_t *fmask = nullptr;
+  unsigned fmask_gen = ~0u;
   bool accum_dirty = false;
   // vy's boundary cells (both buffers) hold what setVBCs computes from vy's
   // front interior; anything else that writes velocities clears it
@@ -400,6 +404,13 @@ struct ubg_sim {
     vy_bc_fresh = true;
   }
 
+  // face factors current for the flag field (outside any captured graph)
+  void ensure_fmask() {
+    if (fmask_gen == mg->flag_gen) return;
+    launch_face_masks(flag, fmask, W, H, stream);
+    fmask_gen = mg->flag_gen;
+  }
+
   // The reference's setVBCs after the first vx iteration of diffuse leaves
   // vy's boundary current for vy's interior before vy is diffused; here the
   // vx and vy iterations are paired, so that refresh (vy only: vx must keep
@@ -414,13 +425,14 @@ struct ubg_sim {
   void diffuse_fused() {
     const float a = dt * mu * (W - 1.0f) / pwidth;
     for (int it = 1; it < 3; it++) {
-      launch_diffuse_both(vel(), flag, a, bc, stream);
+      launch_diffuse_both(vel(), fmask, a, bc, stream);
       fx = 1 - fx;
       fy = 1 - fy;
     }
     vy_bc_fresh = true;
   }
   void diffuse() {
+    ensure_fmask();
     refresh_vy_bcs();
     diffuse_fused();
   }
@@ -501,7 +513,7 @@ struct ubg_sim {
     launch_vbcs_divergence(vel(), bc, f, 1.0f / h, stream);
     if (with_sinks) UBG_TRY(apply_sinks());
     for (int c = 0; c < prm.vcycles_per_step; c++) mg_solve();
-    launch_correct_tail(vel(), P(), flag, 1.0f / h, bc, prm.save_current ? vx_cur : nullptr,
+    launch_correct_tail(vel(), P(), fmask, 1.0f / h, bc, prm.save_current ? vx_cur : nullptr,
                         prm.save_current ? vy_cur : nullptr, stream);
     vy_bc_fresh = true;
     return UBG_OK;
@@ -516,6 +528,7 @@ struct ubg_sim {
       vy_bc_fresh = false;
     }
     refresh_vy_bcs(); // outside any captured graph: it depends on the history
+    ensure_fmask();
     bool any_timer = false;
     for (auto &t : timers) any_timer = any_timer || t.on;
     bool graphable = prm.use_graph && sinks.empty() && !any_timer;
@@ -606,7 +619,7 @@ UBG_API int ubg_sim_create(const float *flag, int width, int height, float pwidt
   const size_t nc = (size_t)width * height, nx = (size_t)(width - 1) * height,
                ny = (size_t)width * (height - 1);
   size_t bytes = 256 + 4 * align_up(nx * 4) + 4 * align_up(ny * 4) + 4 * align_up(nc * 4) +
-                 2 * align_up(nc);
+                 2 * align_up(nc) + align_up(nc * 2);
   int rc = s->pool.init(bytes);
   if (rc) {
     delete s;
@@ -626,6 +639,7 @@ UBG_API int ubg_sim_create(const float *flag, int width, int height, float pwidt
   s->r = s->pool.take<float>(nc);
   s->flag = s->pool.take<uint8_t>(nc);
   s->mask = s->pool.take<uint8_t>(nc);
+  s->fmask = s->pool.take<uint16_t>(nc);
   hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     s->pool.release();
@@ -829,6 +843,7 @@ UBG_API int ubg_sim_set_grids(ubg_sim *s, int x, int y, float val) {
   }
   // the level-0 neighbour mask around the cell
   launch_build_mask(s->flag, s->mask, s->W, s->H, s->stream);
+  s->mg->flag_gen++;
   UBG_TRY(check_launch());
   UBG_HIP(hipStreamSynchronize(s->stream));
   return UBG_OK;

@@ -152,6 +152,7 @@ struct ubg_mg {
   ubg_params prm{};
   double *d_acc = nullptr;
   IRState *ir = nullptr;   // allocated by the first refinement solve
+  unsigned flag_gen = 0;   // bumped whenever the level-0 flag may have changed
   Timer *timers = nullptr; // borrowed from the owning simulation (UBG_TIMER_COUNT)
   std::recursive_mutex mu;
 
@@ -229,6 +230,7 @@ struct ubg_mg {
 
   // coarse flags of levels 1..L-1 from the level-0 flag (updateFields)
   void classify() {
+    flag_gen++;
     launch_build_mask(lv[0].flag, lv[0].mask, lv[0].w, lv[0].h, stream);
     for (int l = 1; l < levels; l++) {
       launch_coarsen(lv[l - 1].flag, lv[l - 1].w, lv[l].flag, lv[l].w, lv[l].h, stream);

@@ -203,45 +203,84 @@ __global__ void k_diffuse_vx(const float *__restrict__ s, float *__restrict__ d,
 // the swap) and the boundary cells of both buffers.  The vy boundary must be
 // current for vy's interior when the first iteration starts (the reference
 // refreshes it in the setVBCs after the first vx iteration).
+__device__ __forceinline__ float fbit(unsigned m, int k) { return ((m >> k) & 1u) ? 1.0f : 0.0f; }
+
+__global__ void k_face_masks(const uint8_t *__restrict__ fl, uint16_t *__restrict__ fm, int W,
+                             int H) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= W) return;
+  auto F = [&](int xx, int yy) -> unsigned {
+    return (xx >= 0 && xx < W && yy >= 0 && yy < H) ? fl[(size_t)yy * W + xx] : 0u;
+  };
+  unsigned m = 0;
+  if (x <= W - 2) { // vx face (x, y)
+    m |= (F(x, y) & F(x + 1, y)) << 0;
+    m |= (F(x + 1, y) & F(x + 2, y)) << 1;
+    m |= (F(x, y) & F(x - 1, y)) << 2;
+    m |= (F(x, y + 1) & F(x - 1, y + 1)) << 3;
+    m |= (F(x, y - 1) & F(x - 1, y - 1)) << 4;
+  }
+  if (y <= H - 2) { // vy face (x, y)
+    m |= (F(x, y) & F(x, y + 1)) << 8;
+    m |= (F(x, y) & F(x, y - 1)) << 9;
+    m |= (F(x, y + 1) & F(x, y + 2)) << 10;
+    m |= (F(x + 1, y) & F(x + 1, y + 1)) << 11;
+    m |= (F(x - 1, y) & F(x - 1, y + 1)) << 12;
+  }
+  fm[(size_t)y * W + x] = (uint16_t)m;
+}
+
+// One iteration of both diffuse loops (simulation.cpp:114-129 and 140-155)
+// with the setVBCs that follows each of them: vx and vy do not read each
+// other, and setVBCs on one component only reads that component.  Reads the
+// front buffers sx / sy, writes the back buffers dx / dy (the fronts after
+// the swap) and the boundary cells of both buffers.  The vy boundary must be
+// current for vy's interior when the first iteration starts (the reference
+// refreshes it in the setVBCs after the first vx iteration).  The flag
+// products come from the face factors: s * F1 * F2 evaluated left to right
+// equals s * (F1 AND F2) for 0/1 flags, signed zeros included.
 __global__ void k_diffuse_both(const float *__restrict__ sx, float *__restrict__ dx,
                                const float *__restrict__ sy, float *__restrict__ dy,
-                               const uint8_t *__restrict__ fl, int W, int H, float a, BCGrid gx,
-                               BCGrid gy) {
+                               const uint16_t *__restrict__ fm, int W, int H, float a,
+                               BCGrid gx, BCGrid gy) {
   const int Wx = W - 1, Hy = H - 1;
   const int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int y = blockIdx.y + 1;
+  if (x >= W - 1) return;
+  const unsigned m = fm[(size_t)y * W + x];
   float vxn = 0.0f, vyn = 0.0f;
-  if (x < Wx - 1 && y < H - 1) {
+  if (x < Wx - 1) {
     const float *s = sx;
     size_t i = (size_t)y * Wx + x;
     float vc = s[i];
     float val = 0.0f;
-    val += s[i + 1] * FL(fl, W, x + 1, y) * FL(fl, W, x + 2, y);
-    val += s[i - 1] * FL(fl, W, x, y) * FL(fl, W, x - 1, y);
-    float fvn = FL(fl, W, x, y + 1) * FL(fl, W, x - 1, y + 1);
+    val += s[i + 1] * fbit(m, 1);
+    val += s[i - 1] * fbit(m, 2);
+    float fvn = fbit(m, 3);
     val += s[i + Wx] * fvn + (1.0f - fvn) * -vc;
-    float fvs = FL(fl, W, x, y - 1) * FL(fl, W, x - 1, y - 1);
+    float fvs = fbit(m, 4);
     val += s[i - Wx] * fvs + (1.0f - fvs) * -vc;
-    vxn = FL(fl, W, x, y) * FL(fl, W, x + 1, y) * (vc + a * val) / (1.0f + 4.0f * a);
+    vxn = fbit(m, 0) * (vc + a * val) / (1.0f + 4.0f * a);
     dx[i] = vxn;
   }
-  if (x < W - 1 && y < Hy - 1) {
+  if (y < Hy - 1) {
     const float *s = sy;
     size_t i = (size_t)y * W + x;
     float vc = s[i];
     float val = 0.0f;
-    val += s[i - W] * FL(fl, W, x, y) * FL(fl, W, x, y - 1);
-    val += s[i + W] * FL(fl, W, x, y + 1) * FL(fl, W, x, y + 2);
-    float fve = FL(fl, W, x + 1, y) * FL(fl, W, x + 1, y + 1);
+    val += s[i - W] * fbit(m, 9);
+    val += s[i + W] * fbit(m, 10);
+    float fve = fbit(m, 11);
     val += s[i + 1] * fve + (1.0f - fve) * -vc;
-    float fvw = FL(fl, W, x - 1, y) * FL(fl, W, x - 1, y + 1);
+    float fvw = fbit(m, 12);
     val += s[i - 1] * fvw + (1.0f - fvw) * -vc;
-    vyn = FL(fl, W, x, y) * FL(fl, W, x, y + 1) * (vc + a * val) / (1.0f + 4.0f * a);
+    vyn = fbit(m, 8) * (vc + a * val) / (1.0f + 4.0f * a);
     dy[i] = vyn;
   }
   if (x == 1 || x >= W - 3 || y == 1 || y >= H - 3) { // boundary duties
-    if (x < Wx - 1 && y < H - 1) bc_duty(gx, x, y, vxn, nullptr);
-    if (x < W - 1 && y < Hy - 1) bc_duty(gy, x, y, vyn, nullptr);
+    if (x < Wx - 1) bc_duty(gx, x, y, vxn, nullptr);
+    if (y < Hy - 1) bc_duty(gy, x, y, vyn, nullptr);
   }
 }
 
@@ -537,7 +576,7 @@ __device__ __forceinline__ void pbc_duty(float *__restrict__ p, int w, int h, in
 // do not reach it; every vx / vy face is either corrected here or a boundary
 // face, so the copy into vx_cur / vy_cur is complete (cur may be null).
 __global__ void k_correct_tail(float *__restrict__ xf, float *__restrict__ yf,
-                               float *__restrict__ p, const uint8_t *__restrict__ fl, int W,
+                               float *__restrict__ p, const uint16_t *__restrict__ fm, int W,
                                int H, float ih, BCGrid gx, BCGrid gy, float *__restrict__ xcur,
                                float *__restrict__ ycur, int bw, int be, int bn, int bs) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -545,18 +584,18 @@ __global__ void k_correct_tail(float *__restrict__ xf, float *__restrict__ yf,
   if (x >= W - 1) return;
   const int Wx = W - 1;
   const size_t c = (size_t)y * W + x;
-  const float Fc = (float)fl[c];
+  const unsigned m = fm[c]; // F(x)F(x+1) (bit 0) and F(y)F(y+1) (bit 8)
   const float pc = p[c];
   float vx = 0.0f, vy = 0.0f;
   if (x < W - 2) {
     const size_t i = (size_t)y * Wx + x;
-    vx = xf[i] - Fc * (float)fl[c + 1] * ih * (p[c + 1] - pc);
+    vx = xf[i] - fbit(m, 0) * ih * (p[c + 1] - pc);
     xf[i] = vx;
     if (xcur) xcur[i] = vx;
   }
   if (y < H - 2) {
     const size_t i = (size_t)y * W + x;
-    vy = yf[i] - Fc * (float)fl[c + W] * ih * (p[c + W] - pc);
+    vy = yf[i] - fbit(m, 8) * ih * (p[c + W] - pc);
     yf[i] = vy;
     if (ycur) ycur[i] = vy;
   }
@@ -712,12 +751,15 @@ void launch_set_vbcs_vy(const VelView &v, const BCs &bc, hipStream_t s) {
 }
 // one iteration of both diffuse loops with their setVBCs; the back buffers
 // become the fronts (the caller swaps both)
-void launch_diffuse_both(const VelView &v, const uint8_t *flag, float a, const BCs &bc,
+void launch_face_masks(const uint8_t *flag, uint16_t *fmask, int W, int H, hipStream_t s) {
+  hipLaunchKernelGGL(k_face_masks, dim3(ceil_div(W, TPB), H), dim3(TPB), 0, s, flag, fmask, W, H);
+}
+void launch_diffuse_both(const VelView &v, const uint16_t *fmask, float a, const BCs &bc,
                          hipStream_t s) {
   BCGrid gx{v.vxb, v.vxf, v.W - 1, v.H, bc.west, bc.east, bc.south, bc.north, true};
   BCGrid gy{v.vyb, v.vyf, v.W, v.H - 1, bc.west, bc.east, bc.south, bc.north, false};
   hipLaunchKernelGGL(k_diffuse_both, dim3(ceil_div(v.W - 2, TPB), v.H - 2), dim3(TPB), 0, s, v.vxf,
-                     v.vxb, v.vyf, v.vyb, flag, v.W, v.H, a, gx, gy);
+                     v.vxb, v.vyf, v.vyb, fmask, v.W, v.H, a, gx, gy);
 }
 void launch_vbcs_divergence(const VelView &v, const BCs &bc, float *f, float ih, hipStream_t s) {
   BCGrid gx{v.vxf, v.vxb, v.W - 1, v.H, bc.west, bc.east, bc.south, bc.north, true};
@@ -725,12 +767,12 @@ void launch_vbcs_divergence(const VelView &v, const BCs &bc, float *f, float ih,
   hipLaunchKernelGGL(k_vbcs_divergence, dim3(ceil_div(v.W - 2, TPB), v.H - 2), dim3(TPB), 0, s,
                      v.vxf, v.vyf, f, v.W, v.H, ih, gx, gy);
 }
-void launch_correct_tail(const VelView &v, float *p, const uint8_t *flag, float ih, const BCs &bc,
-                         float *xcur, float *ycur, hipStream_t s) {
+void launch_correct_tail(const VelView &v, float *p, const uint16_t *fmask, float ih,
+                         const BCs &bc, float *xcur, float *ycur, hipStream_t s) {
   BCGrid gx{v.vxf, v.vxb, v.W - 1, v.H, bc.west, bc.east, bc.south, bc.north, true};
   BCGrid gy{v.vyf, v.vyb, v.W, v.H - 1, bc.west, bc.east, bc.south, bc.north, false};
   hipLaunchKernelGGL(k_correct_tail, dim3(ceil_div(v.W - 2, TPB), v.H - 2), dim3(TPB), 0, s, v.vxf,
-                     v.vyf, p, flag, v.W, v.H, ih, gx, gy, xcur, ycur, bc.west, bc.east, bc.north,
+                     v.vyf, p, fmask, v.W, v.H, ih, gx, gy, xcur, ycur, bc.west, bc.east, bc.north,
                      bc.south);
 }
 void launch_set_pbc(float *p, int W, int H, const BCs &bc, hipStream_t s) {

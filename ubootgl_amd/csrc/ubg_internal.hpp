@@ -114,11 +114,17 @@ void launch_set_pbc(float *p, int W, int H, const BCs &bc, hipStream_t s);
 void launch_set_vbcs_vy(const VelView &v, const BCs &bc, hipStream_t s);
 // fused passes of the step (sim_kernels.hip); each gives the same values as
 // the reference's sequence of loops it replaces
-void launch_diffuse_both(const VelView &v, const uint8_t *flag, float a, const BCs &bc,
+// Face factors from the 0/1 flag F (one uint16 per cell (x, y)): vx face
+// bits 0-4 = F(x)F(x+1), F(x+1)F(x+2), F(x)F(x-1), F(x,y+1)F(x-1,y+1),
+// F(x,y-1)F(x-1,y-1); vy face bits 8-12 = F(y)F(y+1), F(y)F(y-1),
+// F(y+1)F(y+2), F(x+1,y)F(x+1,y+1), F(x-1,y)F(x-1,y+1) (zero where a face
+// or a neighbour is missing).  A 0/1 float product of flags equals the AND.
+void launch_face_masks(const uint8_t *flag, uint16_t *fmask, int W, int H, hipStream_t s);
+void launch_diffuse_both(const VelView &v, const uint16_t *fmask, float a, const BCs &bc,
                          hipStream_t s);
 void launch_vbcs_divergence(const VelView &v, const BCs &bc, float *f, float ih, hipStream_t s);
-void launch_correct_tail(const VelView &v, float *p, const uint8_t *flag, float ih, const BCs &bc,
-                         float *xcur, float *ycur, hipStream_t s);
+void launch_correct_tail(const VelView &v, float *p, const uint16_t *fmask, float ih,
+                         const BCs &bc, float *xcur, float *ycur, hipStream_t s);
 void launch_accumulate(const VelView &v, float *vx_acc, float *vy_acc, hipStream_t s);
 void launch_diffuse_vx(const VelView &v, const uint8_t *flag, float a, hipStream_t s);
 void launch_diffuse_vy(const VelView &v, const uint8_t *flag, float a, hipStream_t s);
