@@ -194,8 +194,12 @@ UBG_API int ubg_mg_ir_device_ptrs(ubg_mg *mg, double **x, float **f);
  * Domains need W, H >= 16. */
 typedef struct ubg_batch ubg_batch;
 typedef struct ubg_batch_tuning {
-  int tile_w, tile_h; /* smoother tile (cells), one of the compiled shapes */
+  int tile_w, tile_h; /* smoother tile (cells), one of the compiled shapes;
+                         used when a smoothing count exceeds 3            */
   int blocks;         /* workgroups per launch (0: one per tile)            */
+  int pass_rows;      /* rows of the fused level-pass tiles: 8, 16 or 32
+                         (0: 16); their width follows from the smoothing
+                         count                                              */
   double sweep_ms;    /* measured time of the tuning pass (output)          */
 } ubg_batch_tuning;
 /* device bytes a batch of these domains allocates */
@@ -213,9 +217,11 @@ UBG_API int ubg_batch_set_guess(ubg_batch *b, int d, const double *x);
 UBG_API int ubg_batch_solve(ubg_batch *b, const ubg_ir_params *prm, int *graph_replays);
 UBG_API int ubg_batch_result(ubg_batch *b, int d, ubg_ir_result *res);
 UBG_API int ubg_batch_get_solution(ubg_batch *b, int d, double *x);
-/* time the compiled smoother tile shapes and workgroup counts on this batch
- * and keep the fastest; with db_path, reuse / append the choice keyed by the
- * batch's size signature (one "key tile_w tile_h blocks ms" line each) */
+/* time the candidate shapes and workgroup counts on this batch (the fused
+ * level passes' tile rows, or the smoother tiles when a smoothing count
+ * exceeds 3) and keep the fastest; with db_path, reuse / append the choice
+ * keyed by the batch's size signature (one "key tile_w tile_h blocks
+ * pass_rows ms" line each) */
 UBG_API int ubg_batch_autotune(ubg_batch *b, const char *db_path, ubg_batch_tuning *out);
 UBG_API int ubg_batch_set_tuning(ubg_batch *b, const ubg_batch_tuning *t);
 UBG_API int ubg_batch_get_tuning(ubg_batch *b, ubg_batch_tuning *t);

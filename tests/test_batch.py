@@ -92,9 +92,9 @@ def test_batch_alone_equals_in_mix_and_tuning_invariant(gpu):
     x_alone = alone.solution(0)
     r_alone = alone.result(0)
     alone.close()
-    for shape, blocks in (((32, 8), 0), ((128, 4), 64), ((64, 16), 7)):
+    for shape, blocks, rows in (((32, 8), 0, 8), ((128, 4), 64, 32), ((64, 16), 7, 16)):
         bt = B.Batch([(P.W, P.H) for P in probs])
-        bt.set_tuning(*shape, blocks)
+        bt.set_tuning(*shape, blocks, rows)
         for d, P in enumerate(probs):
             bt.set_problem(d, P)
         bt.solve(tol=1e-7)
@@ -135,10 +135,12 @@ def test_batch_autotune_and_db(gpu, tmp_path):
     db = str(tmp_path / "tune.db")
     t1 = bt.autotune(db)
     assert (t1["tile_w"], t1["tile_h"]) in B.TILE_SHAPES and t1["sweep_ms"] > 0
+    assert t1["pass_rows"] in (8, 16, 32)
     assert os.path.getsize(db) > 0
     t2 = bt.autotune(db)  # reused from the file
-    assert (t2["tile_w"], t2["tile_h"], t2["blocks"]) == (t1["tile_w"], t1["tile_h"], t1["blocks"])
-    assert bt.tuning() == {k: t1[k] for k in ("tile_w", "tile_h", "blocks")}
+    keys = ("tile_w", "tile_h", "blocks", "pass_rows")
+    assert tuple(t2[k] for k in keys) == tuple(t1[k] for k in keys)
+    assert bt.tuning() == {k: t1[k] for k in keys}
     bt.solve(tol=1e-6)
     rs, xs = single(probs[0], 1e-6)
     assert np.array_equal(bt.solution(0), xs)
@@ -158,6 +160,9 @@ def test_batch_error_paths(gpu):
     assert e.value.code == L.E_FLAG_VALUE
     with pytest.raises(L.UbgError) as e:
         bt.set_tuning(48, 8)
+    assert e.value.code == L.E_INVALID
+    with pytest.raises(L.UbgError) as e:
+        bt.set_tuning(64, 8, 0, 12)
     assert e.value.code == L.E_INVALID
     bt.set_problem(0, P)
     P2 = W.poisson_problem(32, 32, seed=4, solid=0.0)

@@ -70,7 +70,7 @@ class IRResult(C.Structure):
 
 class BatchTuning(C.Structure):
     _fields_ = [("tile_w", C.c_int), ("tile_h", C.c_int), ("blocks", C.c_int),
-                ("sweep_ms", C.c_double)]
+                ("pass_rows", C.c_int), ("sweep_ms", C.c_double)]
 
 
 class ServerParams(C.Structure):

@@ -87,16 +87,19 @@ class Batch:
         t = L.BatchTuning()
         check(lib().ubg_batch_autotune(self._h, db_path.encode() if db_path else None,
                                        C.byref(t)))
-        return dict(tile_w=t.tile_w, tile_h=t.tile_h, blocks=t.blocks, sweep_ms=t.sweep_ms)
+        return dict(tile_w=t.tile_w, tile_h=t.tile_h, blocks=t.blocks, pass_rows=t.pass_rows,
+                    sweep_ms=t.sweep_ms)
 
-    def set_tuning(self, tile_w, tile_h, blocks=0):
-        t = L.BatchTuning(tile_w, tile_h, blocks, 0.0)
+    def set_tuning(self, tile_w, tile_h, blocks=0, pass_rows=16):
+        """tile_w x tile_h: smoother tile (used when a smoothing count exceeds
+        3); pass_rows: rows of the fused level-pass tiles (8, 16 or 32)."""
+        t = L.BatchTuning(tile_w, tile_h, blocks, pass_rows, 0.0)
         check(lib().ubg_batch_set_tuning(self._h, C.byref(t)))
 
     def tuning(self):
         t = L.BatchTuning()
         check(lib().ubg_batch_get_tuning(self._h, C.byref(t)))
-        return dict(tile_w=t.tile_w, tile_h=t.tile_h, blocks=t.blocks)
+        return dict(tile_w=t.tile_w, tile_h=t.tile_h, blocks=t.blocks, pass_rows=t.pass_rows)
 
     def close(self):
         if getattr(self, "_h", None) is not None:

@@ -14,6 +14,7 @@
 // parities run in the same launch.  Domains whose solve has ended are
 // skipped on the device; no host round trip is needed per stage.
 #include "batch_engine.hpp"
+#include "tile_device.hpp"
 
 #include <array>
 #include <cstdio>
@@ -39,6 +40,7 @@ struct StageArgs {
   const uint8_t *ia; // per-domain index row a
   const uint8_t *ib; // per-domain index row b
   float alpha;
+  int otx, oty; // tile of the fused level passes
 };
 
 __device__ __forceinline__ bool dom_active(const BDom &D) {
@@ -258,6 +260,46 @@ __global__ void __launch_bounds__(TPB) kb_pc(StageArgs a) {
   } while (tw.next(a));
 }
 
+// The fused level passes of tile_device.hpp over the tiles of every member
+// domain at one level (otx x oty tiles, walked as the other stages): pre
+// reads p[ia], writes p[ia^1], the coarse rhs and a zero coarse p[ib]; post
+// reads p[ia] and the coarse solution p[ib], writes p[ia^1].
+constexpr int BT_TPB = 512;
+template <int NU>
+__global__ void __launch_bounds__(BT_TPB) kb_tile_pre(StageArgs a) {
+  extern __shared__ __align__(16) char smem[];
+  TileWalk tw;
+  if (!tw.init(a)) return;
+  do {
+    const BDom &D = a.dom[tw.d];
+    if (!dom_active(D)) continue;
+    const BLev &F = a.lev[tw.d * BMAXL + a.level];
+    const BLev &C = a.lev[tw.d * BMAXL + a.level + 1];
+    const int s = a.ia[tw.d];
+    __syncthreads(); // the previous tile's LDS readers are done
+    tile_pre_body<NU, BT_TPB>(smem, tw.tx() * a.otx, tw.ty() * a.oty, a.otx, a.oty, F.p[s],
+                              F.p[s ^ 1], F.f, F.mask, F.w, F.h, F.hh, a.alpha, F.ihsq, C.f,
+                              C.p[a.ib[tw.d]], C.w, C.h);
+  } while (tw.next(a));
+}
+template <int NU>
+__global__ void __launch_bounds__(BT_TPB) kb_tile_post(StageArgs a) {
+  extern __shared__ __align__(16) char smem[];
+  TileWalk tw;
+  if (!tw.init(a)) return;
+  do {
+    const BDom &D = a.dom[tw.d];
+    if (!dom_active(D)) continue;
+    const BLev &F = a.lev[tw.d * BMAXL + a.level];
+    const BLev &C = a.lev[tw.d * BMAXL + a.level + 1];
+    const int s = a.ia[tw.d];
+    __syncthreads();
+    tile_post_body<NU, BT_TPB>(smem, tw.tx() * a.otx, tw.ty() * a.oty, a.otx, a.oty, F.p[s],
+                               F.p[s ^ 1], F.f, F.mask, F.w, F.h, F.hh, a.alpha,
+                               C.p[a.ib[tw.d]], C.mask, C.w, C.h);
+  } while (tw.next(a));
+}
+
 // the coarse levels of each domain in LDS, one workgroup per domain slot;
 // ia: index of p at the domain's first LDS level
 __global__ void __launch_bounds__(TPB_C) kb_coarse(StageArgs a, const CoarseArgs *cargs,
@@ -445,6 +487,24 @@ bool shape_ok(int tx, int ty) {
 }
 
 size_t al(size_t n) { return align_up(n, 256); }
+
+// rows of the fused-pass tiles the tuning may choose
+bool pass_rows_ok(int r) { return r == 8 || r == 16 || r == 32; }
+
+void launch_tile_pass(bool pre, int nu, int blocks, hipStream_t s, const StageArgs &a) {
+  const size_t shm = pre ? tile_pre_lds(a.oty, nu) : tile_post_lds(a.oty, nu);
+#define UBG_TP(N)                                                                               \
+  if (nu == N) {                                                                                \
+    if (pre) hipLaunchKernelGGL((kb_tile_pre<N>), dim3(blocks), dim3(BT_TPB), shm, s, a);       \
+    else hipLaunchKernelGGL((kb_tile_post<N>), dim3(blocks), dim3(BT_TPB), shm, s, a);          \
+    return;                                                                                     \
+  }
+  UBG_TP(0)
+  UBG_TP(1)
+  UBG_TP(2)
+  UBG_TP(3)
+#undef UBG_TP
+}
 
 } // namespace
 
@@ -877,6 +937,22 @@ int BatchEngine::build_tiles() {
       ny = cdiv(G.h[l] - 2, PCH);
     });
   }
+  const int ty_p = tune_.prow;
+  const int tx_pre = tile_otx_max(prm_.nu_pre), tx_post = tile_otx_max(prm_.nu_post);
+  for (int l = 0; l < BMAXL; l++) {
+    span(tp_[l], nr, [&](int s, int &nx, int &ny) {
+      const BGeom &G = *run_[s]->g;
+      if (l >= G.cfirst) return;
+      nx = cdiv(G.w[l], tx_pre);
+      ny = cdiv(G.h[l], ty_p);
+    });
+    span(tq_[l], nr, [&](int s, int &nx, int &ny) {
+      const BGeom &G = *run_[s]->g;
+      if (l >= G.cfirst) return;
+      nx = cdiv(G.w[l], tx_post);
+      ny = cdiv(G.h[l], ty_p);
+    });
+  }
   span(upd_, (int)all_.size(), [&](int s, int &nx, int &ny) {
     nx = all_[s]->gx;
     ny = all_[s]->gy;
@@ -893,7 +969,7 @@ int BatchEngine::build_tiles() {
   return UBG_OK;
 }
 
-enum { ST_SWEEP, ST_RR, ST_PC, ST_COARSE, ST_UPDATE, ST_NORM };
+enum { ST_SWEEP, ST_RR, ST_PC, ST_COARSE, ST_UPDATE, ST_NORM, ST_TPRE, ST_TPOST };
 
 // The stage sequence of nsteps refinement steps and, per stage, the two
 // per-domain index rows, simulated from (and advancing) the domains' host
@@ -911,7 +987,35 @@ void BatchEngine::gen_steps(int nsteps, std::vector<StageRec> &st, std::vector<u
   };
   for (int step = 0; step < nsteps; step++) {
     const bool with_fresh = step == 0 && (all_fresh || !fresh_.empty());
-    for (int v = 0; v < k_ && nr; v++) {
+    for (int v = 0; v < k_ && nr && tiled(); v++) {
+      // fused level passes: one pre and one post per level, each flipping
+      // the level's p once
+      for (int l = 0; l < lmax_; l++) {
+        if (!tp_[l].n) continue;
+        auto r = row(ST_TPRE, l);
+        for (size_t d = 0; d < nr; d++)
+          if (l < run_[d]->g->cfirst) {
+            r.first[d] = (uint8_t)run_[d]->cur[l];
+            r.second[d] = (uint8_t)run_[d]->cur[l + 1];
+            run_[d]->cur[l] ^= 1;
+          }
+      }
+      {
+        auto r = row(ST_COARSE, 0);
+        for (size_t d = 0; d < nr; d++) r.first[d] = (uint8_t)run_[d]->cur[run_[d]->g->cfirst];
+      }
+      for (int l = lmax_ - 1; l >= 0; l--) {
+        if (!tq_[l].n) continue;
+        auto r = row(ST_TPOST, l);
+        for (size_t d = 0; d < nr; d++)
+          if (l < run_[d]->g->cfirst) {
+            r.first[d] = (uint8_t)run_[d]->cur[l];
+            r.second[d] = (uint8_t)run_[d]->cur[l + 1];
+            run_[d]->cur[l] ^= 1;
+          }
+      }
+    }
+    for (int v = 0; v < k_ && nr && !tiled(); v++) {
       for (int l = 0; l < lmax_; l++) {
         if (!sw_[l].n && !rr_[l].n) continue;
         for (int s = 0; s < prm_.nu_pre; s++) {
@@ -1003,6 +1107,18 @@ int BatchEngine::launch_stages(const std::vector<StageRec> &st, const uint8_t *d
     case ST_PC:
       use(a, pc_[r.level]);
       if (a.ntiles) hipLaunchKernelGGL(kb_pc, dim3(grid(a.ntiles)), dim3(TPB), 0, stream_, a);
+      break;
+    case ST_TPRE:
+      use(a, tp_[r.level]);
+      a.otx = tile_otx_max(prm_.nu_pre);
+      a.oty = tune_.prow;
+      if (a.ntiles) launch_tile_pass(true, prm_.nu_pre, grid(a.ntiles), stream_, a);
+      break;
+    case ST_TPOST:
+      use(a, tq_[r.level]);
+      a.otx = tile_otx_max(prm_.nu_post);
+      a.oty = tune_.prow;
+      if (a.ntiles) launch_tile_pass(false, prm_.nu_post, grid(a.ntiles), stream_, a);
       break;
     case ST_COARSE:
       if (!run_.empty())
@@ -1183,6 +1299,42 @@ int BatchEngine::launch_sweeps_only(int reps) {
   return UBG_OK;
 }
 
+// the fused pre passes of every member level, then the post passes, `reps`
+// times (scratch buffers only, as launch_sweeps_only)
+int BatchEngine::launch_passes_only(int reps) {
+  const int n = (int)all_.size();
+  std::vector<uint8_t> idx(2 * (size_t)n, 0);
+  uint8_t *d_idx = nullptr;
+  UBG_TRY(upload_idx(idx, &d_idx));
+  auto pass = [&](bool pre, int l) {
+    const Span &sp = pre ? tp_[l] : tq_[l];
+    if (!sp.n) return;
+    StageArgs a{};
+    a.lev = d_lev_;
+    a.dom = d_dom_;
+    a.level = l;
+    a.ia = d_idx;
+    a.ib = d_idx + n;
+    a.alpha = prm_.alpha;
+    a.off = d_off_ + sp.off;
+    a.ndom = sp.nd;
+    a.ntiles = sp.n;
+    a.otx = tile_otx_max(pre ? prm_.nu_pre : prm_.nu_post);
+    a.oty = tune_.prow;
+    int b = tune_.blocks > 0 ? std::min(a.ntiles, tune_.blocks) : a.ntiles;
+    launch_tile_pass(pre, pre ? prm_.nu_pre : prm_.nu_post, std::max(b, 1), stream_, a);
+  };
+  for (int r = 0; r < reps; r++) {
+    for (int l = 0; l < lmax_; l++) pass(true, l);
+    for (int l = lmax_ - 1; l >= 0; l--) pass(false, l);
+  }
+  UBG_TRY(check_launch());
+  return UBG_OK;
+}
+
+// Times the candidate shapes over the member levels and keeps the fastest:
+// the fused passes' tile rows x block counts when the levels run as fused
+// passes, else the smoother tile shapes x block counts.
 int BatchEngine::autotune(double *best_ms) {
   const Tuning saved = tune_;
   Tuning best = saved;
@@ -1191,6 +1343,14 @@ int BatchEngine::autotune(double *best_ms) {
   UBG_HIP(hipEventCreate(&e0));
   UBG_HIP(hipEventCreate(&e1));
   const int block_opts[] = {0, 2048, 1024};
+  std::vector<Tuning> cand;
+  if (tiled()) {
+    for (int r : {8, 16, 32})
+      for (int bo : block_opts) cand.push_back(Tuning{saved.tx, saved.ty, bo, r});
+  } else {
+    for (const Shape &sh : kShapes)
+      for (int bo : block_opts) cand.push_back(Tuning{sh.tx, sh.ty, bo, saved.prow});
+  }
   // the solve state must allow the sweeps: mark members active meanwhile
   std::vector<DState> st(all_.size());
   for (size_t i = 0; i < all_.size(); i++) UBG_TRY(domain_read_state(all_[i], &st[i]));
@@ -1200,23 +1360,21 @@ int BatchEngine::autotune(double *best_ms) {
     UBG_HIP(hipMemcpyAsync(all_[i]->st, &s, sizeof(s), hipMemcpyHostToDevice, stream_));
   }
   int rc = UBG_OK;
-  for (const Shape &sh : kShapes) {
-    for (int bo : block_opts) {
-      tune_ = Tuning{sh.tx, sh.ty, bo};
-      if ((rc = build_tiles()) != UBG_OK) break;
-      if ((rc = launch_sweeps_only(1)) != UBG_OK) break; // warm
-      (void)hipEventRecord(e0, stream_);
-      if ((rc = launch_sweeps_only(3)) != UBG_OK) break;
-      (void)hipEventRecord(e1, stream_);
-      (void)hipEventSynchronize(e1);
-      float ms = 0.0f;
-      (void)hipEventElapsedTime(&ms, e0, e1);
-      if (ms < best_t) {
-        best_t = ms;
-        best = tune_;
-      }
+  for (const Tuning &c : cand) {
+    tune_ = c;
+    if ((rc = build_tiles()) != UBG_OK) break;
+    auto run = [&](int reps) { return tiled() ? launch_passes_only(reps) : launch_sweeps_only(reps); };
+    if ((rc = run(1)) != UBG_OK) break; // warm
+    (void)hipEventRecord(e0, stream_);
+    if ((rc = run(3)) != UBG_OK) break;
+    (void)hipEventRecord(e1, stream_);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms < best_t) {
+      best_t = ms;
+      best = tune_;
     }
-    if (rc) break;
   }
   for (size_t i = 0; i < all_.size(); i++)
     (void)hipMemcpyAsync(all_[i]->st, &st[i], sizeof(DState), hipMemcpyHostToDevice, stream_);
@@ -1445,8 +1603,10 @@ UBG_API int ubg_batch_set_tuning(ubg_batch *b, const ubg_batch_tuning *t) {
   if (!b || !t) return fail(UBG_E_INVALID, "null argument");
   if (!shape_ok(t->tile_w, t->tile_h)) return fail(UBG_E_INVALID, "tile shape not compiled");
   if (t->blocks < 0) return fail(UBG_E_INVALID, "negative block count");
+  const int rows = t->pass_rows ? t->pass_rows : 16;
+  if (!pass_rows_ok(rows)) return fail(UBG_E_INVALID, "pass_rows must be 8, 16 or 32");
   std::lock_guard<std::recursive_mutex> g(b->mu);
-  b->eng.set_tuning(ubg::Tuning{t->tile_w, t->tile_h, t->blocks});
+  b->eng.set_tuning(ubg::Tuning{t->tile_w, t->tile_h, t->blocks, rows});
   return UBG_OK;
 }
 
@@ -1456,6 +1616,7 @@ UBG_API int ubg_batch_get_tuning(ubg_batch *b, ubg_batch_tuning *t) {
   t->tile_w = x.tx;
   t->tile_h = x.ty;
   t->blocks = x.blocks;
+  t->pass_rows = x.prow;
   t->sweep_ms = 0.0;
   return UBG_OK;
 }
@@ -1472,11 +1633,12 @@ UBG_API int ubg_batch_autotune(ubg_batch *b, const char *db_path, ubg_batch_tuni
     while (std::getline(in, line)) {
       std::istringstream ls(line);
       std::string k;
-      int tx, ty, bl;
+      int tx, ty, bl, rows;
       double ms;
-      if (ls >> k >> tx >> ty >> bl >> ms && k == key && shape_ok(tx, ty) && bl >= 0) {
-        b->eng.set_tuning(ubg::Tuning{tx, ty, bl});
-        if (out) *out = ubg_batch_tuning{tx, ty, bl, ms};
+      if (ls >> k >> tx >> ty >> bl >> rows >> ms && k == key && shape_ok(tx, ty) && bl >= 0 &&
+          pass_rows_ok(rows)) {
+        b->eng.set_tuning(ubg::Tuning{tx, ty, bl, rows});
+        if (out) *out = ubg_batch_tuning{tx, ty, bl, rows, ms};
         return UBG_OK;
       }
     }
@@ -1485,11 +1647,11 @@ UBG_API int ubg_batch_autotune(ubg_batch *b, const char *db_path, ubg_batch_tuni
   double ms = 0.0;
   UBG_TRY(b->eng.autotune(&ms));
   const ubg::Tuning &t = b->eng.tuning();
-  if (out) *out = ubg_batch_tuning{t.tx, t.ty, t.blocks, ms};
+  if (out) *out = ubg_batch_tuning{t.tx, t.ty, t.blocks, t.prow, ms};
   if (db_path) {
     std::ofstream o(db_path, std::ios::app);
     if (!o) return fail(UBG_E_INVALID, std::string("cannot write tuning db ") + db_path);
-    o << key << " " << t.tx << " " << t.ty << " " << t.blocks << " " << ms << "\n";
+    o << key << " " << t.tx << " " << t.ty << " " << t.blocks << " " << t.prow << " " << ms << "\n";
   }
   return UBG_OK;
 }

@@ -99,7 +99,8 @@ struct BDomain {
 };
 
 struct Tuning {
-  int tx = 64, ty = 8, blocks = 0;
+  int tx = 64, ty = 8, blocks = 0; // smoother tile (sweep stages), workgroups per launch
+  int prow = 16;                   // rows of the fused-pass tiles
 };
 
 // Level sizes, first LDS level and byte counts of a W x H domain.
@@ -194,6 +195,10 @@ private:
     int nd = 0;     // domain slots covered
   };
   Span sw_[BMAXL], rr_[BMAXL], pc_[BMAXL], upd_;
+  Span tp_[BMAXL], tq_[BMAXL]; // fused level passes (pre, post)
+  // levels run as fused tile passes (smoothing counts with a tiled pass)
+  bool tiled() const { return tile_pass_ok(prm_.nu_pre) && tile_pass_ok(prm_.nu_post); }
+  int launch_passes_only(int reps);
   // stage index rows
   uint8_t *d_idx_ = nullptr;
   size_t idx_cap_ = 0;

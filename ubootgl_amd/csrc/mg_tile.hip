@@ -1,16 +1,7 @@
-// mg_tile.hip -- LDS-tiled level passes for the multigrid levels below the
-// streaming size (MG::solveLevel, pressure_solver.cpp:201-248).  A level
-// costs two launches instead of 2 + 2*nu:
-//   pre : nu red/black sweeps, residual, full-weighting restriction
-//   post: prolongation, correction, nu red/black sweeps
-// Each workgroup stages its tile plus a halo of 2 cells per sweep (and the
-// residual's and restriction's reach) in LDS and recomputes the halo cells
-// instead of exchanging them; a cell's value after k colour phases depends
-// only on cells within k of it, so every cell a workgroup writes is the value
-// the sweep-by-sweep schedule produces, bit for bit.  The sweeps read one
-// buffer and write the other (the level's p / p_alt), so halos read by one
-// workgroup are never overwritten by another.
-#include "mg_device.hpp"
+// mg_tile.hip -- LDS-tiled level passes of the single-domain multigrid for
+// the levels below the streaming size: one launch per pass, one workgroup
+// per tile (tile bodies in tile_device.hpp).
+#include "tile_device.hpp"
 
 namespace ubg {
 namespace {
@@ -21,76 +12,14 @@ namespace {
 // levels with at least this many cells use 512-thread workgroups (more of
 // them fit a CU), smaller ones 1024-thread workgroups (more waves per tile)
 constexpr size_t kTileBigCells = UBG_TILE_BIG_CELLS;
-constexpr int KL = 8; // global loads in flight per thread while staging
-constexpr int PITCH = 130; // LDS row pitch; 64 lanes cover one row's cells of a colour
-constexpr int CPITCH = 68; // coarse rows under a tile of at most 127 fine columns
-constexpr int NU_TILE = 3; // largest smoothing count with a tiled pass
 
-// One colour phase over the staged region: the cells of colour `want`
-// ((x + y) & 1 == want; red = 1 first, pressure_solver.cpp:35-47) that have
-// all four neighbours in the region and lie in the domain's interior.
-template <int WAVES>
-__device__ __forceinline__ void tile_phase(float *P, const float *F, const uint8_t *M, int LW,
-                                           int LH, int gx0, int gy0, int w, int h, int want,
-                                           float hh, float alpha) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int xhi = min(LW - 2, w - 2 - gx0);
-  const int yhi = min(LH - 2, h - 2 - gy0);
-  for (int ly = 1 + wave; ly <= yhi; ly += WAVES) {
-    const int lx = 1 + (((gx0 + 1 + gy0 + ly) ^ want) & 1) + 2 * lane;
-    if (lx <= xhi) {
-      const int i = ly * PITCH + lx;
-      P[i] = smooth(P[i], P[i - 1], P[i + 1], P[i - PITCH], P[i + PITCH], F[i], M[i], hh, alpha);
-    }
-  }
-  __syncthreads();
-}
-
-// Stage an LW x LH region of p, f and the mask (origin gx0, gy0) into LDS
-// with pitch PITCH.  Each round issues KL loads of each array per thread
-// before the first LDS store, so a round costs one memory latency.
-template <int TPB_M>
-__device__ __forceinline__ void stage_fine(float *P, float *F, uint8_t *M, const float *p,
-                                           const float *f, const uint8_t *m, int w, int gx0,
-                                           int gy0, int LW, int LH) {
-  const int n = LW * LH;
-  for (int base = 0; base < n; base += KL * TPB_M) {
-    float vp[KL], vf[KL];
-    uint8_t vm[KL];
-    int li[KL];
-#pragma unroll
-    for (int k = 0; k < KL; k++) {
-      const int j = base + (int)threadIdx.x + k * TPB_M;
-      li[k] = -1;
-      if (j < n) {
-        const int ly = j / LW, lx = j - ly * LW;
-        const size_t g = (size_t)(gy0 + ly) * w + gx0 + lx;
-        vp[k] = p[g];
-        vf[k] = f[g];
-        vm[k] = m[g];
-        li[k] = ly * PITCH + lx;
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < KL; k++)
-      if (li[k] >= 0) {
-        P[li[k]] = vp[k];
-        F[li[k]] = vf[k];
-        M[li[k]] = vm[k];
-      }
-  }
-}
-
-// fine tile (tx, ty) of an otx x oty partition (otx, oty even)
+// tile (tx, ty) of an otx x oty partition, blocks in XCD-grouped order
 __device__ __forceinline__ void tile_origin(int ntx, int &X0, int &Y0, int otx, int oty) {
   const int b = xcd_tile(blockIdx.x, gridDim.x);
   X0 = (b % ntx) * otx;
   Y0 = (b / ntx) * oty;
 }
 
-// Pre-smoothing + calculateResidualField + restrict (pressure_solver.cpp:
-// 213-227): reads p, writes the smoothed level to p_out (every cell of the
-// tile), the coarse rhs fc and a zero coarse solution ec (interior cells).
 template <int NU, int TPB_M>
 __global__ void __launch_bounds__(TPB_M)
     k_tile_pre(const float *__restrict__ p, float *__restrict__ p_out, const float *__restrict__ f,
@@ -98,62 +27,12 @@ __global__ void __launch_bounds__(TPB_M)
                float *__restrict__ fc, float *__restrict__ ec, int wc, int hc, int otx, int oty,
                int ntx) {
   extern __shared__ __align__(16) char smem[];
-  const int lh_max = oty + 3 + 4 * NU;
-  float *P = (float *)smem;
-  float *F = P + PITCH * lh_max;
-  uint8_t *M = (uint8_t *)(F + PITCH * lh_max);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int X0, Y0;
   tile_origin(ntx, X0, Y0, otx, oty);
-  // the restriction reads residuals at [X0-1, X0+otx-1], which read p at
-  // [X0-2, X0+otx]; each sweep widens that by 2
-  const int gx0 = max(X0 - 2 - 2 * NU, 0), gy0 = max(Y0 - 2 - 2 * NU, 0);
-  const int gx1 = min(X0 + otx + 2 * NU, w - 1), gy1 = min(Y0 + oty + 2 * NU, h - 1);
-  const int LW = gx1 - gx0 + 1, LH = gy1 - gy0 + 1;
-  stage_fine<TPB_M>(P, F, M, p, f, m, w, gx0, gy0, LW, LH);
-  __syncthreads();
-#pragma unroll
-  for (int s = 0; s < NU; s++) {
-    tile_phase<TPB_M / 64>(P, F, M, LW, LH, gx0, gy0, w, h, 1, hh, alpha);
-    tile_phase<TPB_M / 64>(P, F, M, LW, LH, gx0, gy0, w, h, 0, hh, alpha);
-  }
-  // the tile's own cells (boundary cells unchanged) ...
-  const int ox1 = min(X0 + otx, w), oy1 = min(Y0 + oty, h);
-  for (int y = Y0 + wave; y < oy1; y += TPB_M / 64)
-    for (int x = X0 + lane; x < ox1; x += 64)
-      p_out[(size_t)y * w + x] = P[(y - gy0) * PITCH + (x - gx0)];
-  // ... and the residual in place of f where the restriction reads it
-  const int rx0 = max(X0 - 1, 1), rx1 = min(X0 + otx - 1, w - 2);
-  const int ry0 = max(Y0 - 1, 1), ry1 = min(Y0 + oty - 1, h - 2);
-  for (int y = ry0 + wave; y <= ry1; y += TPB_M / 64)
-    for (int x = rx0 + lane; x <= rx1; x += 64) {
-      const int i = (y - gy0) * PITCH + (x - gx0);
-      F[i] = residual_of(P[i], P[i - 1], P[i + 1], P[i - PITCH], P[i + PITCH], F[i], M[i], ihsq);
-    }
-  __syncthreads();
-  // restrict (pressure_solver.cpp:118-132) for the coarse cells whose centre
-  // 2xc lies in the tile
-  const int cx0 = max(X0 / 2, 1), cx1 = min((X0 + otx) / 2, wc - 1);
-  const int cy0 = max(Y0 / 2, 1), cy1 = min((Y0 + oty) / 2, hc - 1);
-  for (int yc = cy0 + wave; yc < cy1; yc += TPB_M / 64)
-    for (int xc = cx0 + lane; xc < cx1; xc += 64) {
-      const int c = (2 * yc - gy0) * PITCH + (2 * xc - gx0);
-      auto R = [&](int dx, int dy) { return F[c + dy * PITCH + dx]; };
-      float v = 0.0f;
-      v = R(-1, -1) * 1.0f + R(0, -1) * 2.0f + R(1, -1) * 1.0f;
-      v += R(-1, 0) * 2.0f + R(0, 0) * 4.0f + R(1, 0) * 2.0f;
-      v += R(-1, 1) * 1.0f + R(0, 1) * 2.0f + R(1, 1) * 1.0f;
-      const size_t ci = (size_t)yc * wc + xc;
-      fc[ci] = v * (1.0f / 16.0f);
-      ec[ci] = 0.0f;
-    }
+  tile_pre_body<NU, TPB_M>(smem, X0, Y0, otx, oty, p, p_out, f, m, w, h, hh, alpha, ihsq, fc, ec,
+                           wc, hc);
 }
 
-// flag sum + the double literal 0.0001, stored as float (pressure_solver.cpp:149)
-__device__ __forceinline__ float eps_flag_sum(int s) { return (float)((double)s + 0.0001); }
-
-// prolongate + correct (pressure_solver.cpp:134-181) + post-smoothing: reads
-// p and the coarse solution ec, writes p_out (every cell of the tile).
 template <int NU, int TPB_M>
 __global__ void __launch_bounds__(TPB_M)
     k_tile_post(const float *__restrict__ p, float *__restrict__ p_out,
@@ -161,98 +40,10 @@ __global__ void __launch_bounds__(TPB_M)
                 float hh, float alpha, const float *__restrict__ ec,
                 const uint8_t *__restrict__ mc, int wc, int hc, int otx, int oty, int ntx) {
   extern __shared__ __align__(16) char smem[];
-  const int lh_max = oty + 4 * NU;
-  const int ch_max = lh_max / 2 + 2;
-  float *P = (float *)smem;
-  float *F = P + PITCH * lh_max;
-  float *E = F + PITCH * lh_max;          // coarse solution, pitch CPITCH
-  uint8_t *M = (uint8_t *)(E + CPITCH * ch_max);
-  uint8_t *FC = M + PITCH * lh_max;       // coarse flags, pitch CPITCH
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int X0, Y0;
   tile_origin(ntx, X0, Y0, otx, oty);
-  const int gx0 = max(X0 - 2 * NU, 0), gy0 = max(Y0 - 2 * NU, 0);
-  const int gx1 = min(X0 + otx - 1 + 2 * NU, w - 1), gy1 = min(Y0 + oty - 1 + 2 * NU, h - 1);
-  const int LW = gx1 - gx0 + 1, LH = gy1 - gy0 + 1;
-  const int ex0 = gx0 >> 1, ey0 = gy0 >> 1;
-  const int ex1 = min((gx1 >> 1) + 1, wc - 1), ey1 = min((gy1 >> 1) + 1, hc - 1);
-  {
-    // coarse values first (one round), then the fine region
-    const int CW = ex1 - ex0 + 1, nc = CW * (ey1 - ey0 + 1);
-    for (int base = 0; base < nc; base += 2 * TPB_M) {
-      float ve[2];
-      uint8_t vc[2];
-      int li[2];
-#pragma unroll
-      for (int k = 0; k < 2; k++) {
-        const int j = base + (int)threadIdx.x + k * TPB_M;
-        li[k] = -1;
-        if (j < nc) {
-          const int cy = j / CW, cx = j - cy * CW;
-          const size_t g = (size_t)(ey0 + cy) * wc + ex0 + cx;
-          ve[k] = ec[g];
-          vc[k] = mc[g] & MK_SELF;
-          li[k] = cy * CPITCH + cx;
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < 2; k++)
-        if (li[k] >= 0) {
-          E[li[k]] = ve[k];
-          FC[li[k]] = vc[k];
-        }
-    }
-  }
-  stage_fine<TPB_M>(P, F, M, p, f, m, w, gx0, gy0, LW, LH);
-  __syncthreads();
-  // e for every interior cell of the region (no neighbour dependence), by
-  // 2x2 blocks of fine cells over one coarse cell (a, b): even-even, odd-even,
-  // even-odd, odd-odd, each by its parity rule
-  {
-    const int xlo = max(gx0, 1), xhi = min(gx1, w - 2);
-    const int ylo = max(gy0, 1), yhi = min(gy1, h - 2);
-    const int a0 = xlo >> 1, na = (xhi >> 1) - a0 + 1;
-    const int b0 = ylo >> 1, nb = (yhi >> 1) - b0 + 1;
-    auto Ev = [&](int xx, int yy) { return E[(yy - ey0) * CPITCH + (xx - ex0)]; };
-    auto Fv = [&](int xx, int yy) { return (int)FC[(yy - ey0) * CPITCH + (xx - ex0)]; };
-    for (int j = threadIdx.x; j < na * nb; j += TPB_M) {
-      const int bq = j / na, a = a0 + (j - bq * na), b = b0 + bq;
-      const float e00 = Ev(a, b);
-      const int f00 = Fv(a, b);
-#pragma unroll
-      for (int dy = 0; dy < 2; dy++)
-#pragma unroll
-        for (int dx = 0; dx < 2; dx++) {
-          const int x = 2 * a + dx, y = 2 * b + dy;
-          if (x < xlo || x > xhi || y < ylo || y > yhi) continue;
-          const int i = (y - gy0) * PITCH + (x - gx0);
-          const float Fs = (M[i] & MK_SELF) ? 1.0f : 0.0f;
-          float e = 0.0f;
-          if (dx == 0 && dy == 0) {
-            e = e00 * Fs;
-          } else if (dy == 0) {
-            if (x <= w - 3) e = Fs * (e00 + Ev(a + 1, b)) / eps_flag_sum(f00 + Fv(a + 1, b));
-          } else if (dx == 0) {
-            if (y <= h - 3) e = Fs * (e00 + Ev(a, b + 1)) / eps_flag_sum(f00 + Fv(a, b + 1));
-          } else {
-            if (x <= w - 3 && y <= h - 3)
-              e = Fs * (e00 + Ev(a + 1, b + 1) + Ev(a + 1, b) + Ev(a, b + 1)) /
-                  eps_flag_sum(f00 + Fv(a + 1, b + 1) + Fv(a + 1, b) + Fv(a, b + 1));
-          }
-          P[i] += e;
-        }
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int s = 0; s < NU; s++) {
-    tile_phase<TPB_M / 64>(P, F, M, LW, LH, gx0, gy0, w, h, 1, hh, alpha);
-    tile_phase<TPB_M / 64>(P, F, M, LW, LH, gx0, gy0, w, h, 0, hh, alpha);
-  }
-  const int ox1 = min(X0 + otx, w), oy1 = min(Y0 + oty, h);
-  for (int y = Y0 + wave; y < oy1; y += TPB_M / 64)
-    for (int x = X0 + lane; x < ox1; x += 64)
-      p_out[(size_t)y * w + x] = P[(y - gy0) * PITCH + (x - gx0)];
+  tile_post_body<NU, TPB_M>(smem, X0, Y0, otx, oty, p, p_out, f, m, w, h, hh, alpha, ec, mc, wc,
+                            hc);
 }
 
 struct TileGrid {
@@ -262,7 +53,7 @@ struct TileGrid {
 // Tile shape for a level: columns as wide as the LDS pitch allows, split
 // evenly; rows (8 to 32) so that the grid holds about `target` workgroups.
 TileGrid tile_grid(int w, int h, int nu, int target) {
-  const int otx_max = (PITCH - 3 - 4 * nu) & ~1;
+  const int otx_max = tile_otx_max(nu);
   TileGrid t;
   t.ntx = (w + otx_max - 1) / otx_max;
   t.otx = ((w + t.ntx - 1) / t.ntx + 1) & ~1;
@@ -277,8 +68,7 @@ TileGrid tile_grid(int w, int h, int nu, int target) {
 template <int NU, int TPB>
 void pre_launch(Level &L, Level &C, float h, float alpha, hipStream_t s) {
   const TileGrid t = tile_grid(L.w, L.h, NU, TPB == 512 ? 1024 : 512);
-  const int lh_max = t.oty + 3 + 4 * NU;
-  const size_t shm = (size_t)PITCH * lh_max * 9;
+  const size_t shm = tile_pre_lds(t.oty, NU);
   const float ihsq = 1.0f / h / h;
   hipLaunchKernelGGL((k_tile_pre<NU, TPB>), dim3(t.ntx * t.nty), dim3(TPB), shm, s, L.p, L.p_alt,
                      L.f, L.mask, L.w, L.h, h, alpha, ihsq, C.f, C.p, C.w, C.h, t.otx, t.oty, t.ntx);
@@ -287,9 +77,7 @@ void pre_launch(Level &L, Level &C, float h, float alpha, hipStream_t s) {
 template <int NU, int TPB>
 void post_launch(Level &L, const Level &C, float h, float alpha, hipStream_t s) {
   const TileGrid t = tile_grid(L.w, L.h, NU, TPB == 512 ? 1024 : 512);
-  const int lh_max = t.oty + 4 * NU;
-  const int ch_max = lh_max / 2 + 2;
-  const size_t shm = (size_t)PITCH * lh_max * 9 + (size_t)CPITCH * ch_max * 5;
+  const size_t shm = tile_post_lds(t.oty, NU);
   hipLaunchKernelGGL((k_tile_post<NU, TPB>), dim3(t.ntx * t.nty), dim3(TPB), shm, s, L.p,
                      L.p_alt, L.f, L.mask, L.w, L.h, h, alpha, C.p, C.mask, C.w, C.h, t.otx,
                      t.oty, t.ntx);
