@@ -57,6 +57,8 @@ def main():
     ap.add_argument("--rate", type=float, default=2000.0)
     ap.add_argument("--duration", type=float, default=1.0)
     ap.add_argument("--templates", type=int, default=48)
+    ap.add_argument("--seq-requests", type=int, default=200,
+                    help="serve: requests of the stream timed as sequential solves")
     ap.add_argument("--domains", type=int, default=256)
     args = ap.parse_args()
 
@@ -351,6 +353,31 @@ def bench_serve(args):
     infos = [srv.wait(r) for r in rids]
     st = srv.stats()
     srv.close()
+    # sequential plain-GPU solves of the same requests (the first --seq-requests
+    # of the stream): one single-domain refinement solve at a time, each
+    # timestep warm-started from the previous one and stopped at the same
+    # tol * ||F f||; only the solve calls are timed (geometry setup and rhs
+    # upload are not)
+    from ubootgl_amd import simulation as Usim
+    mgs = {}
+    seq_t, seq_solves = 0.0, 0
+    for i in pick[:args.seq_requests]:
+        t = tpls[i]
+        if i not in mgs:
+            mgs[i] = Usim.MG(t.flag)
+        mg = mgs[i]
+        for k in range(len(t.rhs)):
+            mg.ir_set_rhs(t.rhs[k])  # host checks and upload: not timed
+            if k == 0:
+                mg.ir_set_guess(None)
+            t1 = time.perf_counter()
+            mg.ir_solve(float(t.h), tol=1e-6, vcycles_per_iter=4, ref_norm=float(t.fnorm[k]))
+            seq_t += time.perf_counter() - t1
+            seq_solves += 1
+    for mg in mgs.values():
+        mg.close()
+    sequential = seq_solves / seq_t
+    log(f"sequential: {seq_solves} solves in {seq_t:.3f}s -> {sequential:.1f} solves/s")
     lat = np.array([i["t_done"] - i["t_submit"] for i in infos])
     ok = sum(1 for i in infos if i["state"] == "done" and i["converged"])
     span = max(i["t_done"] for i in infos) - min(i["t_submit"] for i in infos)
@@ -374,7 +401,13 @@ def bench_serve(args):
                    "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
                    "peak_active": st["peak_active"],
                    "peak_GB": round(st["peak_bytes"] / 1e9, 3),
-                   "worker_steps": st["steps"], "worker_busy_s": round(st["busy_s"], 3)},
+                   "worker_steps": st["steps"], "worker_busy_s": round(st["busy_s"], 3),
+                   "worker_s": {k: round(st[k + "_s"], 3)
+                                for k in ("admit", "setup", "wait", "retire")},
+                   "sequential_solves_per_s": round(sequential, 3),
+                   "sequential_sample": f"the first {min(args.seq_requests, len(pick))} "
+                                        f"requests of the stream, one solve at a time",
+                   "speedup_vs_sequential": round(solves / span / sequential, 2)},
     }
     print(json.dumps(out), flush=True)
 

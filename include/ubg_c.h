@@ -160,6 +160,9 @@ typedef struct ubg_ir_params {
   double tol;           /* stop when ||r_k||_2 <= tol * ||r_0||_2, 1e-10 */
   int vcycles_per_iter; /* FP32 V-cycles per refinement step, 4 */
   int max_iter;         /* refinement steps at most, 200 */
+  double ref_norm;      /* > 0: residuals relative to this norm instead of
+                           ||r_0|| (a warm start measured against ||F f||
+                           of its rhs, as served solves are); 0 */
 } ubg_ir_params;
 typedef struct ubg_ir_result {
   int iterations;       /* refinement steps taken */
@@ -198,7 +201,7 @@ typedef struct ubg_batch_tuning {
                          used when a smoothing count exceeds 3            */
   int blocks;         /* workgroups per launch (0: one per tile)            */
   int pass_rows;      /* rows of the fused level-pass tiles: 8, 16 or 32
-                         (0: 16); their width follows from the smoothing
+                         (0: 32); their width follows from the smoothing
                          count                                              */
   double sweep_ms;    /* measured time of the tuning pass (output)          */
 } ubg_batch_tuning;
@@ -259,6 +262,10 @@ typedef struct ubg_server_stats {
   int active, queued, peak_active;
   size_t bytes_in_use, peak_bytes;
   double busy_s;        /* worker time spent in steps                      */
+  double setup_s;       /* of which: member tables, schedule, launches     */
+  double wait_s;        /*           waiting for the device                */
+  double retire_s;      /*           reading states, next rhs, results     */
+  double admit_s;       /* worker time admitting requests (not in busy_s)  */
 } ubg_server_stats;
 UBG_API void ubg_server_params_default(ubg_server_params *out);
 UBG_API int ubg_server_create(const ubg_server_params *prm, const ubg_params *mg,

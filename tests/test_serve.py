@@ -71,6 +71,34 @@ def test_serve_residual_independent_check(gpu, templates):
     assert info["rel_residual"] <= 1e-6
 
 
+def test_serve_equals_sequential_warm_solves(gpu, templates):
+    """A request's timesteps equal the single-domain refinement solver run
+    one timestep after another, warm-started, with ref_norm = ||F f|| of each
+    rhs (the criterion the server applies), bit for bit."""
+    import ubootgl_amd as U
+    t = templates[1]
+    info = run_alone(t)
+    mg = U.MG(t.flag)
+    its = 0
+    for k in range(len(t.rhs)):
+        mg.ir_set_rhs(t.rhs[k])
+        if k == 0:
+            mg.ir_set_guess(None)
+        r = mg.ir_solve(float(t.h), tol=1e-6, vcycles_per_iter=4, ref_norm=t.fnorm[k])
+        assert r["converged"] and r["rel_residual"] <= 1e-6
+        its += r["iterations"]
+    x = mg.ir_solution()
+    mg.close()
+    assert np.array_equal(x, info["x"])
+    # a warm start that already meets the tolerance takes no step
+    mg = U.MG(t.flag)
+    mg.ir_set_rhs(t.rhs[-1])
+    mg.ir_set_guess(x)
+    r = mg.ir_solve(float(t.h), tol=1e-6, ref_norm=t.fnorm[-1])
+    assert r["converged"] and r["iterations"] == 0
+    mg.close()
+
+
 def test_serve_budget_admission(gpu, templates):
     """A budget that holds the geometries and about two requests: the rest
     queue and are admitted as memory frees up; all finish, the arena's peak

@@ -59,7 +59,8 @@ IR_HISTORY = 64
 
 
 class IRParams(C.Structure):
-    _fields_ = [("tol", C.c_double), ("vcycles_per_iter", C.c_int), ("max_iter", C.c_int)]
+    _fields_ = [("tol", C.c_double), ("vcycles_per_iter", C.c_int), ("max_iter", C.c_int),
+                ("ref_norm", C.c_double)]
 
 
 class IRResult(C.Structure):
@@ -88,7 +89,8 @@ class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_long), ("completed", C.c_long), ("failed", C.c_long),
                 ("solves", C.c_long), ("steps", C.c_long), ("active", C.c_int),
                 ("queued", C.c_int), ("peak_active", C.c_int), ("bytes_in_use", C.c_size_t),
-                ("peak_bytes", C.c_size_t), ("busy_s", C.c_double)]
+                ("peak_bytes", C.c_size_t), ("busy_s", C.c_double), ("setup_s", C.c_double),
+                ("wait_s", C.c_double), ("retire_s", C.c_double), ("admit_s", C.c_double)]
 
 
 _vp = C.c_void_p

@@ -1603,7 +1603,7 @@ UBG_API int ubg_batch_set_tuning(ubg_batch *b, const ubg_batch_tuning *t) {
   if (!b || !t) return fail(UBG_E_INVALID, "null argument");
   if (!shape_ok(t->tile_w, t->tile_h)) return fail(UBG_E_INVALID, "tile shape not compiled");
   if (t->blocks < 0) return fail(UBG_E_INVALID, "negative block count");
-  const int rows = t->pass_rows ? t->pass_rows : 16;
+  const int rows = t->pass_rows ? t->pass_rows : 32;
   if (!pass_rows_ok(rows)) return fail(UBG_E_INVALID, "pass_rows must be 8, 16 or 32");
   std::lock_guard<std::recursive_mutex> g(b->mu);
   b->eng.set_tuning(ubg::Tuning{t->tile_w, t->tile_h, t->blocks, rows});

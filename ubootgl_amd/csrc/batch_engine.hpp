@@ -100,7 +100,7 @@ struct BDomain {
 
 struct Tuning {
   int tx = 64, ty = 8, blocks = 0; // smoother tile (sweep stages), workgroups per launch
-  int prow = 16;                   // rows of the fused-pass tiles
+  int prow = 32;                   // rows of the fused-pass tiles
 };
 
 // Level sizes, first LDS level and byte counts of a W x H domain.

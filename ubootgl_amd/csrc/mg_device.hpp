@@ -32,17 +32,22 @@ __device__ __forceinline__ float keep_if(unsigned mk, int k, float v) {
 
 // smoothingKernel (pressure_solver.cpp:10-24) from neighbour values and mask;
 // the neighbour-flag count is stored in mask bits 5-7.
-__device__ __forceinline__ float smooth(float pc, float pw, float pe, float ps, float pn,
-                                        float fv, unsigned mk, float h, float alpha) {
+// ... with the source term fhh = (f * h) * h already formed
+__device__ __forceinline__ float smooth_fhh(float pc, float pw, float pe, float ps, float pn,
+                                            float fhh, unsigned mk, float alpha) {
   float val = 0.0f;
   val += keep_if(mk, 1, pw);
   val += keep_if(mk, 2, pe);
   val += keep_if(mk, 3, ps);
   val += keep_if(mk, 4, pn);
-  val += fv * h * h;
+  val += fhh;
   val = div_count(val, mk >> 5);
   float F = (mk & MK_SELF) ? 1.0f : 0.0f;
   return F * (alpha * val + (1.0f - alpha) * pc);
+}
+__device__ __forceinline__ float smooth(float pc, float pw, float pe, float ps, float pn,
+                                        float fv, unsigned mk, float h, float alpha) {
+  return smooth_fhh(pc, pw, pe, ps, pn, fv * h * h, mk, alpha);
 }
 
 // calculateResidualField for one cell (pressure_solver.cpp:100-108):

@@ -175,6 +175,7 @@ UBG_API void ubg_ir_params_default(ubg_ir_params *out) {
   out->tol = 1e-10;
   out->vcycles_per_iter = 4;
   out->max_iter = 200;
+  out->ref_norm = 0.0;
 }
 
 UBG_API int ubg_mg_ir_set_rhs(ubg_mg *m, const float *f) {
@@ -216,6 +217,8 @@ UBG_API int ubg_mg_ir_solve(ubg_mg *m, float h, const ubg_ir_params *prm, ubg_ir
   if (P.vcycles_per_iter < 1 || P.vcycles_per_iter > 256 || P.max_iter < 1 || P.max_iter > 1000000)
     return fail(UBG_E_INVALID, "vcycles_per_iter or max_iter out of range");
   if (!(h > 0.0f) || !std::isfinite(h)) return fail(UBG_E_INVALID, "h must be positive");
+  if (!(P.ref_norm >= 0.0) || !std::isfinite(P.ref_norm))
+    return fail(UBG_E_INVALID, "ref_norm must be finite and >= 0");
   std::lock_guard<std::recursive_mutex> g(m->mu);
   UBG_TRY(ir_ensure(m));
   std::memset(res, 0, sizeof(*res));
@@ -230,13 +233,21 @@ UBG_API int ubg_mg_ir_solve(ubg_mg *m, float h, const ubg_ir_params *prm, ubg_ir
     res->converged = 1;
     return UBG_OK;
   }
-  double best = 1.0, rel = 1.0;
+  // relative to ref_norm when given (the batched solver's rule: a warm start
+  // that already meets tol takes no step)
+  const double denom = P.ref_norm > 0.0 ? P.ref_norm : r0;
+  double best = 1.0, rel = r0 / denom;
+  res->rel_residual = rel;
+  if (rel <= P.tol) {
+    res->converged = 1;
+    return UBG_OK;
+  }
   int stall = 0;
   for (int it = 1; it <= P.max_iter; it++) {
     for (int v = 0; v < P.vcycles_per_iter; v++) m->solve_level(0, h, false);
     UBG_TRY(check_launch());
     UBG_TRY(ir_update(m, true, ihsq, &n2));
-    rel = std::sqrt(n2) / r0;
+    rel = std::sqrt(n2) / denom;
     res->iterations = it;
     res->vcycles += P.vcycles_per_iter;
     res->rel_residual = rel;

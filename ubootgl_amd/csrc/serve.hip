@@ -100,6 +100,7 @@ bool worker_turn(ubg_server *S) {
     }
   }
   std::lock_guard<std::mutex> elk(S->eng_mu);
+  const double t_turn = S->now();
   // admission, in arrival order
   std::vector<SrvReq *> admitted;
   {
@@ -144,7 +145,9 @@ bool worker_turn(ubg_server *S) {
   for (SrvReq *r : S->fresh) fd.push_back(r->dom);
   int rc = S->eng.set_members(rd, fd);
   if (rc == UBG_OK) rc = S->eng.enqueue_steps(1);
+  const double t_launched = S->now();
   if (rc == UBG_OK) rc = S->eng.sync();
+  const double t_synced = S->now();
   std::vector<SrvReq *> members = S->running;
   members.insert(members.end(), S->fresh.begin(), S->fresh.end());
   S->running.clear();
@@ -193,7 +196,12 @@ bool worker_turn(ubg_server *S) {
   S->fresh = next_fresh;
   std::lock_guard<std::mutex> lk(S->mu);
   S->st.steps++;
-  S->st.busy_s += S->now() - t_step;
+  const double t_end = S->now();
+  S->st.busy_s += t_end - t_step;
+  S->st.setup_s += t_launched - t_step;
+  S->st.wait_s += t_synced - t_launched;
+  S->st.retire_s += t_end - t_synced;
+  S->st.admit_s += t_step - t_turn;
   S->st.active = (int)(S->running.size() + S->fresh.size());
   S->st.peak_active = std::max(S->st.peak_active, S->st.active);
   return true;

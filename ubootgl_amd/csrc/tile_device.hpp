@@ -21,33 +21,38 @@ constexpr int CPITCH = 68; // coarse rows under a tile of at most 127 fine colum
 constexpr int NU_TILE = 3; // largest smoothing count with a tiled pass
 constexpr int KL = 8;      // global loads in flight per thread while staging
 
-// One colour phase over the staged region: the cells of colour `want`
-// ((x + y) & 1 == want; red = 1 first, pressure_solver.cpp:35-47) that have
-// all four neighbours in the region and lie in the domain's interior.
+// Colour phase j (1-based) over the staged region: the cells of colour
+// `want` ((x + y) & 1 == want; red = 1 first, pressure_solver.cpp:35-47) in
+// the domain's interior that can still become exact: after phase j-1 only
+// cells at distance >= j-1 from a halo edge of the region hold the
+// sweep-by-sweep values, so phase j updates those at distance >= j (an edge
+// on the domain boundary is no halo edge: nothing outside it is missing).
+// F holds (f * h) * h.
 template <int WAVES>
 __device__ __forceinline__ void tile_phase(float *P, const float *F, const uint8_t *M, int LW,
                                            int LH, int gx0, int gy0, int w, int h, int want,
-                                           float hh, float alpha) {
+                                           int j, float alpha) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int xhi = min(LW - 2, w - 2 - gx0);
-  const int yhi = min(LH - 2, h - 2 - gy0);
-  for (int ly = 1 + wave; ly <= yhi; ly += WAVES) {
-    const int lx = 1 + (((gx0 + 1 + gy0 + ly) ^ want) & 1) + 2 * lane;
+  const int xlo = gx0 == 0 ? 1 : j, ylo = gy0 == 0 ? 1 : j;
+  const int xhi = (gx0 + LW == w) ? LW - 2 : LW - 1 - j;
+  const int yhi = (gy0 + LH == h) ? LH - 2 : LH - 1 - j;
+  for (int ly = ylo + wave; ly <= yhi; ly += WAVES) {
+    const int lx = xlo + (((gx0 + xlo + gy0 + ly) ^ want) & 1) + 2 * lane;
     if (lx <= xhi) {
       const int i = ly * PITCH + lx;
-      P[i] = smooth(P[i], P[i - 1], P[i + 1], P[i - PITCH], P[i + PITCH], F[i], M[i], hh, alpha);
+      P[i] = smooth_fhh(P[i], P[i - 1], P[i + 1], P[i - PITCH], P[i + PITCH], F[i], M[i], alpha);
     }
   }
   __syncthreads();
 }
 
-// Stage an LW x LH region of p, f and the mask (origin gx0, gy0) into LDS
-// with pitch PITCH.  Each round issues KL loads of each array per thread
-// before the first LDS store, so a round costs one memory latency.
+// Stage an LW x LH region of p, (f * hh) * hh and the mask (origin gx0,
+// gy0) into LDS with pitch PITCH.  Each round issues KL loads of each array
+// per thread before the first LDS store, so a round costs one memory latency.
 template <int TPB_M>
 __device__ __forceinline__ void stage_fine(float *P, float *F, uint8_t *M, const float *p,
                                            const float *f, const uint8_t *m, int w, int gx0,
-                                           int gy0, int LW, int LH) {
+                                           int gy0, int LW, int LH, float hh) {
   const int n = LW * LH;
   for (int base = 0; base < n; base += KL * TPB_M) {
     float vp[KL], vf[KL];
@@ -70,7 +75,7 @@ __device__ __forceinline__ void stage_fine(float *P, float *F, uint8_t *M, const
     for (int k = 0; k < KL; k++)
       if (li[k] >= 0) {
         P[li[k]] = vp[k];
-        F[li[k]] = vf[k];
+        F[li[k]] = vf[k] * hh * hh;
         M[li[k]] = vm[k];
       }
   }
@@ -106,25 +111,43 @@ tile_pre_body(char *smem, int X0, int Y0, int otx, int oty, const float *__restr
   const int gx0 = max(X0 - 2 - 2 * NU, 0), gy0 = max(Y0 - 2 - 2 * NU, 0);
   const int gx1 = min(X0 + otx + 2 * NU, w - 1), gy1 = min(Y0 + oty + 2 * NU, h - 1);
   const int LW = gx1 - gx0 + 1, LH = gy1 - gy0 + 1;
-  stage_fine<TPB_M>(P, F, M, p, f, m, w, gx0, gy0, LW, LH);
+  constexpr int WAVES = TPB_M / 64;
+  // f where the residual is taken (rows ry0..ry1 <= 33 of them, columns
+  // rx0..rx1 <= 127), loaded now and used after the sweeps
+  const int rx0 = max(X0 - 1, 1), rx1 = min(X0 + otx - 1, w - 2);
+  const int ry0 = max(Y0 - 1, 1), ry1 = min(Y0 + oty - 1, h - 2);
+  constexpr int RR = (33 + WAVES - 1) / WAVES;
+  float fr[RR][2];
+#pragma unroll
+  for (int k = 0; k < RR; k++)
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      const int y = ry0 + wave + k * WAVES, x = rx0 + lane + 64 * c;
+      fr[k][c] = (y <= ry1 && x <= rx1) ? f[(size_t)y * w + x] : 0.0f;
+    }
+  stage_fine<TPB_M>(P, F, M, p, f, m, w, gx0, gy0, LW, LH, hh);
   __syncthreads();
 #pragma unroll
   for (int s = 0; s < NU; s++) {
-    tile_phase<TPB_M / 64>(P, F, M, LW, LH, gx0, gy0, w, h, 1, hh, alpha);
-    tile_phase<TPB_M / 64>(P, F, M, LW, LH, gx0, gy0, w, h, 0, hh, alpha);
+    tile_phase<WAVES>(P, F, M, LW, LH, gx0, gy0, w, h, 1, 2 * s + 1, alpha);
+    tile_phase<WAVES>(P, F, M, LW, LH, gx0, gy0, w, h, 0, 2 * s + 2, alpha);
   }
   // the tile's own cells (boundary cells unchanged) ...
   const int ox1 = min(X0 + otx, w), oy1 = min(Y0 + oty, h);
   for (int y = Y0 + wave; y < oy1; y += TPB_M / 64)
     for (int x = X0 + lane; x < ox1; x += 64)
       p_out[(size_t)y * w + x] = P[(y - gy0) * PITCH + (x - gx0)];
-  // ... and the residual in place of f where the restriction reads it
-  const int rx0 = max(X0 - 1, 1), rx1 = min(X0 + otx - 1, w - 2);
-  const int ry0 = max(Y0 - 1, 1), ry1 = min(Y0 + oty - 1, h - 2);
-  for (int y = ry0 + wave; y <= ry1; y += TPB_M / 64)
-    for (int x = rx0 + lane; x <= rx1; x += 64) {
-      const int i = (y - gy0) * PITCH + (x - gx0);
-      F[i] = residual_of(P[i], P[i - 1], P[i + 1], P[i - PITCH], P[i + PITCH], F[i], M[i], ihsq);
+  // ... and the residual in place of F where the restriction reads it
+#pragma unroll
+  for (int k = 0; k < RR; k++)
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      const int y = ry0 + wave + k * WAVES, x = rx0 + lane + 64 * c;
+      if (y <= ry1 && x <= rx1) {
+        const int i = (y - gy0) * PITCH + (x - gx0);
+        F[i] = residual_of(P[i], P[i - 1], P[i + 1], P[i - PITCH], P[i + PITCH], fr[k][c], M[i],
+                           ihsq);
+      }
     }
   __syncthreads();
   // restrict (pressure_solver.cpp:118-132) for the coarse cells whose centre
@@ -197,7 +220,7 @@ tile_post_body(char *smem, int X0, int Y0, int otx, int oty, const float *__rest
         }
     }
   }
-  stage_fine<TPB_M>(P, F, M, p, f, m, w, gx0, gy0, LW, LH);
+  stage_fine<TPB_M>(P, F, M, p, f, m, w, gx0, gy0, LW, LH, hh);
   __syncthreads();
   // e for every interior cell of the region (no neighbour dependence), by
   // 2x2 blocks of fine cells over one coarse cell (a, b): even-even, odd-even,
@@ -240,8 +263,8 @@ tile_post_body(char *smem, int X0, int Y0, int otx, int oty, const float *__rest
   __syncthreads();
 #pragma unroll
   for (int s = 0; s < NU; s++) {
-    tile_phase<TPB_M / 64>(P, F, M, LW, LH, gx0, gy0, w, h, 1, hh, alpha);
-    tile_phase<TPB_M / 64>(P, F, M, LW, LH, gx0, gy0, w, h, 0, hh, alpha);
+    tile_phase<TPB_M / 64>(P, F, M, LW, LH, gx0, gy0, w, h, 1, 2 * s + 1, alpha);
+    tile_phase<TPB_M / 64>(P, F, M, LW, LH, gx0, gy0, w, h, 0, 2 * s + 2, alpha);
   }
   const int ox1 = min(X0 + otx, w), oy1 = min(Y0 + oty, h);
   for (int y = Y0 + wave; y < oy1; y += TPB_M / 64)

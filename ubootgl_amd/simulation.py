@@ -148,9 +148,11 @@ class MG:
         x = np.ascontiguousarray(x, np.float64)
         check(lib().ubg_mg_ir_set_guess(self._h, x.ctypes.data_as(C.POINTER(C.c_double))))
 
-    def ir_solve(self, h, tol=1e-10, vcycles_per_iter=4, max_iter=200, raise_on_stall=True):
-        """Refine to ||r|| <= tol ||r0||; returns a dict of the ubg_ir_result."""
-        prm = L.IRParams(tol, vcycles_per_iter, max_iter)
+    def ir_solve(self, h, tol=1e-10, vcycles_per_iter=4, max_iter=200, raise_on_stall=True,
+                 ref_norm=0.0):
+        """Refine to ||r|| <= tol ||r0|| (or tol * ref_norm when ref_norm > 0);
+        returns a dict of the ubg_ir_result."""
+        prm = L.IRParams(tol, vcycles_per_iter, max_iter, ref_norm)
         res = L.IRResult()
         rc = lib().ubg_mg_ir_solve(self._h, h, C.byref(prm), C.byref(res))
         if rc != L.E_NOT_CONVERGED or raise_on_stall:
