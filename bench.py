@@ -30,9 +30,14 @@ PRE_PASS_BYTES = 15.0       # p r 4 + f 4 + flag 1 + p w 4 + coarse rhs w 1 + co
 POST_PASS_BYTES = 14.25     # p r 4 + f 4 + flag 1 + coarse p 1 + coarse flag 0.25 + p w 4
 STEP_BYTES_PER_CELL_22 = 278.6  # step at 2/2 smoothing, 2 V-cycles (algorithmic)
 TRACER_BYTES = 24  # xy r/w 16 + age r/w 8 per point
+ADVECT_BYTES_PER_CELL = 17  # vx, vy r 8 + flag 1 + vx, vy w 8
+# VALU issue peak in wave64 instructions/s from MI355X's published fp32
+# vector peak: 157.3 TFLOP/s / (64 lanes x 2 flops of an FMA)
+VALU_PEAK_GINST = 157.3e3 / 128
 KERNEL_NAMES = {"sweep": "level-0 rbgs sweep (red + black, one launch)",
                 "pre": "level-0 fused pass: rbgs sweep + residual + restriction",
-                "post": "level-0 fused pass: prolongation + correction + rbgs sweep"}
+                "post": "level-0 fused pass: prolongation + correction + rbgs sweep",
+                "advect": "advect: RK2 backtrace + 3 Catmull-Rom samples per face (vx and vy)"}
 
 
 def log(msg):
@@ -50,6 +55,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=120)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--copies", type=int, default=0,
+                    help="step: after the run, N saveCurrentVelocityFields stages (plain "
+                         "33.5 MB device copies; counter calibration)")
     ap.add_argument("--workload", choices=["step", "ir", "batch", "serve"], default="step",
                     help="step: the 4096x2048 fluid step (default); ir: the 8192x4096 "
                          "FP64-refined solve to 1e-10; batch: 256 uneven domains to 1e-6; "
@@ -122,7 +130,8 @@ def main():
     ninstr = max(10, min(args.steps, 50))
     timers = ((LL.TIMER_SMOOTH_L0, "sweep", SMOOTH_BYTES_PER_CELL, SMOOTH_BYTES_PER_CELL),
               (LL.TIMER_L0_PRE, "pre", PRE_PASS_BYTES, SMOOTH_BYTES_PER_CELL + RR_BYTES_PER_CELL),
-              (LL.TIMER_L0_POST, "post", POST_PASS_BYTES, PC_BYTES_PER_CELL + SMOOTH_BYTES_PER_CELL))
+              (LL.TIMER_L0_POST, "post", POST_PASS_BYTES, PC_BYTES_PER_CELL + SMOOTH_BYTES_PER_CELL),
+              (LL.TIMER_ADVECT, "advect", ADVECT_BYTES_PER_CELL, ADVECT_BYTES_PER_CELL))
     for t, *_ in timers:
         sim.timer_enable(t, True)
     sim.run(dt, ninstr)
@@ -139,21 +148,39 @@ def main():
                           pass_GBps=pass_b * cells / (avg * 1e-3) / 1e9,
                           unfused_equiv_GBps=alg_b * cells / (avg * 1e-3) / 1e9,
                           pass_bytes=pass_b * cells)
-        log(f"level-0 {name}: {n} launches, avg {avg * 1e3:.2f} us -> "
+        log(f"{name}: {n} launches, avg {avg * 1e3:.2f} us -> "
             f"{kern[name]['pass_GBps']:.0f} GB/s compulsory, "
             f"{kern[name]['unfused_equiv_GBps']:.0f} GB/s unfused-equivalent")
+    # the dominant kernel: largest total time of the timed kernels
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
     K = kern[dom]
     achieved = K["pass_GBps"]
-    # HBM bytes per launch of the same kernel from two rocprofv3 --pmc runs
-    # (FETCH_SIZE, WRITE_SIZE), scaled by what the counters read for the
-    # step's plain 33.5 MB device copy
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "mg_l0_4096x2048_traffic.json")
+    # HBM bytes (and VALU instructions) per launch of the same kernels from
+    # rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE, SQ_INSTS_VALU), the byte
+    # counters scaled by what they read for plain 33.5 MB device copies
+    prof = {}
+    tpath = os.path.join(ROOT, "profiles", "step_4096x2048_traffic.json")
     if (d.W, d.H) == (4096, 2048) and os.path.exists(tpath):
         with open(tpath) as fh:
-            traffic = json.load(fh).get(dom, {}).get("bytes_per_launch")
+            prof = json.load(fh)
+    traffic = prof.get(dom, {}).get("bytes_per_launch")
+    valu = None
+    vi = prof.get(dom, {}).get("valu_wave_instructions_per_launch")
+    if vi:
+        rate = vi / (K["avg_us"] * 1e-6) / 1e9
+        valu = {"wave_instructions_per_launch": vi, "achieved": round(rate, 1),
+                "peak": VALU_PEAK_GINST, "unit": "G wave-instructions/s",
+                "frac": round(rate / VALU_PEAK_GINST, 4)}
+    sw = kern.get("sweep")
+    smoother = None if sw is None else {
+        "kernel": KERNEL_NAMES["sweep"], "achieved": round(sw["pass_GBps"], 1),
+        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(sw["pass_GBps"] / HBM_PEAK_GBS, 4),
+        "avg_launch_us": round(sw["avg_us"], 3),
+        "traffic": prof.get("sweep", {}).get("bytes_per_launch")}
 
+    for _ in range(args.copies):
+        sim.stage(LL.STAGE_SAVE_CURRENT, dt)
+    sim.sync()
     vxg = sim.get("vx")
     finite = bool(np.isfinite(vxg).all())
 
@@ -196,7 +223,9 @@ def main():
             "launches": K["launches"],
             "bytes_per_launch": K["pass_bytes"],
             "unfused_equiv_GBps": round(K["unfused_equiv_GBps"], 1),
-            "other_pass": {k: round(v["avg_us"], 3) for k, v in kern.items() if k != dom},
+            "valu": valu,
+            "smoother": smoother,
+            "other_kernels_us": {k: round(v["avg_us"], 3) for k, v in kern.items() if k != dom},
         },
         "cpu_baseline": cpu,
     }

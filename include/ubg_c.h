@@ -379,11 +379,12 @@ UBG_API int ubg_sim_advect_tracers(ubg_sim *sim, float dt, uint32_t seed);
 UBG_API int ubg_sim_sync(ubg_sim *sim);
 
 /* ---- measurement ---------------------------------------------------------- */
-/* Kernel timing with HIP events recorded on the handle's stream around every
- * launch of one kernel class while enabled: level-0 stand-alone smoother
- * passes, the level-0 fused pre-smoothing + residual + restriction pass, and
- * the level-0 fused prolongation + correction + post-smoothing pass. */
-enum { UBG_TIMER_SMOOTH_L0 = 0, UBG_TIMER_L0_PRE, UBG_TIMER_L0_POST, UBG_TIMER_COUNT };
+/* Kernel timing with HIP events attached to every dispatch of one kernel
+ * class while enabled: level-0 stand-alone smoother passes, the level-0
+ * fused pre-smoothing + residual + restriction pass, the level-0 fused
+ * prolongation + correction + post-smoothing pass, and advect. */
+enum { UBG_TIMER_SMOOTH_L0 = 0, UBG_TIMER_L0_PRE, UBG_TIMER_L0_POST, UBG_TIMER_ADVECT,
+       UBG_TIMER_COUNT };
 UBG_API int ubg_sim_timer_enable(ubg_sim *sim, int timer, int on);
 UBG_API int ubg_sim_timer_read(ubg_sim *sim, int timer, double *total_ms, long *launches);
 

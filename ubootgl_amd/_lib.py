@@ -34,7 +34,7 @@ FIELD_NAMES = ["vx", "vx_back", "vy", "vy_back", "vx_accum", "vy_accum", "vx_cur
 # MG level operations
 MGOP_RBGS, MGOP_ZERO_GRADIENT, MGOP_RESIDUAL, MGOP_RESTRICT_RESIDUAL, MGOP_PROLONG_CORRECT = range(5)
 LEVEL_P, LEVEL_F, LEVEL_R = range(3)
-TIMER_SMOOTH_L0, TIMER_L0_PRE, TIMER_L0_POST = range(3)
+TIMER_SMOOTH_L0, TIMER_L0_PRE, TIMER_L0_POST, TIMER_ADVECT = range(4)
 
 
 # ubg_item (components.hpp:6-42): CoItem + CoKinematics

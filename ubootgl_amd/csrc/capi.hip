@@ -440,7 +440,8 @@ struct ubg_sim {
   // simulation.cpp:241-354
   void advect() {
     const float ih = 1.0f / h;
-    launch_advect(vel(), flag, 0.5f * dt * ih, dt * ih, stream);
+    with_timer(&timers[UBG_TIMER_ADVECT],
+               [&] { launch_advect(vel(), flag, 0.5f * dt * ih, dt * ih, stream); });
     fx = 1 - fx;
     fy = 1 - fy;
     vy_bc_fresh = false;

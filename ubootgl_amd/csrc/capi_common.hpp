@@ -114,6 +114,25 @@ struct Timer {
   long launches = 0;
 };
 
+// run `fn` (one kernel launch through launch_k) with the timer's next event
+// pair attached to its dispatch, when the timer is given and enabled
+template <typename Fn> void with_timer(Timer *t, Fn fn) {
+  const bool on = t && t->on;
+  if (on) {
+    if (t->used + 2 > t->ev.size()) {
+      size_t n = t->ev.size();
+      t->ev.resize(n + 512);
+      for (size_t i = n; i < t->ev.size(); i++) (void)hipEventCreate(&t->ev[i]);
+    }
+    g_launch_ev = LaunchEvents{t->ev[t->used], t->ev[t->used + 1]};
+  }
+  fn();
+  if (on) {
+    if (!g_launch_ev.start) t->used += 2; // the launch took the pair
+    g_launch_ev = LaunchEvents{};
+  }
+}
+
 inline void default_params(ubg_params *p) {
   p->nu_pre = 3;
   p->nu_post = 3;
@@ -160,20 +179,7 @@ struct ubg_mg {
   // to its dispatch when the timer is enabled and the work is on level 0
   template <typename Fn> void timed(int id, int l, Fn fn) {
     Timer *t = timers ? &timers[id] : nullptr;
-    bool on = t && t->on && l == 0;
-    if (on) {
-      if (t->used + 2 > t->ev.size()) {
-        size_t n = t->ev.size();
-        t->ev.resize(n + 512);
-        for (size_t i = n; i < t->ev.size(); i++) (void)hipEventCreate(&t->ev[i]);
-      }
-      g_launch_ev = LaunchEvents{t->ev[t->used], t->ev[t->used + 1]};
-    }
-    fn();
-    if (on) {
-      if (!g_launch_ev.start) t->used += 2; // the launch took the pair
-      g_launch_ev = LaunchEvents{};
-    }
+    with_timer(t && l == 0 ? t : nullptr, fn);
   }
 
   // n sweeps at level l, one launch each

@@ -798,8 +798,8 @@ void launch_advect(const VelView &v, const uint8_t *flag, float hdt, float fdt, 
   int nck_y = v.W > 9 ? (v.W - 2) / 8 : 0;
   int lanes = 8 * (nck_x > nck_y ? nck_x : nck_y);
   if (lanes == 0 || v.H < 3) return;
-  hipLaunchKernelGGL(k_advect, dim3(ceil_div(lanes, TPB), v.H - 2, 2), dim3(TPB), 0, s, v.vxf,
-                     v.vyf, v.vxb, v.vyb, flag, v.W, v.H, nck_x, nck_y, hdt, fdt);
+  launch_k(k_advect, dim3(ceil_div(lanes, TPB), v.H - 2, 2), dim3(TPB), 0, s, v.vxf, v.vyf, v.vxb,
+           v.vyb, flag, v.W, v.H, nck_x, nck_y, hdt, fdt);
 }
 void launch_divergence(const VelView &v, float *f, float ih, hipStream_t s) {
   hipLaunchKernelGGL(k_divergence, dim3(ceil_div(v.W - 2, TPB), v.H - 2), dim3(TPB), 0, s, v.vxf,
