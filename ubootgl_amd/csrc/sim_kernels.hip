@@ -328,25 +328,24 @@ __device__ __forceinline__ float bicubic(const float *__restrict__ g, int gw, in
   return cubic_hermite(stx, c0, c1, c2, c3);
 }
 
-// advect (simulation.cpp:241-354).  Thread = one lane of one 8-lane chunk.
-// blockIdx.z selects the vx (0) or vy (1) sweep.  A chunk starting at
-// x0 = 1 + 8k is processed only if some lane has both flags of the skip test
-// set; the skip test and write mask differ as in the reference.  Lanes never
-// reached (tail columns, skipped chunks) keep the back buffer.
-__global__ void k_advect(const float *__restrict__ xf, const float *__restrict__ yf,
-                         float *__restrict__ xb, float *__restrict__ yb,
-                         const uint8_t *__restrict__ fl, int W, int H, int nck_x, int nck_y,
-                         float hdt, float fdt) {
+// advect (simulation.cpp:241-354) of one face: lane = one lane of one
+// 8-lane chunk (x = lane + 1) of row y, of the vx (IS_VY false) or vy sweep.
+// A chunk starting at x0 = 1 + 8k is processed only if some lane has both
+// flags of the skip test set; the skip test and write mask differ as in the
+// reference.  Lanes never reached (tail columns, skipped chunks) keep the back
+// buffer.
+template <bool IS_VY>
+__device__ __forceinline__ void advect_face(const float *__restrict__ xf,
+                                            const float *__restrict__ yf,
+                                            float *__restrict__ xb, float *__restrict__ yb,
+                                            const uint8_t *__restrict__ fl, int W, int H,
+                                            int lane, int y, float hdt, float fdt) {
   const int Wx = W - 1, Hy = H - 1;
-  const int lane = blockIdx.x * blockDim.x + threadIdx.x; // 0-based: x = lane + 1
-  const int y = blockIdx.y + 1;
-  const bool is_vy = blockIdx.z != 0;
-  if (lane >= 8 * (is_vy ? nck_y : nck_x)) return;
   const int x = lane + 1;
   const int x0 = 1 + (lane & ~7);
   const uint8_t *frow = fl + (size_t)y * W;
   bool any = false;
-  if (!is_vy) {
+  if (!IS_VY) {
 #pragma unroll
     for (int k = 0; k < 8; k++) any |= (frow[x0 + k - 1] & frow[x0 + k]) != 0;
   } else {
@@ -355,7 +354,7 @@ __global__ void k_advect(const float *__restrict__ xf, const float *__restrict__
     for (int k = 0; k < 8; k++) any |= (frow[x0 + k] & fdn[x0 + k]) != 0;
   }
   if (!any) return;
-  if (!is_vy) {
+  if (!IS_VY) {
     float posx = (float)x + 0.5f, posy = (float)y;
     float vx1 = xf[(size_t)x + (size_t)y * Wx];
     float vy1 = ((yf[x + (size_t)y * W] + yf[x + (size_t)(y - 1) * W]) +
@@ -380,6 +379,30 @@ __global__ void k_advect(const float *__restrict__ xf, const float *__restrict__
     v = v * (float)frow[x] * (float)frow[x + W];
     yb[(size_t)W * y + x] = v;
   }
+}
+
+#ifndef UBG_ADVECT_PAIRED
+#define UBG_ADVECT_PAIRED 1
+#endif
+// One launch for both sweeps.  Paired (default): a thread advects its vx face
+// and its vy face, two independent dependency chains for the scheduler to
+// interleave; else blockIdx.z selects the vx (0) or vy (1) sweep.
+__global__ void k_advect(const float *__restrict__ xf, const float *__restrict__ yf,
+                         float *__restrict__ xb, float *__restrict__ yb,
+                         const uint8_t *__restrict__ fl, int W, int H, int nck_x, int nck_y,
+                         float hdt, float fdt) {
+  const int lane = blockIdx.x * blockDim.x + threadIdx.x; // 0-based: x = lane + 1
+  const int y = blockIdx.y + 1;
+#if UBG_ADVECT_PAIRED
+  if (lane < 8 * nck_x) advect_face<false>(xf, yf, xb, yb, fl, W, H, lane, y, hdt, fdt);
+  if (lane < 8 * nck_y) advect_face<true>(xf, yf, xb, yb, fl, W, H, lane, y, hdt, fdt);
+#else
+  if (blockIdx.z == 0) {
+    if (lane < 8 * nck_x) advect_face<false>(xf, yf, xb, yb, fl, W, H, lane, y, hdt, fdt);
+  } else {
+    if (lane < 8 * nck_y) advect_face<true>(xf, yf, xb, yb, fl, W, H, lane, y, hdt, fdt);
+  }
+#endif
 }
 
 // Collocated velocity for rendering (interp_shader.cs:14-33, dispatched by
@@ -798,8 +821,8 @@ void launch_advect(const VelView &v, const uint8_t *flag, float hdt, float fdt, 
   int nck_y = v.W > 9 ? (v.W - 2) / 8 : 0;
   int lanes = 8 * (nck_x > nck_y ? nck_x : nck_y);
   if (lanes == 0 || v.H < 3) return;
-  launch_k(k_advect, dim3(ceil_div(lanes, TPB), v.H - 2, 2), dim3(TPB), 0, s, v.vxf, v.vyf, v.vxb,
-           v.vyb, flag, v.W, v.H, nck_x, nck_y, hdt, fdt);
+  launch_k(k_advect, dim3(ceil_div(lanes, TPB), v.H - 2, UBG_ADVECT_PAIRED ? 1 : 2), dim3(TPB), 0,
+           s, v.vxf, v.vyf, v.vxb, v.vyb, flag, v.W, v.H, nck_x, nck_y, hdt, fdt);
 }
 void launch_divergence(const VelView &v, float *f, float ih, hipStream_t s) {
   hipLaunchKernelGGL(k_divergence, dim3(ceil_div(v.W - 2, TPB), v.H - 2), dim3(TPB), 0, s, v.vxf,
