@@ -65,6 +65,8 @@ def main():
     ap.add_argument("--rate", type=float, default=2000.0)
     ap.add_argument("--duration", type=float, default=1.0)
     ap.add_argument("--templates", type=int, default=48)
+    ap.add_argument("--steps-per-turn", type=int, default=1,
+                    help="serve: refinement steps the worker enqueues per turn")
     ap.add_argument("--seq-requests", type=int, default=200,
                     help="serve: requests of the stream timed as sequential solves")
     ap.add_argument("--domains", type=int, default=256)
@@ -365,7 +367,8 @@ def bench_serve(args):
     tpls = workloads.serving_templates(args.templates, seed=7)
     log(f"{len(tpls)} templates, mean {np.mean([t.cells for t in tpls]) / 1e3:.0f} k cells, "
         f"generated in {time.time() - t0:.1f}s")
-    srv = S.Server(hbm_budget=200 * 10**9, tol=1e-6, vcycles_per_iter=4)
+    srv = S.Server(hbm_budget=200 * 10**9, tol=1e-6, vcycles_per_iter=4,
+                   steps_per_turn=args.steps_per_turn)
     gids = [srv.add_geometry(t.flag, t.h) for t in tpls]
     # warm-up: one request per template
     for g, t in zip(gids, tpls):

@@ -247,6 +247,9 @@ typedef struct ubg_server_params {
   double tol;           /* per timestep, relative to ||F f||, 1e-6          */
   int vcycles_per_iter; /* FP32 V-cycles per refinement step, 4            */
   int max_iter;         /* refinement steps per timestep at most, 200      */
+  int steps_per_turn;   /* refinement steps the worker enqueues per turn
+                           before it syncs and retires finished timesteps
+                           (finished domains skip the rest on the device), 1 */
 } ubg_server_params;
 typedef struct ubg_request_info {
   uint64_t id;

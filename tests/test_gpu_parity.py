@@ -244,6 +244,22 @@ def test_step_bc_kinds(gpu, bcs):
     compare(sim, ref, FIELDS + [("vx_current", O.VX_CURRENT), ("vy_current", O.VY_CURRENT)])
 
 
+@pytest.mark.parametrize("W,H,bcs", [(8, 8, (0, 1, 3, 2)), (9, 11, (1, 0, 1, 3)),
+                                     (13, 8, (3, 3, 0, 0)), (17, 9, (2, 2, 2, 1)),
+                                     (33, 20, (0, 2, 3, 3))])
+def test_step_tiny_and_odd(gpu, W, H, bcs):
+    """The smallest and odd domains: boundary duties of neighbouring edges
+    and corners meet, tiles are partial, the hierarchy is 1-3 levels."""
+    d = workloads.small_domain(W, H, seed=W * 7 + H, ndisks=1 if min(W, H) > 12 else 0)
+    sim, ref = make_pair(d)
+    sim.setBCs(*bcs)
+    ref.set_bcs(*bcs)
+    for _ in range(3):
+        sim.step(float(d.dt))
+        ref.step(float(d.dt))
+    compare(sim, ref, FIELDS + [("vx_current", O.VX_CURRENT), ("vy_current", O.VY_CURRENT)])
+
+
 def test_step_after_host_writes(gpu):
     """Accumulated velocity written by the host, then a step; a velocity
     written by the host, then the diffuse stage alone: vy's boundary is

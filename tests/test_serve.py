@@ -28,8 +28,8 @@ def templates():
     return [W.serving_template(w, h, seed=70 + i) for i, (w, h) in enumerate(SIZES)]
 
 
-def run_alone(tpl, gpu_budget=2 * 10**9):
-    srv = S.Server(hbm_budget=gpu_budget, tol=1e-6)
+def run_alone(tpl, gpu_budget=2 * 10**9, steps_per_turn=1):
+    srv = S.Server(hbm_budget=gpu_budget, tol=1e-6, steps_per_turn=steps_per_turn)
     gid = srv.add_geometry(tpl.flag, tpl.h)
     rid = srv.submit(gid, tpl.rhs, tpl.fnorm, want_x=True)
     info = srv.wait(rid)
@@ -59,6 +59,17 @@ def test_serve_alone_equals_mix(gpu, templates):
     assert st["solves"] == 3 * len(order)
     assert st["peak_active"] > 1  # requests did overlap on the device
     srv.close()
+
+
+def test_serve_steps_per_turn_invariant(gpu, templates):
+    """Several refinement steps per worker turn: a domain that converges
+    inside a turn skips the remaining ones on the device, so results and
+    iteration counts do not change."""
+    for t in (templates[0], templates[3]):
+        a = run_alone(t)
+        b = run_alone(t, steps_per_turn=3)
+        assert b["iterations"] == a["iterations"]
+        assert np.array_equal(b["x"], a["x"])
 
 
 def test_serve_residual_independent_check(gpu, templates):

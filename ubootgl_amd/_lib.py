@@ -76,7 +76,8 @@ class BatchTuning(C.Structure):
 
 class ServerParams(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int), ("tol", C.c_double),
-                ("vcycles_per_iter", C.c_int), ("max_iter", C.c_int)]
+                ("vcycles_per_iter", C.c_int), ("max_iter", C.c_int),
+                ("steps_per_turn", C.c_int)]
 
 
 class RequestInfo(C.Structure):

@@ -143,8 +143,17 @@ bool worker_turn(ubg_server *S) {
   std::vector<ubg::BDomain *> rd, fd;
   for (SrvReq *r : S->running) rd.push_back(r->dom);
   for (SrvReq *r : S->fresh) fd.push_back(r->dom);
+  // one step with the fresh members (their first update), then the rest of
+  // the turn with everyone as running: a domain whose first update already
+  // met the tolerance is inactive and skipped on the device
   int rc = S->eng.set_members(rd, fd);
   if (rc == UBG_OK) rc = S->eng.enqueue_steps(1);
+  if (rc == UBG_OK && S->prm.steps_per_turn > 1) {
+    std::vector<ubg::BDomain *> all = rd;
+    all.insert(all.end(), fd.begin(), fd.end());
+    rc = S->eng.set_members(all, {});
+    if (rc == UBG_OK) rc = S->eng.enqueue_steps(S->prm.steps_per_turn - 1);
+  }
   const double t_launched = S->now();
   if (rc == UBG_OK) rc = S->eng.sync();
   const double t_synced = S->now();
@@ -223,6 +232,7 @@ UBG_API void ubg_server_params_default(ubg_server_params *out) {
   out->tol = 1e-6;
   out->vcycles_per_iter = 4;
   out->max_iter = 200;
+  out->steps_per_turn = 1;
 }
 
 UBG_API int ubg_server_create(const ubg_server_params *prm, const ubg_params *mg,
@@ -235,7 +245,8 @@ UBG_API int ubg_server_create(const ubg_server_params *prm, const ubg_params *mg
   ubg_server_params_default(&P);
   if (prm) P = *prm;
   if (!(P.tol > 0.0 && P.tol < 1.0) || P.vcycles_per_iter < 1 || P.vcycles_per_iter > 256 ||
-      P.max_iter < 1 || P.max_active < 1 || P.hbm_budget < ((size_t)1 << 20))
+      P.max_iter < 1 || P.max_active < 1 || P.hbm_budget < ((size_t)1 << 20) ||
+      P.steps_per_turn < 1 || P.steps_per_turn > 64)
     return fail(UBG_E_INVALID, "server parameters out of range");
   ubg_params M;
   default_params(&M);

@@ -19,7 +19,7 @@ STATES = {0: "queued", 1: "running", 2: "done", 3: "failed"}
 
 class Server:
     def __init__(self, hbm_budget=200 * 10**9, max_active=1024, tol=1e-6, vcycles_per_iter=4,
-                 max_iter=200, params=None):
+                 max_iter=200, params=None, steps_per_turn=1):
         prm = L.ServerParams()
         lib().ubg_server_params_default(C.byref(prm))
         prm.hbm_budget = int(hbm_budget)
@@ -27,6 +27,7 @@ class Server:
         prm.tol = float(tol)
         prm.vcycles_per_iter = int(vcycles_per_iter)
         prm.max_iter = int(max_iter)
+        prm.steps_per_turn = int(steps_per_turn)
         h = C.c_void_p()
         check(lib().ubg_server_create(C.byref(prm), C.byref(params._p) if params else None,
                                       C.byref(h)))
