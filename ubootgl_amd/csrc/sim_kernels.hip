@@ -46,9 +46,11 @@ __device__ __forceinline__ float bc_we(const BCGrid &g, int kind, float a, float
 __device__ __forceinline__ float bc_sn(const BCGrid &g, int kind, float a, float bb) {
   return g.xpar ? vbc_per(kind, a, bb) : vbc_par(kind, a, bb);
 }
+// the front is written only when its bits change (an INFLOW side keeps its
+// value), so a reader of an unchanged front cell never meets a write
 __device__ __forceinline__ void put(const BCGrid &g, int x, int y, float v) {
   size_t i = (size_t)y * g.w + x;
-  g.f[i] = v;
+  if (__float_as_int(g.f[i]) != __float_as_int(v)) g.f[i] = v;
   g.b[i] = v;
 }
 __device__ void bc_thread(const BCGrid &g, int t) {
@@ -172,37 +174,7 @@ __global__ void k_accumulate(float *__restrict__ xf, float *__restrict__ xa,
   }
 }
 
-__device__ __forceinline__ float FL(const uint8_t *__restrict__ fl, int W, int x, int y) {
-  return (float)fl[(size_t)y * W + x];
-}
 
-// diffuse, vx part (simulation.cpp:114-129): one Jacobi iteration of the
-// masked implicit diffusion; the y-neighbour masks use column x-1.
-__global__ void k_diffuse_vx(const float *__restrict__ s, float *__restrict__ d,
-                             const uint8_t *__restrict__ fl, int W, int H, float a) {
-  const int Wx = W - 1;
-  int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int y = blockIdx.y + 1;
-  if (x >= Wx - 1) return;
-  size_t i = (size_t)y * Wx + x;
-  float vc = s[i];
-  float val = 0.0f;
-  val += s[i + 1] * FL(fl, W, x + 1, y) * FL(fl, W, x + 2, y);
-  val += s[i - 1] * FL(fl, W, x, y) * FL(fl, W, x - 1, y);
-  float fvn = FL(fl, W, x, y + 1) * FL(fl, W, x - 1, y + 1);
-  val += s[i + Wx] * fvn + (1.0f - fvn) * -vc;
-  float fvs = FL(fl, W, x, y - 1) * FL(fl, W, x - 1, y - 1);
-  val += s[i - Wx] * fvs + (1.0f - fvs) * -vc;
-  d[i] = FL(fl, W, x, y) * FL(fl, W, x + 1, y) * (vc + a * val) / (1.0f + 4.0f * a);
-}
-
-// One iteration of both diffuse loops (simulation.cpp:114-129 and 140-155)
-// with the setVBCs that follows each of them: vx and vy do not read each
-// other, and setVBCs on one component only reads that component.  Reads the
-// front buffers sx / sy, writes the back buffers dx / dy (the fronts after
-// the swap) and the boundary cells of both buffers.  The vy boundary must be
-// current for vy's interior when the first iteration starts (the reference
-// refreshes it in the setVBCs after the first vx iteration).
 __device__ __forceinline__ float fbit(unsigned m, int k) { return ((m >> k) & 1u) ? 1.0f : 0.0f; }
 
 __global__ void k_face_masks(const uint8_t *__restrict__ fl, uint16_t *__restrict__ fm, int W,
@@ -282,25 +254,6 @@ __global__ void k_diffuse_both(const float *__restrict__ sx, float *__restrict__
     if (x < Wx - 1) bc_duty(gx, x, y, vxn, nullptr);
     if (y < Hy - 1) bc_duty(gy, x, y, vyn, nullptr);
   }
-}
-
-// diffuse, vy part (simulation.cpp:140-155)
-__global__ void k_diffuse_vy(const float *__restrict__ s, float *__restrict__ d,
-                             const uint8_t *__restrict__ fl, int W, int H, float a) {
-  const int Hy = H - 1;
-  int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int y = blockIdx.y + 1;
-  if (x >= W - 1 || y >= Hy - 1) return;
-  size_t i = (size_t)y * W + x;
-  float vc = s[i];
-  float val = 0.0f;
-  val += s[i - W] * FL(fl, W, x, y) * FL(fl, W, x, y - 1);
-  val += s[i + W] * FL(fl, W, x, y + 1) * FL(fl, W, x, y + 2);
-  float fve = FL(fl, W, x + 1, y) * FL(fl, W, x + 1, y + 1);
-  val += s[i + 1] * fve + (1.0f - fve) * -vc;
-  float fvw = FL(fl, W, x - 1, y) * FL(fl, W, x - 1, y + 1);
-  val += s[i - 1] * fvw + (1.0f - fvw) * -vc;
-  d[i] = FL(fl, W, x, y) * FL(fl, W, x, y + 1) * (vc + a * val) / (1.0f + 4.0f * a);
 }
 
 // CubicHermite (interpolators.hpp:78-85), Catmull-Rom
@@ -644,15 +597,19 @@ __global__ void k_vbcs_divergence(const float *__restrict__ xf, const float *__r
   // vx faces x - 1 and x of row y, vy faces y - 1 and y of column x (all
   // exist for x <= W - 2, y <= H - 2), then their values after setVBCs: a
   // boundary face among them gets the rule applied to the other face, its
-  // interior neighbour.  That face is written by the thread of its interior
-  // neighbour (possibly this one) in the same pass; the rule reads the old
-  // value only for INFLOW, which keeps it, so old and new reads agree.
-  const float xw = xf[(size_t)y * Wx + x - 1], xe = xf[(size_t)y * Wx + x];
-  const float ys = yf[(size_t)(y - 1) * W + x], yn = yf[(size_t)y * W + x];
+  // interior neighbour.  The west / south boundary faces are written by this
+  // thread (after its reads); the east / north ones by the thread of their
+  // interior neighbour, so they are read only for INFLOW, whose rule keeps
+  // the old value and whose front write is skipped (put).
+  const bool east = x == Wx - 1, north = y == Hy - 1;
+  const float xw = xf[(size_t)y * Wx + x - 1];
+  const float xe = (!east || gx.pe == BC_INFLOW) ? xf[(size_t)y * Wx + x] : 0.0f;
+  const float ys = yf[(size_t)(y - 1) * W + x];
+  const float yn = (!north || gy.pn == BC_INFLOW) ? yf[(size_t)y * W + x] : 0.0f;
   const float vx_w = (x == 1) ? bc_we(gx, gx.pw, xe, xw) : xw;
-  const float vx_e = (x == Wx - 1) ? bc_we(gx, gx.pe, xw, xe) : xe;
+  const float vx_e = east ? bc_we(gx, gx.pe, xw, xe) : xe;
   const float vy_s = (y == 1) ? bc_sn(gy, gy.ps, yn, ys) : ys;
-  const float vy_n = (y == Hy - 1) ? bc_sn(gy, gy.pn, ys, yn) : yn;
+  const float vy_n = north ? bc_sn(gy, gy.pn, ys, yn) : yn;
   // same association as the reference's expression: ((e - w) + n) - s
   f[(size_t)y * W + x] = -ih * (vx_e - vx_w + vy_n - vy_s);
   if (x == 1 || x >= W - 3 || y == 1 || y >= H - 3) { // boundary duties
@@ -806,14 +763,6 @@ void launch_set_pbc(float *p, int W, int H, const BCs &bc, hipStream_t s) {
 void launch_accumulate(const VelView &v, float *xa, float *ya, hipStream_t s) {
   hipLaunchKernelGGL(k_accumulate, dim3(ceil_div(v.W - 2, TPB), v.H - 2), dim3(TPB), 0, s, v.vxf,
                      xa, v.vyf, ya, v.W, v.H);
-}
-void launch_diffuse_vx(const VelView &v, const uint8_t *flag, float a, hipStream_t s) {
-  hipLaunchKernelGGL(k_diffuse_vx, dim3(ceil_div(v.W - 3, TPB), v.H - 2), dim3(TPB), 0, s, v.vxf,
-                     v.vxb, flag, v.W, v.H, a);
-}
-void launch_diffuse_vy(const VelView &v, const uint8_t *flag, float a, hipStream_t s) {
-  hipLaunchKernelGGL(k_diffuse_vy, dim3(ceil_div(v.W - 2, TPB), v.H - 3), dim3(TPB), 0, s, v.vyf,
-                     v.vyb, flag, v.W, v.H, a);
 }
 void launch_advect(const VelView &v, const uint8_t *flag, float hdt, float fdt, hipStream_t s) {
   const int Wx = v.W - 1;

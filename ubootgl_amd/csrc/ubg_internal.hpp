@@ -126,8 +126,6 @@ void launch_vbcs_divergence(const VelView &v, const BCs &bc, float *f, float ih,
 void launch_correct_tail(const VelView &v, float *p, const uint16_t *fmask, float ih,
                          const BCs &bc, float *xcur, float *ycur, hipStream_t s);
 void launch_accumulate(const VelView &v, float *vx_acc, float *vy_acc, hipStream_t s);
-void launch_diffuse_vx(const VelView &v, const uint8_t *flag, float a, hipStream_t s);
-void launch_diffuse_vy(const VelView &v, const uint8_t *flag, float a, hipStream_t s);
 void launch_advect(const VelView &v, const uint8_t *flag, float hdt, float fdt, hipStream_t s);
 void launch_divergence(const VelView &v, float *f, float ih, hipStream_t s);
 void launch_stamp_sinks(float *f, int W, const int *xy, const float *val, int n, hipStream_t s);
