@@ -25,9 +25,14 @@ __device__ __forceinline__ float div_count(float v, unsigned s) {
   return (s == 0u) ? 0.0f : q2;
 }
 
-// select v if bit k of the mask is set, else +0 (bitwise, no compare)
-__device__ __forceinline__ float keep_if(unsigned mk, int k, float v) {
-  return __int_as_float(__float_as_int(v) & -(int)((mk >> k) & 1u));
+// select v if bit K of the mask is set, else +0 (bitwise, no compare): the
+// bit sign-extended to 0 / -1 by one v_bfe_i32 (written out because the
+// compiler turns the extract back into an and / compare / select triple)
+template <int K>
+__device__ __forceinline__ float keep_if(unsigned mk, float v) {
+  int sel;
+  asm("v_bfe_i32 %0, %1, %2, 1" : "=v"(sel) : "v"(mk), "i"(K));
+  return __int_as_float(__float_as_int(v) & sel);
 }
 
 // smoothingKernel (pressure_solver.cpp:10-24) from neighbour values and mask;
@@ -36,13 +41,13 @@ __device__ __forceinline__ float keep_if(unsigned mk, int k, float v) {
 __device__ __forceinline__ float smooth_fhh(float pc, float pw, float pe, float ps, float pn,
                                             float fhh, unsigned mk, float alpha) {
   float val = 0.0f;
-  val += keep_if(mk, 1, pw);
-  val += keep_if(mk, 2, pe);
-  val += keep_if(mk, 3, ps);
-  val += keep_if(mk, 4, pn);
+  val += keep_if<1>(mk, pw);
+  val += keep_if<2>(mk, pe);
+  val += keep_if<3>(mk, ps);
+  val += keep_if<4>(mk, pn);
   val += fhh;
   val = div_count(val, mk >> 5);
-  float F = (mk & MK_SELF) ? 1.0f : 0.0f;
+  const float F = (float)(mk & MK_SELF); // 0 or 1 (MK_SELF is bit 0)
   return F * (alpha * val + (1.0f - alpha) * pc);
 }
 __device__ __forceinline__ float smooth(float pc, float pw, float pe, float ps, float pn,

@@ -32,13 +32,18 @@ template <int WAVES>
 __device__ __forceinline__ void tile_phase(float *P, const float *F, const uint8_t *M, int LW,
                                            int LH, int gx0, int gy0, int w, int h, int want,
                                            int j, float alpha) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); // row loop in SALU
   const int xlo = gx0 == 0 ? 1 : j, ylo = gy0 == 0 ? 1 : j;
   const int xhi = (gx0 + LW == w) ? LW - 2 : LW - 1 - j;
   const int yhi = (gy0 + LH == h) ? LH - 2 : LH - 1 - j;
-  for (int ly = ylo + wave; ly <= yhi; ly += WAVES) {
-    const int lx = xlo + (((gx0 + xlo + gy0 + ly) ^ want) & 1) + 2 * lane;
-    if (lx <= xhi) {
+  // a wave's rows step by WAVES (even), so its column of the colour is the
+  // same in every row
+  static_assert(WAVES % 2 == 0, "a wave's rows must share the colour's column parity");
+  const int ly0 = ylo + wave;
+  const int lx = xlo + (((gx0 + xlo + gy0 + ly0) ^ want) & 1) + 2 * lane;
+  if (lx <= xhi) {
+    for (int ly = ly0; ly <= yhi; ly += WAVES) {
       const int i = ly * PITCH + lx;
       P[i] = smooth_fhh(P[i], P[i - 1], P[i + 1], P[i - PITCH], P[i + PITCH], F[i], M[i], alpha);
     }
@@ -54,21 +59,29 @@ __device__ __forceinline__ void stage_fine(float *P, float *F, uint8_t *M, const
                                            const float *f, const uint8_t *m, int w, int gx0,
                                            int gy0, int LW, int LH, float hh) {
   const int n = LW * LH;
+  // element j = threadIdx.x + TPB_M * k of the region, as (row, column)
+  // stepped by (dq, dr) = divmod(TPB_M, LW) instead of divided per element
+  const int dq = TPB_M / LW, dr = TPB_M - dq * LW;
+  int ly = (int)threadIdx.x / LW, lx = (int)threadIdx.x - ly * LW;
   for (int base = 0; base < n; base += KL * TPB_M) {
     float vp[KL], vf[KL];
     uint8_t vm[KL];
     int li[KL];
 #pragma unroll
     for (int k = 0; k < KL; k++) {
-      const int j = base + (int)threadIdx.x + k * TPB_M;
       li[k] = -1;
-      if (j < n) {
-        const int ly = j / LW, lx = j - ly * LW;
+      if (ly < LH) {
         const size_t g = (size_t)(gy0 + ly) * w + gx0 + lx;
         vp[k] = p[g];
         vf[k] = f[g];
         vm[k] = m[g];
         li[k] = ly * PITCH + lx;
+      }
+      lx += dr;
+      ly += dq;
+      if (lx >= LW) {
+        lx -= LW;
+        ly++;
       }
     }
 #pragma unroll
@@ -195,21 +208,29 @@ tile_post_body(char *smem, int X0, int Y0, int otx, int oty, const float *__rest
   const int ex1 = min((gx1 >> 1) + 1, wc - 1), ey1 = min((gy1 >> 1) + 1, hc - 1);
   {
     // coarse values first (one round), then the fine region
-    const int CW = ex1 - ex0 + 1, nc = CW * (ey1 - ey0 + 1);
+    const int CW = ex1 - ex0 + 1, CH = ey1 - ey0 + 1, nc = CW * CH;
+    // (row, column) of element threadIdx.x + TPB_M * k, stepped as in stage_fine
+    const int CWs = CW > 0 ? CW : 1; // CW < 1: nc < 1, nothing is staged
+    const int dq = TPB_M / CWs, dr = TPB_M - dq * CWs;
+    int cy = (int)threadIdx.x / CWs, cx = (int)threadIdx.x - cy * CWs;
     for (int base = 0; base < nc; base += 2 * TPB_M) {
       float ve[2];
       uint8_t vc[2];
       int li[2];
 #pragma unroll
       for (int k = 0; k < 2; k++) {
-        const int j = base + (int)threadIdx.x + k * TPB_M;
         li[k] = -1;
-        if (j < nc) {
-          const int cy = j / CW, cx = j - cy * CW;
+        if (cy < CH) {
           const size_t g = (size_t)(ey0 + cy) * wc + ex0 + cx;
           ve[k] = ec[g];
           vc[k] = mc[g] & MK_SELF;
           li[k] = cy * CPITCH + cx;
+        }
+        cx += dr;
+        cy += dq;
+        if (cx >= CWs) {
+          cx -= CWs;
+          cy++;
         }
       }
 #pragma unroll
@@ -232,8 +253,19 @@ tile_post_body(char *smem, int X0, int Y0, int otx, int oty, const float *__rest
     const int b0 = ylo >> 1, nb = (yhi >> 1) - b0 + 1;
     auto Ev = [&](int xx, int yy) { return E[(yy - ey0) * CPITCH + (xx - ex0)]; };
     auto Fv = [&](int xx, int yy) { return (int)FC[(yy - ey0) * CPITCH + (xx - ex0)]; };
-    for (int j = threadIdx.x; j < na * nb; j += TPB_M) {
-      const int bq = j / na, a = a0 + (j - bq * na), b = b0 + bq;
+    // block (a0 + ja, b0 + jb) of index j = jb * na + ja, j = threadIdx.x +
+    // TPB_M * k, stepped by divmod(TPB_M, na)
+    const int nas = na > 0 ? na : 1;
+    const int dq = TPB_M / nas, dr = TPB_M - dq * nas;
+    int jb = na > 0 ? (int)threadIdx.x / nas : nb, ja = (int)threadIdx.x - jb * nas;
+    while (jb < nb) {
+      const int a = a0 + ja, b = b0 + jb;
+      ja += dr;
+      jb += dq;
+      if (ja >= na) {
+        ja -= na;
+        jb++;
+      }
       const float e00 = Ev(a, b);
       const int f00 = Fv(a, b);
 #pragma unroll
