@@ -29,10 +29,13 @@ struct CoarseArgs {
 // (no integer division in the loops)
 __device__ __forceinline__ void lds_sweep(float *P, const float *Fv, const uint8_t *M, int w,
                                           int h, float hh, float alpha) {
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  // the row index is uniform over a wave (row loop in SALU), and a wave's
+  // rows step by 16, so its first column of the colour is the same in each
+  const int tx = threadIdx.x & 63, ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   for (int color = 0; color < 2; color++) {
+    const int x0 = 1 + ((1 + ty + color) & 1) + 2 * tx;
     for (int y = 1 + ty; y < h - 1; y += 16) {
-      for (int x = 1 + ((y + color) & 1) + 2 * tx; x < w - 1; x += 128) {
+      for (int x = x0; x < w - 1; x += 128) {
         int i = y * w + x;
         P[i] = smooth(P[i], P[i - 1], P[i + 1], P[i - w], P[i + w], Fv[i], M[i], hh, alpha);
       }
@@ -53,6 +56,7 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseArgs &a, const fl
                                                   float *p0) {
   extern __shared__ __align__(16) char smem[];
   const int tid = threadIdx.x;
+  const int wrow = __builtin_amdgcn_readfirstlane(tid >> 6); // wave index, uniform
   // load: f of the first level, masks of all levels; p = 0 everywhere.
   // Each round issues KC loads per thread before its LDS stores, so a round
   // costs one memory latency instead of one per element.
@@ -101,11 +105,11 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseArgs &a, const fl
     float *Pc = (float *)(smem + a.op[l + 1]);
     const int wc = a.w[l + 1], hc = a.h[l + 1];
     const float ihsq = 1.0f / a.hh[l] / a.hh[l];
-    for (int y = 1 + (tid >> 6); y < h - 1; y += 16)
+    for (int y = 1 + wrow; y < h - 1; y += 16)
       for (int x = 1 + (tid & 63); x < w - 1; x += 64)
         Rr[y * w + x] = lds_residual(P, Fv, M, y * w + x, w, ihsq);
     __syncthreads();
-    for (int y = 1 + (tid >> 6); y < hc - 1; y += 16)
+    for (int y = 1 + wrow; y < hc - 1; y += 16)
     for (int x = 1 + (tid & 63); x < wc - 1; x += 64) {
       int c = (2 * y) * w + 2 * x;
       auto R = [&](int dx, int dy) { return Rr[c + dy * w + dx]; };
@@ -139,7 +143,7 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseArgs &a, const fl
     // fine cells over coarse cell (ax, by), each by its parity rule
     auto E = [&](int xx, int yy) { return Pc[yy * wc + xx]; };
     auto FC = [&](int xx, int yy) { return (int)(Mc[yy * wc + xx] & MK_SELF); };
-    for (int by = (tid >> 6); by <= (h - 2) >> 1; by += 16)
+    for (int by = wrow; by <= (h - 2) >> 1; by += 16)
     for (int ax = (tid & 63); ax <= (w - 2) >> 1; ax += 64) {
       const float e00 = E(ax, by);
       const int f00 = FC(ax, by);

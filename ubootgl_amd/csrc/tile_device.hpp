@@ -118,7 +118,8 @@ tile_pre_body(char *smem, int X0, int Y0, int otx, int oty, const float *__restr
   float *P = (float *)smem;
   float *F = P + PITCH * lh_max;
   uint8_t *M = (uint8_t *)(F + PITCH * lh_max);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   // the restriction reads residuals at [X0-1, X0+otx-1], which read p at
   // [X0-2, X0+otx]; each sweep widens that by 2
   const int gx0 = max(X0 - 2 - 2 * NU, 0), gy0 = max(Y0 - 2 - 2 * NU, 0);
@@ -200,7 +201,8 @@ tile_post_body(char *smem, int X0, int Y0, int otx, int oty, const float *__rest
   float *E = F + PITCH * lh_max;          // coarse solution, pitch CPITCH
   uint8_t *M = (uint8_t *)(E + CPITCH * ch_max);
   uint8_t *FC = M + PITCH * lh_max;       // coarse flags, pitch CPITCH
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int gx0 = max(X0 - 2 * NU, 0), gy0 = max(Y0 - 2 * NU, 0);
   const int gx1 = min(X0 + otx - 1 + 2 * NU, w - 1), gy1 = min(Y0 + oty - 1 + 2 * NU, h - 1);
   const int LW = gx1 - gx0 + 1, LH = gy1 - gy0 + 1;
