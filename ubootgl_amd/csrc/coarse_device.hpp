@@ -25,17 +25,41 @@ struct CoarseArgs {
   float alpha;
 };
 
-// 1024 threads as 16 rows x 64 lanes: rows strided by 16, columns by 64
-// (no integer division in the loops)
+// 1024 threads as 16 waves x 64 lanes.  A level row with at most `count`
+// items (cells of a colour, cells, blocks) takes L lanes, L the smallest
+// power of two >= count (at most 64), so a wave holds G = 64 / L rows at
+// once: lane group g of wave ty starts at row offset ty + 16 g and steps by
+// 16 G.  On the narrow coarse levels this packs 2-8 rows into one wave
+// instead of walking them one after another.
+struct RowGroups {
+  int lr;   // lane within the row group
+  int r0;   // first row offset (wave + 16 * group)
+  int step; // row step, 16 * G
+  int L;    // lanes per row
+};
+__device__ __forceinline__ RowGroups row_groups(int count) {
+  int lg = 6;
+  while (lg > 0 && (1 << (lg - 1)) >= count) lg--;
+  const int tx = threadIdx.x & 63;
+  const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  RowGroups r;
+  r.L = 1 << lg;
+  r.lr = tx & (r.L - 1);
+  r.r0 = ty + 16 * (tx >> lg);
+  r.step = 16 << (6 - lg);
+  return r;
+}
+
+// One red+black sweep of a w x h level in LDS.  A row's cells of a colour
+// are at most (w - 1) / 2; rows of one lane group share the colour parity
+// (their offsets differ by multiples of 16).
 __device__ __forceinline__ void lds_sweep(float *P, const float *Fv, const uint8_t *M, int w,
                                           int h, float hh, float alpha) {
-  // the row index is uniform over a wave (row loop in SALU), and a wave's
-  // rows step by 16, so its first column of the colour is the same in each
-  const int tx = threadIdx.x & 63, ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const RowGroups q = row_groups((w - 1) >> 1);
   for (int color = 0; color < 2; color++) {
-    const int x0 = 1 + ((1 + ty + color) & 1) + 2 * tx;
-    for (int y = 1 + ty; y < h - 1; y += 16) {
-      for (int x = x0; x < w - 1; x += 128) {
+    const int x0 = 1 + ((1 + q.r0 + color) & 1) + 2 * q.lr;
+    for (int y = 1 + q.r0; y < h - 1; y += q.step) {
+      for (int x = x0; x < w - 1; x += 2 * q.L) {
         int i = y * w + x;
         P[i] = smooth(P[i], P[i - 1], P[i + 1], P[i - w], P[i + w], Fv[i], M[i], hh, alpha);
       }
@@ -56,7 +80,6 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseArgs &a, const fl
                                                   float *p0) {
   extern __shared__ __align__(16) char smem[];
   const int tid = threadIdx.x;
-  const int wrow = __builtin_amdgcn_readfirstlane(tid >> 6); // wave index, uniform
   // load: f of the first level, masks of all levels; p = 0 everywhere.
   // Each round issues KC loads per thread before its LDS stores, so a round
   // costs one memory latency instead of one per element.
@@ -105,12 +128,14 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseArgs &a, const fl
     float *Pc = (float *)(smem + a.op[l + 1]);
     const int wc = a.w[l + 1], hc = a.h[l + 1];
     const float ihsq = 1.0f / a.hh[l] / a.hh[l];
-    for (int y = 1 + wrow; y < h - 1; y += 16)
-      for (int x = 1 + (tid & 63); x < w - 1; x += 64)
+    const RowGroups qr = row_groups(w - 2);
+    for (int y = 1 + qr.r0; y < h - 1; y += qr.step)
+      for (int x = 1 + qr.lr; x < w - 1; x += qr.L)
         Rr[y * w + x] = lds_residual(P, Fv, M, y * w + x, w, ihsq);
     __syncthreads();
-    for (int y = 1 + wrow; y < hc - 1; y += 16)
-    for (int x = 1 + (tid & 63); x < wc - 1; x += 64) {
+    const RowGroups qc = row_groups(wc - 2);
+    for (int y = 1 + qc.r0; y < hc - 1; y += qc.step)
+    for (int x = 1 + qc.lr; x < wc - 1; x += qc.L) {
       int c = (2 * y) * w + 2 * x;
       auto R = [&](int dx, int dy) { return Rr[c + dy * w + dx]; };
       float v = 0.0f;
@@ -143,8 +168,9 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseArgs &a, const fl
     // fine cells over coarse cell (ax, by), each by its parity rule
     auto E = [&](int xx, int yy) { return Pc[yy * wc + xx]; };
     auto FC = [&](int xx, int yy) { return (int)(Mc[yy * wc + xx] & MK_SELF); };
-    for (int by = wrow; by <= (h - 2) >> 1; by += 16)
-    for (int ax = (tid & 63); ax <= (w - 2) >> 1; ax += 64) {
+    const RowGroups qp = row_groups(((w - 2) >> 1) + 1);
+    for (int by = qp.r0; by <= (h - 2) >> 1; by += qp.step)
+    for (int ax = qp.lr; ax <= (w - 2) >> 1; ax += qp.L) {
       const float e00 = E(ax, by);
       const int f00 = FC(ax, by);
 #pragma unroll
