@@ -200,9 +200,10 @@ typedef struct ubg_batch_tuning {
   int tile_w, tile_h; /* smoother tile (cells), one of the compiled shapes;
                          used when a smoothing count exceeds 3            */
   int blocks;         /* workgroups per launch (0: one per tile)            */
-  int pass_rows;      /* rows of the fused level-pass tiles: 8, 16 or 32
-                         (0: 32); their width follows from the smoothing
-                         count                                              */
+  int pass_rows;      /* rows of the fused level-pass tiles: 8, 16 or 32,
+                         or 0: chosen per level from the level's tile count
+                         (the default); each domain's columns are split
+                         evenly within the width the smoothing count allows */
   double sweep_ms;    /* measured time of the tuning pass (output)          */
 } ubg_batch_tuning;
 /* device bytes a batch of these domains allocates */

@@ -92,7 +92,8 @@ def test_batch_alone_equals_in_mix_and_tuning_invariant(gpu):
     x_alone = alone.solution(0)
     r_alone = alone.result(0)
     alone.close()
-    for shape, blocks, rows in (((32, 8), 0, 8), ((128, 4), 64, 32), ((64, 16), 7, 16)):
+    for shape, blocks, rows in (((32, 8), 0, 8), ((128, 4), 64, 32), ((64, 16), 7, 16),
+                                ((64, 8), 0, 0)):
         bt = B.Batch([(P.W, P.H) for P in probs])
         bt.set_tuning(*shape, blocks, rows)
         for d, P in enumerate(probs):
@@ -135,7 +136,7 @@ def test_batch_autotune_and_db(gpu, tmp_path):
     db = str(tmp_path / "tune.db")
     t1 = bt.autotune(db)
     assert (t1["tile_w"], t1["tile_h"]) in B.TILE_SHAPES and t1["sweep_ms"] > 0
-    assert t1["pass_rows"] in (8, 16, 32)
+    assert t1["pass_rows"] in (0, 8, 16, 32)
     assert os.path.getsize(db) > 0
     t2 = bt.autotune(db)  # reused from the file
     keys = ("tile_w", "tile_h", "blocks", "pass_rows")

@@ -90,9 +90,10 @@ class Batch:
         return dict(tile_w=t.tile_w, tile_h=t.tile_h, blocks=t.blocks, pass_rows=t.pass_rows,
                     sweep_ms=t.sweep_ms)
 
-    def set_tuning(self, tile_w, tile_h, blocks=0, pass_rows=32):
+    def set_tuning(self, tile_w, tile_h, blocks=0, pass_rows=0):
         """tile_w x tile_h: smoother tile (used when a smoothing count exceeds
-        3); pass_rows: rows of the fused level-pass tiles (8, 16 or 32)."""
+        3); pass_rows: rows of the fused level-pass tiles (8, 16 or 32; 0 =
+        chosen per level from its tile count)."""
         t = L.BatchTuning(tile_w, tile_h, blocks, pass_rows, 0.0)
         check(lib().ubg_batch_set_tuning(self._h, C.byref(t)))
 

@@ -33,14 +33,14 @@ constexpr int TPB = 256;
 struct StageArgs {
   const BLev *lev;
   const BDom *dom;
-  const int *off; // tile prefix over domain slots [ndom + 1], then tiles per row [ndom]
+  const int *off; // tile prefix over domain slots [ndom + 1], tiles per row [ndom], tile width [ndom]
   int ndom;
   int ntiles;
   int level;
   const uint8_t *ia; // per-domain index row a
   const uint8_t *ib; // per-domain index row b
   float alpha;
-  int otx, oty; // tile of the fused level passes
+  int oty; // tile rows of the fused level passes (their widths are per domain)
 };
 
 __device__ __forceinline__ bool dom_active(const BDom &D) {
@@ -55,7 +55,7 @@ __device__ __forceinline__ bool dom_active(const BDom &D) {
 struct TileWalk {
   int t, t1; // current tile, end of the chunk
   int d, j;  // domain slot, tile index within the domain
-  int nt, ntx;
+  int nt, ntx, otx;
   __device__ __forceinline__ bool init(const StageArgs &a) {
     const int per = (a.ntiles + gridDim.x - 1) / gridDim.x;
     const int b = xcd_tile(blockIdx.x, gridDim.x);
@@ -72,6 +72,7 @@ struct TileWalk {
     j = t - a.off[d];
     nt = a.off[d + 1] - a.off[d];
     ntx = a.off[a.ndom + 1 + d];
+    otx = a.off[2 * a.ndom + 1 + d];
     return true;
   }
   __device__ __forceinline__ bool next(const StageArgs &a) {
@@ -83,6 +84,7 @@ struct TileWalk {
       j = 0;
       nt = a.off[d + 1] - a.off[d];
       ntx = a.off[a.ndom + 1 + d];
+      otx = a.off[2 * a.ndom + 1 + d];
     }
     return true;
   }
@@ -261,12 +263,12 @@ __global__ void __launch_bounds__(TPB) kb_pc(StageArgs a) {
 }
 
 // The fused level passes of tile_device.hpp over the tiles of every member
-// domain at one level (otx x oty tiles, walked as the other stages): pre
-// reads p[ia], writes p[ia^1], the coarse rhs and a zero coarse p[ib]; post
-// reads p[ia] and the coarse solution p[ib], writes p[ia^1].
-constexpr int BT_TPB = 512;
-template <int NU>
-__global__ void __launch_bounds__(BT_TPB) kb_tile_pre(StageArgs a) {
+// domain at one level (each domain's own even column split, oty rows, walked
+// as the other stages): pre reads p[ia], writes p[ia^1], the coarse rhs and
+// a zero coarse p[ib]; post reads p[ia] and the coarse solution p[ib],
+// writes p[ia^1].
+template <int NU, int TPB_T>
+__global__ void __launch_bounds__(TPB_T) kb_tile_pre(StageArgs a) {
   extern __shared__ __align__(16) char smem[];
   TileWalk tw;
   if (!tw.init(a)) return;
@@ -277,13 +279,13 @@ __global__ void __launch_bounds__(BT_TPB) kb_tile_pre(StageArgs a) {
     const BLev &C = a.lev[tw.d * BMAXL + a.level + 1];
     const int s = a.ia[tw.d];
     __syncthreads(); // the previous tile's LDS readers are done
-    tile_pre_body<NU, BT_TPB>(smem, tw.tx() * a.otx, tw.ty() * a.oty, a.otx, a.oty, F.p[s],
-                              F.p[s ^ 1], F.f, F.mask, F.w, F.h, F.hh, a.alpha, F.ihsq, C.f,
-                              C.p[a.ib[tw.d]], C.w, C.h);
+    tile_pre_body<NU, TPB_T>(smem, tw.tx() * tw.otx, tw.ty() * a.oty, tw.otx, a.oty, F.p[s],
+                             F.p[s ^ 1], F.f, F.mask, F.w, F.h, F.hh, a.alpha, F.ihsq, C.f,
+                             C.p[a.ib[tw.d]], C.w, C.h);
   } while (tw.next(a));
 }
-template <int NU>
-__global__ void __launch_bounds__(BT_TPB) kb_tile_post(StageArgs a) {
+template <int NU, int TPB_T>
+__global__ void __launch_bounds__(TPB_T) kb_tile_post(StageArgs a) {
   extern __shared__ __align__(16) char smem[];
   TileWalk tw;
   if (!tw.init(a)) return;
@@ -294,9 +296,9 @@ __global__ void __launch_bounds__(BT_TPB) kb_tile_post(StageArgs a) {
     const BLev &C = a.lev[tw.d * BMAXL + a.level + 1];
     const int s = a.ia[tw.d];
     __syncthreads();
-    tile_post_body<NU, BT_TPB>(smem, tw.tx() * a.otx, tw.ty() * a.oty, a.otx, a.oty, F.p[s],
-                               F.p[s ^ 1], F.f, F.mask, F.w, F.h, F.hh, a.alpha,
-                               C.p[a.ib[tw.d]], C.mask, C.w, C.h);
+    tile_post_body<NU, TPB_T>(smem, tw.tx() * tw.otx, tw.ty() * a.oty, tw.otx, a.oty, F.p[s],
+                              F.p[s ^ 1], F.f, F.mask, F.w, F.h, F.hh, a.alpha,
+                              C.p[a.ib[tw.d]], C.mask, C.w, C.h);
   } while (tw.next(a));
 }
 
@@ -488,21 +490,25 @@ bool shape_ok(int tx, int ty) {
 
 size_t al(size_t n) { return align_up(n, 256); }
 
-// rows of the fused-pass tiles the tuning may choose
-bool pass_rows_ok(int r) { return r == 8 || r == 16 || r == 32; }
+// rows of the fused-pass tiles the tuning may fix (0: chosen per level)
+bool pass_rows_ok(int r) { return r == 0 || r == 8 || r == 16 || r == 32; }
 
-void launch_tile_pass(bool pre, int nu, int blocks, hipStream_t s, const StageArgs &a) {
+void launch_tile_pass(bool pre, int nu, int tpb, int blocks, hipStream_t s, const StageArgs &a) {
   const size_t shm = pre ? tile_pre_lds(a.oty, nu) : tile_post_lds(a.oty, nu);
-#define UBG_TP(N)                                                                               \
-  if (nu == N) {                                                                                \
-    if (pre) hipLaunchKernelGGL((kb_tile_pre<N>), dim3(blocks), dim3(BT_TPB), shm, s, a);       \
-    else hipLaunchKernelGGL((kb_tile_post<N>), dim3(blocks), dim3(BT_TPB), shm, s, a);          \
+#define UBG_TP(N, T)                                                                            \
+  if (nu == N && tpb == T) {                                                                    \
+    if (pre) hipLaunchKernelGGL((kb_tile_pre<N, T>), dim3(blocks), dim3(T), shm, s, a);        \
+    else hipLaunchKernelGGL((kb_tile_post<N, T>), dim3(blocks), dim3(T), shm, s, a);           \
     return;                                                                                     \
   }
-  UBG_TP(0)
-  UBG_TP(1)
-  UBG_TP(2)
-  UBG_TP(3)
+  UBG_TP(0, 512)
+  UBG_TP(1, 512)
+  UBG_TP(2, 512)
+  UBG_TP(3, 512)
+  UBG_TP(0, 1024)
+  UBG_TP(1, 1024)
+  UBG_TP(2, 1024)
+  UBG_TP(3, 1024)
 #undef UBG_TP
 }
 
@@ -898,17 +904,20 @@ int BatchEngine::build_tables() {
 // a member change rebuilds O(domains x levels) integers, not tile lists.
 int BatchEngine::build_tiles() {
   h_off_.clear();
+  // per span: tile prefix over the domain slots [nd + 1], tiles per row [nd]
+  // and the tile width [nd] (fused passes; 0 elsewhere)
   auto span = [&](Span &sp, int nd, auto tiles_of) {
     sp.off = h_off_.size();
     sp.nd = nd;
-    h_off_.resize(sp.off + 2 * (size_t)nd + 1);
-    int *pre = h_off_.data() + sp.off, *ntx = pre + nd + 1;
+    h_off_.resize(sp.off + 3 * (size_t)nd + 1);
+    int *pre = h_off_.data() + sp.off, *ntx = pre + nd + 1, *otx = ntx + nd;
     long tot = 0;
     for (int s = 0; s < nd; s++) {
-      int nx = 0, ny = 0;
-      tiles_of(s, nx, ny);
+      int nx = 0, ny = 0, ox = 0;
+      tiles_of(s, nx, ny, ox);
       pre[s] = (int)tot;
       ntx[s] = nx > 0 ? nx : 1;
+      otx[s] = ox;
       tot += (long)nx * ny;
     }
     pre[nd] = (int)tot;
@@ -918,42 +927,67 @@ int BatchEngine::build_tiles() {
   const int tx = tune_.tx, ty = tune_.ty;
   const int nr = (int)run_.size();
   for (int l = 0; l < BMAXL; l++) {
-    span(sw_[l], nr, [&](int s, int &nx, int &ny) {
+    span(sw_[l], nr, [&](int s, int &nx, int &ny, int &) {
       const BGeom &G = *run_[s]->g;
       if (l >= G.cfirst) return;
       nx = cdiv(G.w[l], tx);
       ny = cdiv(G.h[l], ty);
     });
-    span(rr_[l], nr, [&](int s, int &nx, int &ny) {
+    span(rr_[l], nr, [&](int s, int &nx, int &ny, int &) {
       const BGeom &G = *run_[s]->g;
       if (l >= G.cfirst) return;
       nx = cdiv(G.w[l + 1] - 2, RCX);
       ny = cdiv(G.h[l + 1] - 2, RCY);
     });
-    span(pc_[l], nr, [&](int s, int &nx, int &ny) {
+    span(pc_[l], nr, [&](int s, int &nx, int &ny, int &) {
       const BGeom &G = *run_[s]->g;
       if (l >= G.cfirst) return;
       nx = cdiv(G.w[l] - 2, PCW);
       ny = cdiv(G.h[l] - 2, PCH);
     });
   }
-  const int ty_p = tune_.prow;
-  const int tx_pre = tile_otx_max(prm_.nu_pre), tx_post = tile_otx_max(prm_.nu_post);
-  for (int l = 0; l < BMAXL; l++) {
-    span(tp_[l], nr, [&](int s, int &nx, int &ny) {
-      const BGeom &G = *run_[s]->g;
-      if (l >= G.cfirst) return;
-      nx = cdiv(G.w[l], tx_pre);
-      ny = cdiv(G.h[l], ty_p);
-    });
-    span(tq_[l], nr, [&](int s, int &nx, int &ny) {
-      const BGeom &G = *run_[s]->g;
-      if (l >= G.cfirst) return;
-      nx = cdiv(G.w[l], tx_post);
-      ny = cdiv(G.h[l], ty_p);
-    });
-  }
-  span(upd_, (int)all_.size(), [&](int s, int &nx, int &ny) {
+  // Fused passes: each domain's columns split evenly (tile_cols); the rows
+  // fixed by the tuning, or per level the most (32, 16, 8) whose tiles over
+  // all member domains still reach the launch's target, with 512-thread
+  // workgroups for levels of at least kTileBigCells cells in total and
+  // 1024-thread ones below -- the single-domain rule over the whole level.
+  auto fused = [&](Span *out, int nu) {
+    for (int l = 0; l < BMAXL; l++) {
+      size_t cells = 0;
+      for (int s = 0; s < nr; s++) {
+        const BGeom &G = *run_[s]->g;
+        if (l < G.cfirst) cells += (size_t)G.w[l] * G.h[l];
+      }
+      const int tpb = cells >= kTileBigCells ? 512 : 1024;
+      int oty = tune_.prow;
+      if (oty == 0) {
+        auto tiles = [&](int r) {
+          long t = 0;
+          for (int s = 0; s < nr; s++) {
+            const BGeom &G = *run_[s]->g;
+            if (l >= G.cfirst) continue;
+            int ox, nx;
+            tile_cols(G.w[l], nu, ox, nx);
+            t += (long)nx * cdiv(G.h[l], r);
+          }
+          return t;
+        };
+        oty = 32;
+        while (oty > 8 && tiles(oty) < tile_target(tpb)) oty /= 2;
+      }
+      out[l].oty = oty;
+      out[l].tpb = tpb;
+      span(out[l], nr, [&](int s, int &nx, int &ny, int &ox) {
+        const BGeom &G = *run_[s]->g;
+        if (l >= G.cfirst) return;
+        tile_cols(G.w[l], nu, ox, nx);
+        ny = cdiv(G.h[l], oty);
+      });
+    }
+  };
+  fused(tp_, prm_.nu_pre);
+  fused(tq_, prm_.nu_post);
+  span(upd_, (int)all_.size(), [&](int s, int &nx, int &ny, int &) {
     nx = all_[s]->gx;
     ny = all_[s]->gy;
   });
@@ -1110,15 +1144,15 @@ int BatchEngine::launch_stages(const std::vector<StageRec> &st, const uint8_t *d
       break;
     case ST_TPRE:
       use(a, tp_[r.level]);
-      a.otx = tile_otx_max(prm_.nu_pre);
-      a.oty = tune_.prow;
-      if (a.ntiles) launch_tile_pass(true, prm_.nu_pre, grid(a.ntiles), stream_, a);
+      a.oty = tp_[r.level].oty;
+      if (a.ntiles)
+        launch_tile_pass(true, prm_.nu_pre, tp_[r.level].tpb, grid(a.ntiles), stream_, a);
       break;
     case ST_TPOST:
       use(a, tq_[r.level]);
-      a.otx = tile_otx_max(prm_.nu_post);
-      a.oty = tune_.prow;
-      if (a.ntiles) launch_tile_pass(false, prm_.nu_post, grid(a.ntiles), stream_, a);
+      a.oty = tq_[r.level].oty;
+      if (a.ntiles)
+        launch_tile_pass(false, prm_.nu_post, tq_[r.level].tpb, grid(a.ntiles), stream_, a);
       break;
     case ST_COARSE:
       if (!run_.empty())
@@ -1319,10 +1353,9 @@ int BatchEngine::launch_passes_only(int reps) {
     a.off = d_off_ + sp.off;
     a.ndom = sp.nd;
     a.ntiles = sp.n;
-    a.otx = tile_otx_max(pre ? prm_.nu_pre : prm_.nu_post);
-    a.oty = tune_.prow;
+    a.oty = sp.oty;
     int b = tune_.blocks > 0 ? std::min(a.ntiles, tune_.blocks) : a.ntiles;
-    launch_tile_pass(pre, pre ? prm_.nu_pre : prm_.nu_post, std::max(b, 1), stream_, a);
+    launch_tile_pass(pre, pre ? prm_.nu_pre : prm_.nu_post, sp.tpb, std::max(b, 1), stream_, a);
   };
   for (int r = 0; r < reps; r++) {
     for (int l = 0; l < lmax_; l++) pass(true, l);
@@ -1345,7 +1378,7 @@ int BatchEngine::autotune(double *best_ms) {
   const int block_opts[] = {0, 2048, 1024};
   std::vector<Tuning> cand;
   if (tiled()) {
-    for (int r : {8, 16, 32})
+    for (int r : {0, 8, 16, 32})
       for (int bo : block_opts) cand.push_back(Tuning{saved.tx, saved.ty, bo, r});
   } else {
     for (const Shape &sh : kShapes)
@@ -1603,8 +1636,8 @@ UBG_API int ubg_batch_set_tuning(ubg_batch *b, const ubg_batch_tuning *t) {
   if (!b || !t) return fail(UBG_E_INVALID, "null argument");
   if (!shape_ok(t->tile_w, t->tile_h)) return fail(UBG_E_INVALID, "tile shape not compiled");
   if (t->blocks < 0) return fail(UBG_E_INVALID, "negative block count");
-  const int rows = t->pass_rows ? t->pass_rows : 32;
-  if (!pass_rows_ok(rows)) return fail(UBG_E_INVALID, "pass_rows must be 8, 16 or 32");
+  const int rows = t->pass_rows;
+  if (!pass_rows_ok(rows)) return fail(UBG_E_INVALID, "pass_rows must be 0, 8, 16 or 32");
   std::lock_guard<std::recursive_mutex> g(b->mu);
   b->eng.set_tuning(ubg::Tuning{t->tile_w, t->tile_h, t->blocks, rows});
   return UBG_OK;

@@ -100,7 +100,7 @@ struct BDomain {
 
 struct Tuning {
   int tx = 64, ty = 8, blocks = 0; // smoother tile (sweep stages), workgroups per launch
-  int prow = 32;                   // rows of the fused-pass tiles
+  int prow = 0;                    // rows of the fused-pass tiles (0: chosen per level)
 };
 
 // Level sizes, first LDS level and byte counts of a W x H domain.
@@ -193,6 +193,8 @@ private:
     size_t off = 0; // into the int array
     int n = 0;      // tiles
     int nd = 0;     // domain slots covered
+    int oty = 0;    // fused passes: tile rows
+    int tpb = 512;  //   and workgroup size
   };
   Span sw_[BMAXL], rr_[BMAXL], pc_[BMAXL], upd_;
   Span tp_[BMAXL], tq_[BMAXL]; // fused level passes (pre, post)

@@ -6,13 +6,6 @@
 namespace ubg {
 namespace {
 
-#ifndef UBG_TILE_BIG_CELLS
-#define UBG_TILE_BIG_CELLS (1 << 18)
-#endif
-// levels with at least this many cells use 512-thread workgroups (more of
-// them fit a CU), smaller ones 1024-thread workgroups (more waves per tile)
-constexpr size_t kTileBigCells = UBG_TILE_BIG_CELLS;
-
 // tile (tx, ty) of an otx x oty partition, blocks in XCD-grouped order
 __device__ __forceinline__ void tile_origin(int ntx, int &X0, int &Y0, int otx, int oty) {
   const int b = xcd_tile(blockIdx.x, gridDim.x);
@@ -53,11 +46,8 @@ struct TileGrid {
 // Tile shape for a level: columns as wide as the LDS pitch allows, split
 // evenly; rows (8 to 32) so that the grid holds about `target` workgroups.
 TileGrid tile_grid(int w, int h, int nu, int target) {
-  const int otx_max = tile_otx_max(nu);
   TileGrid t;
-  t.ntx = (w + otx_max - 1) / otx_max;
-  t.otx = ((w + t.ntx - 1) / t.ntx + 1) & ~1;
-  t.ntx = (w + t.otx - 1) / t.otx;
+  tile_cols(w, nu, t.otx, t.ntx);
   int oty = 32;
   while (oty > 8 && (long)t.ntx * ((h + oty - 1) / oty) < target) oty /= 2;
   t.oty = oty;
@@ -67,7 +57,7 @@ TileGrid tile_grid(int w, int h, int nu, int target) {
 
 template <int NU, int TPB>
 void pre_launch(Level &L, Level &C, float h, float alpha, hipStream_t s) {
-  const TileGrid t = tile_grid(L.w, L.h, NU, TPB == 512 ? 1024 : 512);
+  const TileGrid t = tile_grid(L.w, L.h, NU, tile_target(TPB));
   const size_t shm = tile_pre_lds(t.oty, NU);
   const float ihsq = 1.0f / h / h;
   hipLaunchKernelGGL((k_tile_pre<NU, TPB>), dim3(t.ntx * t.nty), dim3(TPB), shm, s, L.p, L.p_alt,
@@ -76,7 +66,7 @@ void pre_launch(Level &L, Level &C, float h, float alpha, hipStream_t s) {
 
 template <int NU, int TPB>
 void post_launch(Level &L, const Level &C, float h, float alpha, hipStream_t s) {
-  const TileGrid t = tile_grid(L.w, L.h, NU, TPB == 512 ? 1024 : 512);
+  const TileGrid t = tile_grid(L.w, L.h, NU, tile_target(TPB));
   const size_t shm = tile_post_lds(t.oty, NU);
   hipLaunchKernelGGL((k_tile_post<NU, TPB>), dim3(t.ntx * t.nty), dim3(TPB), shm, s, L.p,
                      L.p_alt, L.f, L.mask, L.w, L.h, h, alpha, C.p, C.mask, C.w, C.h, t.otx,

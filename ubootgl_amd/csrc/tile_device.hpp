@@ -102,6 +102,19 @@ inline size_t tile_post_lds(int oty, int nu) {
 }
 // widest even tile whose staged rows fit the LDS pitch
 inline int tile_otx_max(int nu) { return (PITCH - 3 - 4 * nu) & ~1; }
+// columns of a w-wide level's tiles: as few as the pitch allows, of even,
+// evenly split width otx (ntx of them)
+inline void tile_cols(int w, int nu, int &otx, int &ntx) {
+  const int otx_max = tile_otx_max(nu);
+  ntx = (w + otx_max - 1) / otx_max;
+  otx = ((w + ntx - 1) / ntx + 1) & ~1;
+  ntx = (w + otx - 1) / otx;
+}
+// levels with at least this many cells run 512-thread tile workgroups (more
+// of them fit a CU), smaller ones 1024-thread workgroups (more waves per tile)
+constexpr size_t kTileBigCells = size_t(1) << 18;
+// workgroups a tile launch aims for: about two per CU
+inline int tile_target(int tpb) { return tpb == 512 ? 1024 : 512; }
 
 // Pre-smoothing + calculateResidualField + restrict (pressure_solver.cpp:
 // 213-227): reads p, writes the smoothed level to p_out (every cell of the
