@@ -84,6 +84,24 @@ def test_batch_equals_single_domain(gpu):
 
 
 @pytest.mark.gpu
+def test_batch_streamed_levels_equal_single(gpu):
+    """Levels at least 512 wide with at least 2^18 cells run as row-streamed
+    passes: an odd width (unaligned loads), two streamed levels in one
+    domain, and a small tiled domain beside them -- each bit-identical to the
+    single-domain solve."""
+    probs = problems([(1023, 300), (2048, 512), (100, 60), (1024, 256)], seed=33)
+    bt = B.Batch([(P.W, P.H) for P in probs])
+    for d, P in enumerate(probs):
+        bt.set_problem(d, P)
+    bt.solve(tol=1e-6)
+    for d, P in enumerate(probs):
+        rs, xs = single(P, 1e-6)
+        assert bt.result(d) == rs, f"domain {d} {P.W}x{P.H}"
+        assert np.array_equal(bt.solution(d), xs), f"domain {d} {P.W}x{P.H}"
+    bt.close()
+
+
+@pytest.mark.gpu
 def test_batch_alone_equals_in_mix_and_tuning_invariant(gpu):
     probs = problems(MIXED_SIZES[:8], seed=5)
     alone = B.Batch([(probs[3].W, probs[3].H)])

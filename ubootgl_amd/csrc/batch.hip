@@ -14,6 +14,7 @@
 // parities run in the same launch.  Domains whose solve has ended are
 // skipped on the device; no host round trip is needed per stage.
 #include "batch_engine.hpp"
+#include "stream_device.hpp"
 #include "tile_device.hpp"
 
 #include <array>
@@ -300,6 +301,79 @@ __global__ void __launch_bounds__(TPB_T) kb_tile_post(StageArgs a) {
                               F.p[s ^ 1], F.f, F.mask, F.w, F.h, F.hh, a.alpha,
                               C.p[a.ib[tw.d]], C.mask, C.w, C.h);
   } while (tw.next(a));
+}
+
+// Row-streamed level passes (stream_device.hpp) over the units (strip x row
+// block, one wave each) of every member domain whose level is streamed.  A
+// wave finds its domain slot in the unit prefix (the last d with off[d] <=
+// unit, as the tile walk does); the table's second and third rows hold the
+// domain's strip count and row-block height.
+struct StreamUnit {
+  int d, unit, nstrips, R;
+};
+__device__ __forceinline__ bool stream_unit(const StageArgs &a, StreamUnit &u) {
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int g = xcd_tile(blockIdx.x, gridDim.x) * (TPB / 64) + wid;
+  if (g >= a.ntiles) return false; // whole wave leaves together
+  int lo = 0, hi = a.ndom;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (a.off[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  u.d = lo;
+  u.unit = g - a.off[lo];
+  u.nstrips = a.off[a.ndom + 1 + lo];
+  u.R = a.off[2 * a.ndom + 1 + lo];
+  return true;
+}
+
+// one red+black sweep: reads p[ia], writes p[ia^1]
+__global__ void __launch_bounds__(TPB) kb_stream_sweep(StageArgs a) {
+  StreamUnit u;
+  if (!stream_unit(a, u)) return;
+  if (!dom_active(a.dom[u.d])) return;
+  const BLev &F = a.lev[u.d * BMAXL + a.level];
+  const int s = a.ia[u.d];
+  if ((F.w & 1) == 0)
+    rbgs_pair_unit<false, true>(F.p[s], F.p[s ^ 1], F.f, F.mask, F.w, F.h, F.hh, a.alpha,
+                                u.nstrips, u.R, u.unit);
+  else
+    rbgs_pair_unit<false, false>(F.p[s], F.p[s ^ 1], F.f, F.mask, F.w, F.h, F.hh, a.alpha,
+                                 u.nstrips, u.R, u.unit);
+}
+
+// MODE_RR: last pre-sweep + residual + restriction (coarse rhs, zero coarse
+// p[ib]); MODE_PC: prolongation of the coarse p[ib] + correction + first
+// post-sweep.  Both read p[ia] and write p[ia^1].
+template <int MODE>
+__global__ void __launch_bounds__(TPB) kb_stream_pass(StageArgs a) {
+  StreamUnit u;
+  if (!stream_unit(a, u)) return;
+  if (!dom_active(a.dom[u.d])) return;
+  const BLev &F = a.lev[u.d * BMAXL + a.level];
+  const BLev &C = a.lev[u.d * BMAXL + a.level + 1];
+  const int s = a.ia[u.d];
+  StreamArgs A{};
+  A.src = F.p[s];
+  A.dst = F.p[s ^ 1];
+  A.f = F.f;
+  A.m = F.mask;
+  A.w = F.w;
+  A.h = F.h;
+  A.hh = F.hh;
+  A.alpha = a.alpha;
+  A.ihsq = F.ihsq;
+  A.crc = C.f;
+  A.cec = C.p[a.ib[u.d]];
+  A.cpin = C.p[a.ib[u.d]];
+  A.cmask = C.mask;
+  A.wc = C.w;
+  A.hc = C.h;
+  A.nstrips = u.nstrips;
+  A.R = u.R;
+  if ((F.w & 1) == 0) mg_stream_unit<1, MODE, false, true>(A, u.unit);
+  else mg_stream_unit<1, MODE, false, false>(A, u.unit);
 }
 
 // the coarse levels of each domain in LDS, one workgroup per domain slot;
@@ -902,6 +976,24 @@ int BatchEngine::build_tables() {
 // Per stage kind and level: the prefix of tile counts over the member slots
 // and each domain's tiles per row; kernels locate their tile from these, so
 // a member change rebuilds O(domains x levels) integers, not tile lists.
+// Row-streamed levels: at least kStreamMinW wide and kStreamMinCells large,
+// with both smoothing counts >= 1 (the grid-transfer passes carry one sweep
+// each) and at most 3 (the tiled mode); row blocks of kStreamR rows; used at
+// a level only when the members' sweep units there reach kStreamMinUnits.
+// Measured on gfx950 (256 uneven batched domains and the served mix): below
+// about 4096 units a level is faster as LDS tiles; above, streaming its
+// >= 512-wide, >= 2^18-cell domains gained ~3% on the served mix and was
+// neutral on the batch.
+constexpr int kStreamMinW = 512;
+constexpr size_t kStreamMinCells = size_t(1) << 18;
+constexpr long kStreamMinUnits = 4096;
+constexpr int kStreamR = 16;
+
+bool BatchEngine::stream_eligible(const BGeom &G, int l) const {
+  return l < G.cfirst && tiled() && prm_.nu_pre >= 1 && prm_.nu_post >= 1 &&
+         G.w[l] >= kStreamMinW && (size_t)G.w[l] * G.h[l] >= kStreamMinCells;
+}
+
 int BatchEngine::build_tiles() {
   h_off_.clear();
   // per span: tile prefix over the domain slots [nd + 1], tiles per row [nd]
@@ -926,6 +1018,14 @@ int BatchEngine::build_tiles() {
   auto cdiv = [](int a, int b) { return a > 0 ? (a + b - 1) / b : 0; };
   const int tx = tune_.tx, ty = tune_.ty;
   const int nr = (int)run_.size();
+  for (int l = 0; l < BMAXL; l++) {
+    long units = 0;
+    for (int s = 0; s < nr; s++) {
+      const BGeom &G = *run_[s]->g;
+      if (stream_eligible(G, l)) units += (long)cdiv(G.w[l], SW2) * cdiv(G.h[l] - 2, kStreamR);
+    }
+    stream_on_[l] = units >= kStreamMinUnits;
+  }
   for (int l = 0; l < BMAXL; l++) {
     span(sw_[l], nr, [&](int s, int &nx, int &ny, int &) {
       const BGeom &G = *run_[s]->g;
@@ -956,7 +1056,7 @@ int BatchEngine::build_tiles() {
       size_t cells = 0;
       for (int s = 0; s < nr; s++) {
         const BGeom &G = *run_[s]->g;
-        if (l < G.cfirst) cells += (size_t)G.w[l] * G.h[l];
+        if (l < G.cfirst && !streamed(G, l)) cells += (size_t)G.w[l] * G.h[l];
       }
       const int tpb = cells >= kTileBigCells ? 512 : 1024;
       int oty = tune_.prow;
@@ -965,7 +1065,7 @@ int BatchEngine::build_tiles() {
           long t = 0;
           for (int s = 0; s < nr; s++) {
             const BGeom &G = *run_[s]->g;
-            if (l >= G.cfirst) continue;
+            if (l >= G.cfirst || streamed(G, l)) continue;
             int ox, nx;
             tile_cols(G.w[l], nu, ox, nx);
             t += (long)nx * cdiv(G.h[l], r);
@@ -979,7 +1079,7 @@ int BatchEngine::build_tiles() {
       out[l].tpb = tpb;
       span(out[l], nr, [&](int s, int &nx, int &ny, int &ox) {
         const BGeom &G = *run_[s]->g;
-        if (l >= G.cfirst) return;
+        if (l >= G.cfirst || streamed(G, l)) return;
         tile_cols(G.w[l], nu, ox, nx);
         ny = cdiv(G.h[l], oty);
       });
@@ -987,6 +1087,25 @@ int BatchEngine::build_tiles() {
   };
   fused(tp_, prm_.nu_pre);
   fused(tq_, prm_.nu_post);
+  // row-streamed levels: units = strips x row blocks (the table's third row
+  // holds the row-block height)
+  constexpr int own_pass = 2 * (64 - 2 * (1 + 2)); // columns per strip, one-sweep passes
+  for (int l = 0; l < BMAXL; l++) {
+    span(ss_[l], nr, [&](int s, int &nx, int &ny, int &r) {
+      const BGeom &G = *run_[s]->g;
+      if (!streamed(G, l)) return;
+      nx = cdiv(G.w[l], SW2);
+      ny = cdiv(G.h[l] - 2, kStreamR);
+      r = kStreamR;
+    });
+    span(st_[l], nr, [&](int s, int &nx, int &ny, int &r) {
+      const BGeom &G = *run_[s]->g;
+      if (!streamed(G, l)) return;
+      nx = cdiv(G.w[l], own_pass);
+      ny = cdiv(G.h[l], kStreamR);
+      r = kStreamR;
+    });
+  }
   span(upd_, (int)all_.size(), [&](int s, int &nx, int &ny, int &) {
     nx = all_[s]->gx;
     ny = all_[s]->gy;
@@ -1003,7 +1122,7 @@ int BatchEngine::build_tiles() {
   return UBG_OK;
 }
 
-enum { ST_SWEEP, ST_RR, ST_PC, ST_COARSE, ST_UPDATE, ST_NORM, ST_TPRE, ST_TPOST };
+enum { ST_SWEEP, ST_RR, ST_PC, ST_COARSE, ST_UPDATE, ST_NORM, ST_TPRE, ST_TPOST, ST_SSWEEP, ST_SRR, ST_SPC };
 
 // The stage sequence of nsteps refinement steps and, per stage, the two
 // per-domain index rows, simulated from (and advancing) the domains' host
@@ -1022,31 +1141,49 @@ void BatchEngine::gen_steps(int nsteps, std::vector<StageRec> &st, std::vector<u
   for (int step = 0; step < nsteps; step++) {
     const bool with_fresh = step == 0 && (all_fresh || !fresh_.empty());
     for (int v = 0; v < k_ && nr && tiled(); v++) {
-      // fused level passes: one pre and one post per level, each flipping
-      // the level's p once
-      for (int l = 0; l < lmax_; l++) {
-        if (!tp_[l].n) continue;
-        auto r = row(ST_TPRE, l);
+      // per level: a streamed domain takes nu_pre - 1 sweeps and the fused
+      // pre pass (each flipping the level's p), a tiled one the fused tile
+      // pre pass (one flip); mirrored on the way up
+      auto tile_at = [&](size_t d, int l) {
+        return l < run_[d]->g->cfirst && !streamed(*run_[d]->g, l);
+      };
+      auto stream_at = [&](size_t d, int l) { return streamed(*run_[d]->g, l); };
+      auto sweep_rows = [&](int l, int n) {
+        for (int k = 0; k < n; k++) {
+          auto r = row(ST_SSWEEP, l);
+          for (size_t d = 0; d < nr; d++)
+            if (stream_at(d, l)) {
+              r.first[d] = (uint8_t)run_[d]->cur[l];
+              run_[d]->cur[l] ^= 1;
+            }
+        }
+      };
+      auto pass_row = [&](int kind, int l, auto at) {
+        auto r = row(kind, l);
         for (size_t d = 0; d < nr; d++)
-          if (l < run_[d]->g->cfirst) {
+          if (at(d, l)) {
             r.first[d] = (uint8_t)run_[d]->cur[l];
             r.second[d] = (uint8_t)run_[d]->cur[l + 1];
             run_[d]->cur[l] ^= 1;
           }
+      };
+      for (int l = 0; l < lmax_; l++) {
+        if (st_[l].n) {
+          sweep_rows(l, prm_.nu_pre - 1);
+          pass_row(ST_SRR, l, stream_at);
+        }
+        if (tp_[l].n) pass_row(ST_TPRE, l, tile_at);
       }
       {
         auto r = row(ST_COARSE, 0);
         for (size_t d = 0; d < nr; d++) r.first[d] = (uint8_t)run_[d]->cur[run_[d]->g->cfirst];
       }
       for (int l = lmax_ - 1; l >= 0; l--) {
-        if (!tq_[l].n) continue;
-        auto r = row(ST_TPOST, l);
-        for (size_t d = 0; d < nr; d++)
-          if (l < run_[d]->g->cfirst) {
-            r.first[d] = (uint8_t)run_[d]->cur[l];
-            r.second[d] = (uint8_t)run_[d]->cur[l + 1];
-            run_[d]->cur[l] ^= 1;
-          }
+        if (st_[l].n) {
+          pass_row(ST_SPC, l, stream_at);
+          sweep_rows(l, prm_.nu_post - 1);
+        }
+        if (tq_[l].n) pass_row(ST_TPOST, l, tile_at);
       }
     }
     for (int v = 0; v < k_ && nr && !tiled(); v++) {
@@ -1153,6 +1290,21 @@ int BatchEngine::launch_stages(const std::vector<StageRec> &st, const uint8_t *d
       a.oty = tq_[r.level].oty;
       if (a.ntiles)
         launch_tile_pass(false, prm_.nu_post, tq_[r.level].tpb, grid(a.ntiles), stream_, a);
+      break;
+    case ST_SSWEEP:
+      use(a, ss_[r.level]);
+      if (a.ntiles)
+        hipLaunchKernelGGL(kb_stream_sweep, dim3((a.ntiles + TPB / 64 - 1) / (TPB / 64)),
+                           dim3(TPB), 0, stream_, a);
+      break;
+    case ST_SRR:
+    case ST_SPC:
+      use(a, st_[r.level]);
+      if (a.ntiles) {
+        const dim3 g((a.ntiles + TPB / 64 - 1) / (TPB / 64));
+        if (r.kind == ST_SRR) hipLaunchKernelGGL(kb_stream_pass<MODE_RR>, g, dim3(TPB), 0, stream_, a);
+        else hipLaunchKernelGGL(kb_stream_pass<MODE_PC>, g, dim3(TPB), 0, stream_, a);
+      }
       break;
     case ST_COARSE:
       if (!run_.empty())

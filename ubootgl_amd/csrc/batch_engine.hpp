@@ -198,8 +198,18 @@ private:
   };
   Span sw_[BMAXL], rr_[BMAXL], pc_[BMAXL], upd_;
   Span tp_[BMAXL], tq_[BMAXL]; // fused level passes (pre, post)
+  Span ss_[BMAXL], st_[BMAXL]; // row-streamed levels: sweeps, pre / post passes
+  bool stream_on_[BMAXL] = {};  // level l has enough streamed units over the members
   // levels run as fused tile passes (smoothing counts with a tiled pass)
   bool tiled() const { return tile_pass_ok(prm_.nu_pre) && tile_pass_ok(prm_.nu_post); }
+  // level l of a domain of this shape may run as row-streamed passes (wide
+  // and large levels) ...
+  bool stream_eligible(const BGeom &G, int l) const;
+  // ... and does, when the member domains' streamed units at that level are
+  // enough to fill the GPU (else the LDS tiles cost less per cell)
+  bool streamed(const BGeom &G, int l) const {
+    return l < BMAXL && stream_on_[l] && stream_eligible(G, l);
+  }
   int launch_passes_only(int reps);
   // stage index rows
   uint8_t *d_idx_ = nullptr;
