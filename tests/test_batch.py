@@ -83,13 +83,22 @@ def test_batch_equals_single_domain(gpu):
     bt.close()
 
 
+def stream_units(w, h):
+    """Units (124-column strips x 16-row blocks) of a level's stand-alone
+    sweep; a batch streams a level when those of its eligible domains (>= 512
+    wide, >= 2^18 cells) sum to at least 4096."""
+    return -(-w // 124) * -(-(h - 2) // 16)
+
+
 @pytest.mark.gpu
 def test_batch_streamed_levels_equal_single(gpu):
-    """Levels at least 512 wide with at least 2^18 cells run as row-streamed
-    passes: an odd width (unaligned loads), two streamed levels in one
-    domain, and a small tiled domain beside them -- each bit-identical to the
-    single-domain solve."""
-    probs = problems([(1023, 300), (2048, 512), (100, 60), (1024, 256)], seed=33)
+    """Level 0 runs as row-streamed passes here (an 8192 x 1024 domain lifts
+    the level's units over the gate): an odd width (unaligned loads) and
+    mid-size domains stream beside it, a small domain tiles -- each domain
+    bit-identical to the single-domain solve."""
+    sizes = [(8192, 1024), (1023, 300), (2048, 512), (1024, 256), (100, 60)]
+    assert sum(stream_units(w, h) for w, h in sizes[:4]) >= 4096
+    probs = problems(sizes, seed=33)
     bt = B.Batch([(P.W, P.H) for P in probs])
     for d, P in enumerate(probs):
         bt.set_problem(d, P)
@@ -98,6 +107,25 @@ def test_batch_streamed_levels_equal_single(gpu):
         rs, xs = single(P, 1e-6)
         assert bt.result(d) == rs, f"domain {d} {P.W}x{P.H}"
         assert np.array_equal(bt.solution(d), xs), f"domain {d} {P.W}x{P.H}"
+    bt.close()
+
+
+@pytest.mark.gpu
+def test_batch_streamed_short_levels_equal_single(gpu):
+    """Sixteen 8192 x 64 domains: their 64-row level 0 (four row blocks per
+    strip; shorter wide domains have no coarse level that fits in LDS) runs
+    streamed over the batch, while the single-domain solve tiles it; the
+    results are bit-identical."""
+    assert 16 * stream_units(8192, 64) >= 4096
+    probs = problems([(8192, 64)] * 16, seed=41)
+    bt = B.Batch([(P.W, P.H) for P in probs])
+    for d, P in enumerate(probs):
+        bt.set_problem(d, P)
+    bt.solve(tol=1e-6)
+    for d in (0, 7, 15):
+        rs, xs = single(probs[d], 1e-6)
+        assert bt.result(d) == rs, f"domain {d}"
+        assert np.array_equal(bt.solution(d), xs), f"domain {d}"
     bt.close()
 
 
