@@ -763,7 +763,7 @@ void BatchEngine::geom_release(Arena &arena, BGeom *g) {
   delete g;
 }
 
-int BatchEngine::domain_create(Arena &arena, BGeom *g, float h, BDomain **out) {
+int BatchEngine::domain_create(Arena &arena, BGeom *g, float h, BDomain **out, hipStream_t s) {
   BShape sh;
   UBG_TRY(batch_shape(g->W, g->H, &sh));
   auto *d = new BDomain();
@@ -801,7 +801,7 @@ int BatchEngine::domain_create(Arena &arena, BGeom *g, float h, BDomain **out) {
   // h of each level: h_c = h * (W_l - 1) / (W_{l+1} - 1) (pressure_solver.cpp:229)
   d->hh[0] = h;
   for (int l = 1; l < g->nlev; l++) d->hh[l] = d->hh[l - 1] * (g->w[l - 1] - 1.0f) / (g->w[l] - 1.0f);
-  UBG_HIP(hipMemsetAsync(d->mem, 0, d->bytes, stream_));
+  UBG_HIP(hipMemsetAsync(d->mem, 0, d->bytes, s ? s : stream_));
   g->refs++;
   *out = d;
   return UBG_OK;
@@ -837,31 +837,34 @@ int BatchEngine::domain_set_guess(BDomain *d, const double *x) {
   return UBG_OK;
 }
 
-int BatchEngine::domain_start(BDomain *d, double tol, int max_iter, double ref) {
-  DState s{};
-  s.active = 1;
-  s.tol = tol;
-  s.ref = ref;
-  s.max_iter = max_iter;
-  s.k = k_;
-  s.best = 1.0;
-  s.rel = 1.0;
-  s.xcur = d->xcur;
-  UBG_HIP(hipMemcpyAsync(d->st, &s, sizeof(s), hipMemcpyHostToDevice, stream_));
-  UBG_HIP(hipStreamSynchronize(stream_));
+int BatchEngine::domain_start(BDomain *d, double tol, int max_iter, double ref, hipStream_t s) {
+  if (!s) s = stream_;
+  DState st{};
+  st.active = 1;
+  st.tol = tol;
+  st.ref = ref;
+  st.max_iter = max_iter;
+  st.k = k_;
+  st.best = 1.0;
+  st.rel = 1.0;
+  st.xcur = d->xcur;
+  UBG_HIP(hipMemcpyAsync(d->st, &st, sizeof(st), hipMemcpyHostToDevice, s));
+  UBG_HIP(hipStreamSynchronize(s));
   return UBG_OK;
 }
 
-int BatchEngine::domain_read_state(BDomain *d, DState *out) {
-  UBG_HIP(hipMemcpyAsync(out, d->st, sizeof(DState), hipMemcpyDeviceToHost, stream_));
-  UBG_HIP(hipStreamSynchronize(stream_));
+int BatchEngine::domain_read_state(BDomain *d, DState *out, hipStream_t s) {
+  if (!s) s = stream_;
+  UBG_HIP(hipMemcpyAsync(out, d->st, sizeof(DState), hipMemcpyDeviceToHost, s));
+  UBG_HIP(hipStreamSynchronize(s));
   return UBG_OK;
 }
 
-int BatchEngine::domain_get_solution(BDomain *d, double *x) {
+int BatchEngine::domain_get_solution(BDomain *d, double *x, hipStream_t s) {
+  if (!s) s = stream_;
   const size_t n = (size_t)d->g->W * d->g->H;
-  UBG_HIP(hipMemcpyAsync(x, d->x[d->xcur], n * 8, hipMemcpyDeviceToHost, stream_));
-  UBG_HIP(hipStreamSynchronize(stream_));
+  UBG_HIP(hipMemcpyAsync(x, d->x[d->xcur], n * 8, hipMemcpyDeviceToHost, s));
+  UBG_HIP(hipStreamSynchronize(s));
   return UBG_OK;
 }
 

@@ -126,15 +126,19 @@ public:
   // geometry / domain lifetime (memory from `arena`)
   int geom_create(Arena &arena, int W, int H, const float *flag_host, BGeom **out);
   void geom_release(Arena &arena, BGeom *g);
-  int domain_create(Arena &arena, BGeom *g, float h, BDomain **out);
+  // The domain calls take an optional stream (default: the engine's own).
+  // A caller that keeps a step in flight on the engine's stream may use
+  // another stream for domains that are not members of that step.
+  int domain_create(Arena &arena, BGeom *g, float h, BDomain **out, hipStream_t s = nullptr);
   void domain_release(Arena &arena, BDomain *d);
   int domain_set_rhs(BDomain *d, const float *rhs_host);
   int domain_set_guess(BDomain *d, const double *x_host);
   // reset a domain's solve state: active, tol/max_iter, fresh (next update
   // computes r0 and does not add e)
-  int domain_start(BDomain *d, double tol, int max_iter, double ref = 0.0);
-  int domain_read_state(BDomain *d, DState *out);
-  int domain_get_solution(BDomain *d, double *x_host);
+  int domain_start(BDomain *d, double tol, int max_iter, double ref = 0.0,
+                   hipStream_t s = nullptr);
+  int domain_read_state(BDomain *d, DState *out, hipStream_t s = nullptr);
+  int domain_get_solution(BDomain *d, double *x_host, hipStream_t s = nullptr);
 
   // the member set of the following steps: `running` domains take V-cycles
   // and an accumulating update; `fresh` ones only their first update

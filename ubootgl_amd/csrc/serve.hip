@@ -1,12 +1,16 @@
 // serve.hip -- served refinement solves (ubg_server_*): a request loop with
 // continuous batching on top of the batch engine (batch.hip).
 //
-// One worker thread owns the engine.  Each turn it admits queued requests in
-// arrival order while the device arena and the active limit allow, runs one
-// refinement step of every admitted request together (V-cycles of the
-// running ones, first updates of the newly admitted ones), then retires the
-// timesteps that finished: the next timestep of a request starts warm from
-// its solution, a request whose last timestep finished releases its memory.
+// One worker thread owns the engine.  Each turn it waits for the refinement
+// step in flight, enqueues the next one (the members still iterating, plus
+// the requests made ready during the last turn, which take their first
+// update), and while that step runs on the device it retires the members
+// that finished (the next timestep of a request starts warm from its
+// solution, a request whose last timestep finished copies its solution out
+// and releases its memory) and admits queued requests in arrival order while
+// the device arena and the active limit allow.  Those transfers use a second
+// stream and touch only domains outside the step in flight, so the host work
+// overlaps the device work; a started timestep joins the step after next.
 // No counterpart in the reference.
 #include "batch_engine.hpp"
 
@@ -53,7 +57,11 @@ struct ubg_server {
   std::condition_variable cv;
   std::deque<SrvReq *> queue;
   std::unordered_map<uint64_t, std::unique_ptr<SrvReq>> reqs;
-  std::vector<SrvReq *> running, fresh; // worker only
+  hipStream_t cstream = nullptr; // transfers of domains outside the step in flight
+  // worker only: members of the step in flight (engine order: running, then
+  // fresh), requests ready to join the next step, still-iterating requests
+  std::vector<SrvReq *> members, ready, running;
+  bool inflight = false;
   std::thread worker;
   bool stop = false;
   uint64_t next_id = 1;
@@ -81,17 +89,18 @@ void finish(ubg_server *S, SrvReq *r, int state) {
   S->cv.notify_all();
 }
 
-// start timestep r->t of a running request: rhs upload, warm start
+// start timestep r->t of a request in no step in flight: rhs upload and a
+// warm start from its solution, on the copy stream (domain_start syncs it)
 int start_timestep(ubg_server *S, SrvReq *r) {
   const SrvGeom &G = S->geoms[r->geom];
   UBG_HIP(hipMemcpyAsync(r->dom->rhs, r->rhs[r->t], (size_t)G.W * G.H * 4, hipMemcpyHostToDevice,
-                         S->stream));
-  return S->eng.domain_start(r->dom, S->prm.tol, S->prm.max_iter, r->fnorm[r->t]);
+                         S->cstream));
+  return S->eng.domain_start(r->dom, S->prm.tol, S->prm.max_iter, r->fnorm[r->t], S->cstream);
 }
 
 // one worker turn; returns false when there is nothing left to do and stop is set
 bool worker_turn(ubg_server *S) {
-  if (S->running.empty() && S->fresh.empty()) { // idle: wait for a request
+  if (!S->inflight && S->running.empty() && S->ready.empty()) { // idle: wait for a request
     std::unique_lock<std::mutex> lk(S->mu);
     if (S->queue.empty() && S->stop) return false;
     if (S->queue.empty()) {
@@ -101,18 +110,106 @@ bool worker_turn(ubg_server *S) {
   }
   std::lock_guard<std::mutex> elk(S->eng_mu);
   const double t_turn = S->now();
-  // admission, in arrival order
+  // 1. the step in flight: wait for it, then split its members into those
+  // still iterating and those whose timestep ended
+  std::vector<SrvReq *> finished;
+  if (S->inflight) {
+    S->inflight = false;
+    const int rc = S->eng.sync();
+    std::vector<SrvReq *> members;
+    members.swap(S->members);
+    S->running.clear();
+    if (rc != UBG_OK) {
+      for (SrvReq *r : members) finish(S, r, 3);
+      return true;
+    }
+    std::vector<ubg::BDomain *> act = S->eng.active_members();
+    size_t ai = 0;
+    for (SrvReq *r : members) {
+      if (ai < act.size() && act[ai] == r->dom) { // still iterating
+        ai++;
+        S->running.push_back(r);
+      } else {
+        finished.push_back(r);
+      }
+    }
+  }
+  const double t_synced = S->now();
+  // 2. the next step: the running members, and the ready ones with their
+  // first update (one step with them as fresh, the rest of the turn with
+  // everyone running; a domain whose first update already met the tolerance
+  // is inactive and skipped on the device)
+  std::vector<SrvReq *> fresh;
+  fresh.swap(S->ready);
+  if (!S->running.empty() || !fresh.empty()) {
+    std::vector<ubg::BDomain *> rd, fd;
+    for (SrvReq *r : S->running) rd.push_back(r->dom);
+    for (SrvReq *r : fresh) fd.push_back(r->dom);
+    int rc = S->eng.set_members(rd, fd);
+    if (rc == UBG_OK) rc = S->eng.enqueue_steps(1);
+    if (rc == UBG_OK && S->prm.steps_per_turn > 1) {
+      std::vector<ubg::BDomain *> all = rd;
+      all.insert(all.end(), fd.begin(), fd.end());
+      rc = S->eng.set_members(all, {});
+      if (rc == UBG_OK) rc = S->eng.enqueue_steps(S->prm.steps_per_turn - 1);
+    }
+    std::vector<SrvReq *> members = S->running;
+    members.insert(members.end(), fresh.begin(), fresh.end());
+    S->running.clear();
+    if (rc != UBG_OK) {
+      (void)S->eng.sync();
+      for (SrvReq *r : members) finish(S, r, 3);
+    } else {
+      S->members = members;
+      S->inflight = true;
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->st.steps++;
+    }
+  }
+  const double t_launched = S->now();
+  // 3. while the step runs: retire the finished members of the last one ...
+  for (SrvReq *r : finished) {
+    ubg::DState ds;
+    if (S->eng.domain_read_state(r->dom, &ds, S->cstream) != UBG_OK || ds.nonfinite) {
+      finish(S, r, 3);
+      continue;
+    }
+    r->dom->xcur = ds.xcur;
+    r->iters += ds.iters;
+    r->rel = ds.rel;
+    r->conv = r->conv && ds.converged;
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->st.solves++;
+    }
+    if (r->t + 1 < r->T) {
+      r->t++;
+      if (start_timestep(S, r) != UBG_OK) {
+        finish(S, r, 3);
+        continue;
+      }
+      S->ready.push_back(r);
+      continue;
+    }
+    if (r->x_out && S->eng.domain_get_solution(r->dom, r->x_out, S->cstream) != UBG_OK) {
+      finish(S, r, 3);
+      continue;
+    }
+    finish(S, r, 2);
+  }
+  const double t_retired = S->now();
+  // ... and admit queued requests, in arrival order
   std::vector<SrvReq *> admitted;
   {
     std::unique_lock<std::mutex> lk(S->mu);
     while (!S->queue.empty() &&
-           (int)(S->running.size() + S->fresh.size() + admitted.size()) < S->prm.max_active) {
+           (int)(S->members.size() + S->ready.size() + admitted.size()) < S->prm.max_active) {
       SrvReq *r = S->queue.front();
       const SrvGeom &G = S->geoms[r->geom];
       ubg::BDomain *d = nullptr;
-      int rc = S->eng.domain_create(S->arena, G.g, G.h, &d);
+      int rc = S->eng.domain_create(S->arena, G.g, G.h, &d, S->cstream);
       if (rc != UBG_OK) {
-        if (rc == UBG_E_BUDGET && !(S->running.empty() && S->fresh.empty() && admitted.empty()))
+        if (rc == UBG_E_BUDGET && !(S->members.empty() && S->ready.empty() && admitted.empty()))
           break; // wait for memory to free up
         S->queue.pop_front();
         r->state = 3;
@@ -136,82 +233,16 @@ bool worker_turn(ubg_server *S) {
       finish(S, r, 3);
       continue;
     }
-    S->fresh.push_back(r);
+    S->ready.push_back(r);
   }
-  if (S->running.empty() && S->fresh.empty()) return true;
-  const double t_step = S->now();
-  std::vector<ubg::BDomain *> rd, fd;
-  for (SrvReq *r : S->running) rd.push_back(r->dom);
-  for (SrvReq *r : S->fresh) fd.push_back(r->dom);
-  // one step with the fresh members (their first update), then the rest of
-  // the turn with everyone as running: a domain whose first update already
-  // met the tolerance is inactive and skipped on the device
-  int rc = S->eng.set_members(rd, fd);
-  if (rc == UBG_OK) rc = S->eng.enqueue_steps(1);
-  if (rc == UBG_OK && S->prm.steps_per_turn > 1) {
-    std::vector<ubg::BDomain *> all = rd;
-    all.insert(all.end(), fd.begin(), fd.end());
-    rc = S->eng.set_members(all, {});
-    if (rc == UBG_OK) rc = S->eng.enqueue_steps(S->prm.steps_per_turn - 1);
-  }
-  const double t_launched = S->now();
-  if (rc == UBG_OK) rc = S->eng.sync();
-  const double t_synced = S->now();
-  std::vector<SrvReq *> members = S->running;
-  members.insert(members.end(), S->fresh.begin(), S->fresh.end());
-  S->running.clear();
-  S->fresh.clear();
-  if (rc != UBG_OK) {
-    for (SrvReq *r : members) finish(S, r, 3);
-    return true;
-  }
-  std::vector<ubg::BDomain *> act = S->eng.active_members();
-  std::vector<SrvReq *> next_fresh;
-  size_t ai = 0;
-  for (SrvReq *r : members) {
-    if (ai < act.size() && act[ai] == r->dom) { // still iterating
-      ai++;
-      S->running.push_back(r);
-      continue;
-    }
-    ubg::DState ds;
-    if (S->eng.domain_read_state(r->dom, &ds) != UBG_OK || ds.nonfinite) {
-      finish(S, r, 3);
-      continue;
-    }
-    r->dom->xcur = ds.xcur;
-    r->iters += ds.iters;
-    r->rel = ds.rel;
-    r->conv = r->conv && ds.converged;
-    {
-      std::lock_guard<std::mutex> lk(S->mu);
-      S->st.solves++;
-    }
-    if (r->t + 1 < r->T) {
-      r->t++;
-      if (start_timestep(S, r) != UBG_OK) {
-        finish(S, r, 3);
-        continue;
-      }
-      next_fresh.push_back(r);
-      continue;
-    }
-    if (r->x_out && S->eng.domain_get_solution(r->dom, r->x_out) != UBG_OK) {
-      finish(S, r, 3);
-      continue;
-    }
-    finish(S, r, 2);
-  }
-  S->fresh = next_fresh;
   std::lock_guard<std::mutex> lk(S->mu);
-  S->st.steps++;
   const double t_end = S->now();
-  S->st.busy_s += t_end - t_step;
-  S->st.setup_s += t_launched - t_step;
-  S->st.wait_s += t_synced - t_launched;
-  S->st.retire_s += t_end - t_synced;
-  S->st.admit_s += t_step - t_turn;
-  S->st.active = (int)(S->running.size() + S->fresh.size());
+  S->st.busy_s += t_end - t_turn;
+  S->st.wait_s += t_synced - t_turn;
+  S->st.setup_s += t_launched - t_synced;
+  S->st.retire_s += t_retired - t_launched;
+  S->st.admit_s += t_end - t_retired;
+  S->st.active = (int)(S->members.size() + S->ready.size());
   S->st.peak_active = std::max(S->st.peak_active, S->st.active);
   return true;
 }
@@ -257,6 +288,10 @@ UBG_API int ubg_server_create(const ubg_server_params *prm, const ubg_params *mg
   auto *S = new ubg_server();
   S->prm = P;
   hipError_t e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) {
+    e = hipStreamCreateWithFlags(&S->cstream, hipStreamNonBlocking);
+    if (e != hipSuccess) (void)hipStreamDestroy(S->stream);
+  }
   if (e != hipSuccess) {
     delete S;
     return fail(UBG_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
@@ -266,6 +301,7 @@ UBG_API int ubg_server_create(const ubg_server_params *prm, const ubg_params *mg
   if (rc != UBG_OK) {
     S->arena.release();
     (void)hipStreamDestroy(S->stream);
+    (void)hipStreamDestroy(S->cstream);
     delete S;
     return rc;
   }
@@ -289,12 +325,14 @@ UBG_API int ubg_server_destroy(ubg_server *S) {
   }
   if (S->worker.joinable()) S->worker.join();
   (void)hipStreamSynchronize(S->stream);
+  (void)hipStreamSynchronize(S->cstream);
   for (SrvGeom &g : S->geoms) S->eng.geom_release(S->arena, g.g);
   S->geoms.clear();
   S->arena.release();
-  hipStream_t s = S->stream;
+  hipStream_t s = S->stream, cs = S->cstream;
   delete S;
   (void)hipStreamDestroy(s);
+  (void)hipStreamDestroy(cs);
   return UBG_OK;
 }
 
