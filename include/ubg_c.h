@@ -238,9 +238,10 @@ UBG_API int ubg_batch_get_tuning(ubg_batch *b, ubg_batch_tuning *t);
  * device until the last one is done.  Requests are admitted in arrival order
  * while the device arena (hbm_budget bytes) and max_active allow, and every
  * refinement step advances all admitted requests together (continuous
- * batching: a request arriving while others run joins at the next step).  A
- * request's results do not depend on what else is in flight (bit-identical
- * to running it alone). */
+ * batching: a request arriving while others run joins at the step after
+ * next; the worker admits and retires while the current step runs on the
+ * device).  A request's results do not depend on what else is in flight
+ * (bit-identical to running it alone). */
 typedef struct ubg_server ubg_server;
 typedef struct ubg_server_params {
   size_t hbm_budget;    /* device bytes for geometries + resident requests  */
@@ -265,11 +266,12 @@ typedef struct ubg_server_stats {
   long submitted, completed, failed, solves, steps;
   int active, queued, peak_active;
   size_t bytes_in_use, peak_bytes;
-  double busy_s;        /* worker time spent in steps                      */
-  double setup_s;       /* of which: member tables, schedule, launches     */
-  double wait_s;        /*           waiting for the device                */
-  double retire_s;      /*           reading states, next rhs, results     */
-  double admit_s;       /* worker time admitting requests (not in busy_s)  */
+  double busy_s;        /* worker time in turns (sum of the four below)    */
+  double setup_s;       /* member tables, schedule, launches of a step     */
+  double wait_s;        /* waiting for the step in flight                  */
+  double retire_s;      /* reading states, next rhs, results (while the
+                           next step runs)                                 */
+  double admit_s;       /* admitting requests (while the next step runs)   */
 } ubg_server_stats;
 UBG_API void ubg_server_params_default(ubg_server_params *out);
 UBG_API int ubg_server_create(const ubg_server_params *prm, const ubg_params *mg,
