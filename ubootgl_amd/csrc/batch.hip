@@ -662,14 +662,29 @@ int batch_shape(int W, int H, BShape *s) {
     s->w[l] = ws[l];
     s->h[l] = hs[l];
   }
+  // the LDS V-cycle starts where the single-domain solver starts it (the
+  // first level of at most kCoarseMaxCells cells: larger LDS levels cost more
+  // than their tile passes), or at the first level whose sub-hierarchy fits
+  // if that one does not
   s->cfirst = -1;
+  int fit = -1;
+  size_t fit_bytes = 0;
   for (int l = 1; l <= s->nlev - 2; l++) {
     size_t b = coarse_lds_layout(ws.data(), hs.data(), l, s->nlev, nullptr);
-    if (b <= LDS_BUDGET) {
+    if (b > LDS_BUDGET) continue;
+    if (fit < 0) {
+      fit = l;
+      fit_bytes = b;
+    }
+    if ((size_t)ws[l] * hs[l] <= kCoarseMaxCells) {
       s->cfirst = l;
       s->lds_bytes = b;
       break;
     }
+  }
+  if (s->cfirst < 0 && fit >= 0) {
+    s->cfirst = fit;
+    s->lds_bytes = fit_bytes;
   }
   if (s->cfirst < 0) return fail(UBG_E_INVALID, "no coarse level fits in LDS");
   for (int l = 0; l < s->nlev; l++) s->geom_bytes += 2 * al((size_t)ws[l] * hs[l]);
