@@ -352,6 +352,33 @@ def test_tracers(gpu):
     np.testing.assert_array_equal(gage, rage)
 
 
+@pytest.mark.parametrize("use_graph", [1, 0])
+def test_run_tracers_equal_step_by_step(gpu, use_graph):
+    """Simulation.run (steps with the tracer advection after each, seeds
+    continuing across calls) equals step() + advect_tracers() with seeds 1,
+    2, ... -- fields and tracers, graph-replayed and eager."""
+    d = workloads.small_domain(96, 48, seed=19, ndisks=5)
+    prm = U.Params(use_graph=use_graph)
+    a, _ = make_pair(d, params=prm)
+    b, _ = make_pair(d, params=prm)
+    xy = workloads.tracers_in_fluid(d, 3000, seed=20)
+    age = np.linspace(0, 6.5, xy.shape[0]).astype(np.float32)
+    a.set_tracers(xy, age)
+    b.set_tracers(xy, age)
+    dt = float(d.dt)
+    a.run(dt, 6)
+    a.run(dt, 4)  # a second run continues the seeds
+    for k in range(10):
+        b.step(dt)
+        b.advect_tracers(dt, 1 + k)
+    for name in ("vx", "vy", "p", "vx_back", "vy_back", "vx_current", "vy_current"):
+        np.testing.assert_array_equal(a.get(name), b.get(name), err_msg=name)
+    axy, aage = a.get_tracers()
+    bxy, bage = b.get_tracers()
+    np.testing.assert_array_equal(axy, bxy)
+    np.testing.assert_array_equal(aage, bage)
+
+
 def test_error_codes(gpu):
     with pytest.raises(U.UbgError) as e:
         U.MG(7, 100)
