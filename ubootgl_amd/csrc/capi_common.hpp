@@ -47,18 +47,15 @@ inline int check_launch() {
 
 inline size_t align_up(size_t n, size_t a = 256) { return (n + a - 1) / a * a; }
 
-#ifndef UBG_FUSE_MIN_LOG2
-#define UBG_FUSE_MIN_LOG2 22
-#endif
-#ifndef UBG_COARSE_MAX_CELLS
-#define UBG_COARSE_MAX_CELLS 4096
-#endif
-// levels with at least this many cells run the fused two-pass level kernels
-constexpr size_t kFuseMinCells = (size_t)1 << UBG_FUSE_MIN_LOG2;
+// levels with at least this many cells run the row-streamed level kernels
+// (below, a level has too few strip x row-block units to fill the chip and
+// the LDS tiles are faster)
+constexpr size_t kFuseMinCells = (size_t)1 << 22;
 // the single-workgroup LDS V-cycle starts at the first level with at most
 // this many cells (one CU's lanes do all of its work, so larger levels run
-// faster as tiles spread over the chip)
-constexpr size_t kCoarseMaxCells = UBG_COARSE_MAX_CELLS;
+// faster as tiles spread over the chip; measured: starting at 8192 cells
+// slows the 4096x2048 step by ~2%)
+constexpr size_t kCoarseMaxCells = 4096;
 
 // A simple bump allocator over one hipMalloc.
 struct Pool {
