@@ -1065,10 +1065,11 @@ int BatchEngine::build_tiles() {
     });
   }
   // Fused passes: each domain's columns split evenly (tile_cols); the rows
-  // fixed by the tuning, or per level the most (32, 16, 8) whose tiles over
-  // all member domains still reach the launch's target, with 512-thread
-  // workgroups for levels of at least kTileBigCells cells in total and
-  // 1024-thread ones below -- the single-domain rule over the whole level.
+  // fixed by the tuning, or per level the fewest (even, 8 to 32) whose tiles
+  // over all member domains fit in one residency wave of the launch's
+  // target, with 512-thread workgroups for levels of at least kTileBigCells
+  // cells in total and 1024-thread ones below -- the single-domain rule over
+  // the whole level.
   auto fused = [&](Span *out, int nu) {
     for (int l = 0; l < BMAXL; l++) {
       size_t cells = 0;
@@ -1090,8 +1091,8 @@ int BatchEngine::build_tiles() {
           }
           return t;
         };
-        oty = 32;
-        while (oty > 8 && tiles(oty) < tile_target(tpb)) oty /= 2;
+        oty = 8;
+        while (oty < 32 && tiles(oty) > tile_target(tpb)) oty += 2;
       }
       out[l].oty = oty;
       out[l].tpb = tpb;
