@@ -84,13 +84,13 @@ void launch_stream(int mode, int ns, Level &L, const Level &C, float h, float al
   a.hc = C.h;
   const int own = 64 - 2 * (ns + 2);
   a.nstrips = (L.w + 2 * own - 1) / (2 * own);
-  // rows per wave: nstrips*h/4096 (about 4096 waves), clamped to [16, 64],
-  // even.  Each row block runs up to 5NS+4 extra pipeline rows (halo rows,
-  // re-read from L2), so shorter blocks cost VALU work; longer ones leave too
-  // few waves in flight to cover the serial row loop's load latency.
-  // Measured on gfx950 at 4096x2048: R=18 beats R=2 (2.2x slower) and R=32
-  // (1.5x slower).
-  int R = (int)((long)a.nstrips * L.h / 4096);
+  // rows per wave: nstrips*h/4608 (about 4600 waves, 4.5 per SIMD), clamped
+  // to [16, 64], even.  Each row block runs up to 5NS+4 extra pipeline rows
+  // (halo rows, re-read from L2), so shorter blocks cost VALU work; longer
+  // ones leave too few waves in flight to cover the serial row loop's load
+  // latency.  Measured on gfx950 at 4096x2048: R=16 beats R=18 (~1.5%), R=20
+  // (~3%), R=2 (2.2x slower) and R=32 (1.5x slower).
+  int R = (int)((long)a.nstrips * L.h / 4608);
   R = R < 16 ? 16 : (R > 64 ? 64 : R);
   R &= ~1;
   a.R = R;
