@@ -73,7 +73,7 @@ int mg_create(int W, int H, hipStream_t stream, float *p0, float *p0_alt, float 
   // level 0 too until updateFields.
   for (int l = 0; l < m->levels; l++) {
     if (l == 0 && p0) continue;
-    hipMemsetAsync(m->lv[l].flag, 1, (size_t)m->lv[l].w * m->lv[l].h, m->stream);
+    UBG_HIP(hipMemsetAsync(m->lv[l].flag, 1, (size_t)m->lv[l].w * m->lv[l].h, m->stream));
     launch_build_mask(m->lv[l].flag, m->lv[l].mask, m->lv[l].w, m->lv[l].h, m->stream);
   }
   rc = check_launch();
@@ -913,11 +913,13 @@ UBG_API int ubg_sim_bilinear_vel(ubg_sim *s, const float *c, float *out, int n) 
   std::lock_guard<std::recursive_mutex> g(s->mu_);
   float *d = nullptr;
   UBG_TRY(s->staging(sizeof(float) * 4 * n, (void **)&d));
-  hipMemcpyAsync(d, c, sizeof(float) * 2 * n, hipMemcpyHostToDevice, s->stream);
+  UBG_HIP(hipMemcpyAsync(d, c, sizeof(float) * 2 * n, hipMemcpyHostToDevice, s->stream));
   launch_bilinear_vel(s->vel(), d, d + 2 * n, n, s->stream);
   int rc = check_launch();
-  hipMemcpyAsync(out, d + 2 * n, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, s->stream);
-  hipError_t e = hipStreamSynchronize(s->stream);
+  hipError_t e = hipMemcpyAsync(out, d + 2 * n, sizeof(float) * 2 * n, hipMemcpyDeviceToHost,
+                                s->stream);
+  const hipError_t es = hipStreamSynchronize(s->stream);
+  if (e == hipSuccess) e = es;
   if (rc) return rc;
   if (e != hipSuccess) return fail(UBG_E_HIP, hipGetErrorString(e));
   return UBG_OK;
@@ -938,14 +940,16 @@ UBG_API int ubg_sim_advect_items(ubg_sim *s, ubg_item *items, int n, float game_
     pos[2 * i] = items[i].pos[0];
     pos[2 * i + 1] = items[i].pos[1];
   }
-  hipMemcpyAsync(di, items, sizeof(ubg_item) * n, hipMemcpyHostToDevice, s->stream);
-  hipMemcpyAsync(snap, pos.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice, s->stream);
+  UBG_HIP(hipMemcpyAsync(di, items, sizeof(ubg_item) * n, hipMemcpyHostToDevice, s->stream));
+  UBG_HIP(hipMemcpyAsync(snap, pos.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice,
+                         s->stream));
   launch_items(di, snap, n, s->vx[s->fx], s->vy[s->fy], s->P(), s->flag, s->vx_acc, s->vy_acc, s->W,
                s->H, s->h, s->pwidth, game_dt, simple != 0, s->stream);
   s->accum_dirty = true; // the next step adds the items' reaction to the velocity
   int rc = check_launch();
-  hipMemcpyAsync(items, di, sizeof(ubg_item) * n, hipMemcpyDeviceToHost, s->stream);
-  hipError_t e = hipStreamSynchronize(s->stream);
+  hipError_t e = hipMemcpyAsync(items, di, sizeof(ubg_item) * n, hipMemcpyDeviceToHost, s->stream);
+  const hipError_t es = hipStreamSynchronize(s->stream);
+  if (e == hipSuccess) e = es;
   if (rc) return rc;
   if (e != hipSuccess) return fail(UBG_E_HIP, hipGetErrorString(e));
   return UBG_OK;
@@ -959,7 +963,7 @@ UBG_API int ubg_sim_shift_map(ubg_sim *s, const float *new_flag) {
   UBG_TRY(validate_flag(new_flag, (size_t)s->W * s->H, f8));
   uint8_t *d = nullptr;
   UBG_TRY(s->staging(f8.size(), (void **)&d));
-  hipMemcpyAsync(d, f8.data(), f8.size(), hipMemcpyHostToDevice, s->stream);
+  UBG_HIP(hipMemcpyAsync(d, f8.data(), f8.size(), hipMemcpyHostToDevice, s->stream));
   launch_shift_map(s->vel(), s->P(), d, s->flag, s->stream);
   s->vy_bc_fresh = false;
   s->save_current();              // ubootgl_app.cpp:292
@@ -989,9 +993,11 @@ UBG_API int ubg_terrain_flags(const uint8_t *rgba, int width, int height, int sc
   hipStream_t st = nullptr;
   hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
   if (e == hipSuccess) {
-    hipMemcpyAsync(drgba, rgba, 4 * npx, hipMemcpyHostToDevice, st);
-    launch_terrain_flags(drgba, width, height, scale, dfull, dsim, st);
-    e = hipGetLastError();
+    e = hipMemcpyAsync(drgba, rgba, 4 * npx, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+      launch_terrain_flags(drgba, width, height, scale, dfull, dsim, st);
+      e = hipGetLastError();
+    }
     if (e == hipSuccess && flag_full)
       e = hipMemcpyAsync(flag_full, dfull, 4 * npx, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess && nsim) e = hipMemcpyAsync(flag_sim, dsim, 4 * nsim, hipMemcpyDeviceToHost, st);
@@ -1016,9 +1022,11 @@ UBG_API int ubg_sim_collocate(ubg_sim *s, float *vxy, float *mag, int on_device)
   UBG_TRY(s->staging(sizeof(float) * 3 * n, (void **)&d));
   launch_collocate(s->vx_cur, s->vy_cur, d, d + 2 * n, s->W, s->H, s->stream);
   int rc = check_launch();
-  hipMemcpyAsync(vxy, d, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, s->stream);
-  hipMemcpyAsync(mag, d + 2 * n, sizeof(float) * n, hipMemcpyDeviceToHost, s->stream);
-  hipError_t e = hipStreamSynchronize(s->stream);
+  hipError_t e = hipMemcpyAsync(vxy, d, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(mag, d + 2 * n, sizeof(float) * n, hipMemcpyDeviceToHost, s->stream);
+  const hipError_t es = hipStreamSynchronize(s->stream);
+  if (e == hipSuccess) e = es;
   if (rc) return rc;
   if (e != hipSuccess) return fail(UBG_E_HIP, hipGetErrorString(e));
   return UBG_OK;
