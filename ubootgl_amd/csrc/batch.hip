@@ -1006,6 +1006,7 @@ constexpr int kStreamMinW = 512;
 constexpr size_t kStreamMinCells = size_t(1) << 18;
 constexpr long kStreamMinUnits = 4096;
 constexpr int kStreamR = 16;
+static_assert(kStreamR % 2 == 0, "streamed passes need an even row-block height");
 
 bool BatchEngine::stream_eligible(const BGeom &G, int l) const {
   return l < G.cfirst && tiled() && prm_.nu_pre >= 1 && prm_.nu_post >= 1 &&

@@ -53,7 +53,7 @@ struct StreamArgs {
   const float *cpin;     // MODE_PC: coarse solution (ecs[l+1])
   const uint8_t *cmask;  // MODE_PC: coarse flag mask
   int wc, hc;
-  int nstrips, R, nunits;
+  int nstrips, R, nunits; // R: rows per block, even (mg_stream_unit needs it)
 };
 
 struct P2 {
@@ -148,13 +148,13 @@ __device__ __forceinline__ void mg_stream_unit(const StreamArgs &A, int unit) {
   P2 out1{0.0f, 0.0f}, out2{0.0f, 0.0f};       // last two emitted final rows
   P2 res1{0.0f, 0.0f}, res2{0.0f, 0.0f};       // last two residual rows
 
-  P2 E[NW];      // each stage's output row of the previous iteration
-  int jE[NW];
+  // each stage's output row of the previous iteration: stage k emitted row
+  // (t - 1) - 3k - 2 there, so stage k's input at iteration t is row t - 3k
+  // (before the pipeline is full these are zero rows; no update or store
+  // that uses them is kept)
+  P2 E[NW];
 #pragma unroll
-  for (int k = 0; k < NW; k++) {
-    E[k] = P2{0.0f, 0.0f};
-    jE[k] = -1000000;
-  }
+  for (int k = 0; k < NW; k++) E[k] = P2{0.0f, 0.0f};
 
   // two-row prefetch of the input stream (p, f, mask)
   P2 px1 = ld2(A.src, L0), px2 = ld2(A.src, L0 + 1);
@@ -163,141 +163,145 @@ __device__ __forceinline__ void mg_stream_unit(const StreamArgs &A, int unit) {
   ldm2(L0, pm1a, pm1b);
   ldm2(L0 + 1, pm2a, pm2b);
 
-  for (int t = L0; t <= TEND; t++) {
-    // ---- input row t and the f / mask queue (FQ[q] = row t - q)
+  // Two rows per trip: L0 is even (R is even), so each half of a trip
+  // has a constant row parity and the register queues shift by renaming
+  // inside the trip instead of by moves every row.
+  for (int t2 = L0; t2 <= TEND; t2 += 2) {
 #pragma unroll
-    for (int q = Q - 1; q > 0; q--) {
-      FQ[q] = FQ[q - 1];
-      MQa[q] = MQa[q - 1];
-      MQb[q] = MQb[q - 1];
-    }
-    FQ[0] = pf1;
-    MQa[0] = pm1a;
-    MQb[0] = pm1b;
-    P2 x = px1;
-    px1 = px2;
-    pf1 = pf2;
-    pm1a = pm2a;
-    pm1b = pm2b;
-    px2 = ld2(A.src, t + 2);
-    pf2 = ld2(A.f, t + 2);
-    ldm2(t + 2, pm2a, pm2b);
-    if (MODE == MODE_PC) {
-      // prolongate + correct (pressure_solver.cpp:134-181) on interior cells
-      if (t >= 1 && t <= h - 2) {
-        const int wc = A.wc, hc = A.hc;
-        const int X = max(0, min(x0 >> 1, wc - 1)), X1 = min(X + 1, wc - 1);
-        const int Y = min(t >> 1, hc - 1), Y1 = min(Y + 1, hc - 1);
-        const float *e0 = A.cpin + (size_t)Y * wc, *e1 = A.cpin + (size_t)Y1 * wc;
-        const uint8_t *g0 = A.cmask + (size_t)Y * wc, *g1 = A.cmask + (size_t)Y1 * wc;
-        const float Fa = (MQa[0] & MK_SELF) ? 1.0f : 0.0f, Fb = (MQb[0] & MK_SELF) ? 1.0f : 0.0f;
-        float ea = 0.0f, eb = 0.0f;
-        if (!(t & 1)) {
-          ea = e0[X] * Fa;
-          if (x1 <= w - 3)
-            eb = Fb * (e0[X] + e0[X1]) / eps_count((g0[X] & 1) + (g0[X1] & 1));
-        } else if (t <= h - 3) {
-          ea = Fa * (e0[X] + e1[X]) / eps_count((g0[X] & 1) + (g1[X] & 1));
-          if (x1 <= w - 3)
-            eb = Fb * (e0[X] + e1[X1] + e0[X1] + e1[X]) /
-                 eps_count((g0[X] & 1) + (g1[X1] & 1) + (g0[X1] & 1) + (g1[X] & 1));
+    for (int par = 0; par < 2; par++) {
+      const int t = t2 + par;
+      if (par == 1 && t > TEND) break;
+      // ---- input row t and the f / mask queue (FQ[q] = row t - q)
+#pragma unroll
+      for (int q = Q - 1; q > 0; q--) {
+        FQ[q] = FQ[q - 1];
+        MQa[q] = MQa[q - 1];
+        MQb[q] = MQb[q - 1];
+      }
+      FQ[0] = pf1;
+      MQa[0] = pm1a;
+      MQb[0] = pm1b;
+      P2 x = px1;
+      px1 = px2;
+      pf1 = pf2;
+      pm1a = pm2a;
+      pm1b = pm2b;
+      px2 = ld2(A.src, t + 2);
+      pf2 = ld2(A.f, t + 2);
+      ldm2(t + 2, pm2a, pm2b);
+      if (MODE == MODE_PC) {
+        // prolongate + correct (pressure_solver.cpp:134-181) on interior cells
+        if (t >= 1 && t <= h - 2) {
+          const int wc = A.wc, hc = A.hc;
+          const int X = max(0, min(x0 >> 1, wc - 1)), X1 = min(X + 1, wc - 1);
+          const int Y = min(t >> 1, hc - 1), Y1 = min(Y + 1, hc - 1);
+          const float *e0 = A.cpin + (size_t)Y * wc, *e1 = A.cpin + (size_t)Y1 * wc;
+          const uint8_t *g0 = A.cmask + (size_t)Y * wc, *g1 = A.cmask + (size_t)Y1 * wc;
+          const float Fa = (MQa[0] & MK_SELF) ? 1.0f : 0.0f, Fb = (MQb[0] & MK_SELF) ? 1.0f : 0.0f;
+          float ea = 0.0f, eb = 0.0f;
+          if (par == 0) { // t even
+            ea = e0[X] * Fa;
+            if (x1 <= w - 3)
+              eb = Fb * (e0[X] + e0[X1]) / eps_count((g0[X] & 1) + (g0[X1] & 1));
+          } else if (t <= h - 3) {
+            ea = Fa * (e0[X] + e1[X]) / eps_count((g0[X] & 1) + (g1[X] & 1));
+            if (x1 <= w - 3)
+              eb = Fb * (e0[X] + e1[X1] + e0[X1] + e1[X]) /
+                   eps_count((g0[X] & 1) + (g1[X1] & 1) + (g0[X1] & 1) + (g1[X] & 1));
+          }
+          x.a = int0 ? x.a + ea : x.a;
+          x.b = int1 ? x.b + eb : x.b;
+          if (ZG) x = zg_cols(x);
+        } else if (ZG && t == h - 1) {
+          x = zg_row(x, prev_in);
         }
-        x.a = int0 ? x.a + ea : x.a;
-        x.b = int1 ? x.b + eb : x.b;
-        if (ZG) x = zg_cols(x);
-      } else if (ZG && t == h - 1) {
-        x = zg_row(x, prev_in);
+        prev_in = x;
       }
-      prev_in = x;
-    }
-    // ---- sweep stages; stage k's input is stage k-1's output of the
-    // previous iteration (stage 0: the new input row)
-    P2 nx[NW];
-    int nj[NW];
+      // ---- sweep stages; stage k's input is stage k-1's output of the
+      // previous iteration (stage 0: the new input row)
+      P2 nx[NW];
 #pragma unroll
-    for (int k = 0; k < NS; k++) {
-      const P2 in = k == 0 ? x : E[k - 1];
-      const int jin = k == 0 ? t : jE[k - 1];
-      W[k][0] = W[k][1];
-      W[k][1] = W[k][2];
-      W[k][2] = W[k][3];
-      W[k][3] = in;
-      if (ZG && (k > 0 || MODE == MODE_PC) && jin == 1)
-        W[k][2] = zg_row(W[k][2], in); // row 0 of a produced row set takes row 1's interior
-      const int j = jin - 2;
-      const int qj = 3 * k + 2, qj1 = 3 * k + 1; // queue slots of rows j and j+1
-      if (j + 1 >= 1 && j + 1 <= h - 2) {         // red cells of row j+1
-        if (((j + 1) & 1) == 1) W[k][2].a = relax_a(W[k][2], W[k][1].a, W[k][3].a, FQ[qj1].a, MQa[qj1]);
-        else W[k][2].b = relax_b(W[k][2], W[k][1].b, W[k][3].b, FQ[qj1].b, MQb[qj1]);
+      for (int k = 0; k < NS; k++) {
+        const P2 in = k == 0 ? x : E[k - 1];
+        const int jin = t - 3 * k;
+        W[k][0] = W[k][1];
+        W[k][1] = W[k][2];
+        W[k][2] = W[k][3];
+        W[k][3] = in;
+        if (ZG && (k > 0 || MODE == MODE_PC) && jin == 1)
+          W[k][2] = zg_row(W[k][2], in); // row 0 of a produced row set takes row 1's interior
+        const int j = jin - 2;
+        const int qj = 3 * k + 2, qj1 = 3 * k + 1; // queue slots of rows j and j+1
+        const int jpar = (par + k) & 1;             // parity of j = t - 3k - 2
+        if (j + 1 >= 1 && j + 1 <= h - 2) {         // red cells of row j+1
+          if (jpar == 0) W[k][2].a = relax_a(W[k][2], W[k][1].a, W[k][3].a, FQ[qj1].a, MQa[qj1]);
+          else W[k][2].b = relax_b(W[k][2], W[k][1].b, W[k][3].b, FQ[qj1].b, MQb[qj1]);
+        }
+        if (j >= 1 && j <= h - 2) { // black cells of row j: final
+          if (jpar == 0) W[k][1].a = relax_a(W[k][1], W[k][0].a, W[k][2].a, FQ[qj].a, MQa[qj]);
+          else W[k][1].b = relax_b(W[k][1], W[k][0].b, W[k][2].b, FQ[qj].b, MQb[qj]);
+        }
+        P2 e = W[k][1];
+        if (ZG && j >= 1 && j <= h - 2) e = zg_cols(e);
+        if (ZG && j == h - 1) e = zg_row(e, W[k][0]);
+        nx[k] = e;
       }
-      if (j >= 1 && j <= h - 2) { // black cells of row j: final
-        if ((j & 1) == 0) W[k][1].a = relax_a(W[k][1], W[k][0].a, W[k][2].a, FQ[qj].a, MQa[qj]);
-        else W[k][1].b = relax_b(W[k][1], W[k][0].b, W[k][2].b, FQ[qj].b, MQb[qj]);
-      }
-      P2 e = W[k][1];
-      if (ZG && j >= 1 && j <= h - 2) e = zg_cols(e);
-      if (ZG && j == h - 1) e = zg_row(e, W[k][0]);
-      nx[k] = e;
-      nj[k] = j;
-    }
-    // ---- the final row of this iteration: last stage's output of the
-    // previous iteration (or the input itself when there is no sweep)
-    const P2 fin = NS > 0 ? E[NW - 1] : x;
-    const int jo = NS > 0 ? jE[NW - 1] : t;
+      // ---- the final row of this iteration: last stage's output of the
+      // previous iteration (or the input itself when there is no sweep)
+      const P2 fin = NS > 0 ? E[NW - 1] : x;
+      const int jo = NS > 0 ? t - 3 * NS : t;
 #pragma unroll
-    for (int k = 0; k < NS; k++) {
-      E[k] = nx[k];
-      jE[k] = nj[k];
-    }
-    x = fin;
+      for (int k = 0; k < NS; k++) E[k] = nx[k];
+      x = fin;
 
-    // ---- x is final row jo of the last sweep
-    if (OUT_ZG && jo == 1) out1 = zg_row(out1, x); // row 0 takes row 1's interior
-    if (own) {
-      // write owned rows; row 0 is written once row 1 is known
-      if (jo >= FA && jo <= FB && jo >= 1) {
-        float *row = A.dst + (size_t)jo * w;
-        if (AL) { // w even: an owned in-domain pair is whole and 8-byte aligned
-          if (in0) *(float2 *)(row + x0) = make_float2(x.a, x.b);
-        } else {
-          if (in0) row[x0] = x.a;
-          if (in1) row[x1] = x.b;
+      // ---- x is final row jo of the last sweep
+      if (OUT_ZG && jo == 1) out1 = zg_row(out1, x); // row 0 takes row 1's interior
+      if (own) {
+        // write owned rows; row 0 is written once row 1 is known
+        if (jo >= FA && jo <= FB && jo >= 1) {
+          float *row = A.dst + (size_t)jo * w;
+          if (AL) { // w even: an owned in-domain pair is whole and 8-byte aligned
+            if (in0) *(float2 *)(row + x0) = make_float2(x.a, x.b);
+          } else {
+            if (in0) row[x0] = x.a;
+            if (in1) row[x1] = x.b;
+          }
+        }
+        if (jo == 1 && FA == 0) {
+          if (in0) A.dst[x0] = out1.a;
+          if (in1) A.dst[x1] = out1.b;
         }
       }
-      if (jo == 1 && FA == 0) {
-        if (in0) A.dst[x0] = out1.a;
-        if (in1) A.dst[x1] = out1.b;
-      }
-    }
-    if (MODE == MODE_RR) {
-      // residual of row i = jo - 1 (rows i-1, i, i+1 = out2, out1, x)
-      const int i = jo - 1;
-      const int qi = NS > 0 ? 3 * NS + 1 : 1;
-      P2 rr{0.0f, 0.0f};
-      if (i >= 1 && i <= h - 2) {
-        rr.a = residual_of(out1.a, from_left(out1.b), out1.b, out2.a, x.a, FQ[qi].a, MQa[qi], A.ihsq);
-        rr.b = residual_of(out1.b, out1.a, from_right(out1.a), out2.b, x.b, FQ[qi].b, MQb[qi], A.ihsq);
-      }
-      // restrict (pressure_solver.cpp:118-132) when rows 2Y-1, 2Y, 2Y+1 are done
-      if ((i & 1) && i >= 1) {
-        const int Y = (i - 1) >> 1, X = x0 >> 1;
-        const float l2 = from_left(res2.b), l1 = from_left(res1.b), l0 = from_left(rr.b);
-        float v = 0.0f;
-        v = l2 * 1.0f + res2.a * 2.0f + res2.b * 1.0f;
-        v += l1 * 2.0f + res1.a * 4.0f + res1.b * 2.0f;
-        v += l0 * 1.0f + rr.a * 2.0f + rr.b * 1.0f;
-        if (own && Y >= 1 && Y <= A.hc - 2 && X >= 1 && X <= A.wc - 2 && 2 * Y >= FA &&
-            2 * Y <= FA + A.R - 1) {
-          size_t ci = (size_t)Y * A.wc + X;
-          A.crc[ci] = v * (1.0f / 16.0f);
-          A.cec[ci] = 0.0f;
+      if (MODE == MODE_RR) {
+        // residual of row i = jo - 1 (rows i-1, i, i+1 = out2, out1, x)
+        const int i = jo - 1;
+        const int qi = NS > 0 ? 3 * NS + 1 : 1;
+        P2 rr{0.0f, 0.0f};
+        if (i >= 1 && i <= h - 2) {
+          rr.a = residual_of(out1.a, from_left(out1.b), out1.b, out2.a, x.a, FQ[qi].a, MQa[qi], A.ihsq);
+          rr.b = residual_of(out1.b, out1.a, from_right(out1.a), out2.b, x.b, FQ[qi].b, MQb[qi], A.ihsq);
         }
+        // restrict (pressure_solver.cpp:118-132) when rows 2Y-1, 2Y, 2Y+1 are done
+        if (((par + NS + 1) & 1) && i >= 1) { // i = t - 3NS - 1 odd
+          const int Y = (i - 1) >> 1, X = x0 >> 1;
+          const float l2 = from_left(res2.b), l1 = from_left(res1.b), l0 = from_left(rr.b);
+          float v = 0.0f;
+          v = l2 * 1.0f + res2.a * 2.0f + res2.b * 1.0f;
+          v += l1 * 2.0f + res1.a * 4.0f + res1.b * 2.0f;
+          v += l0 * 1.0f + rr.a * 2.0f + rr.b * 1.0f;
+          if (own && Y >= 1 && Y <= A.hc - 2 && X >= 1 && X <= A.wc - 2 && 2 * Y >= FA &&
+              2 * Y <= FA + A.R - 1) {
+            size_t ci = (size_t)Y * A.wc + X;
+            A.crc[ci] = v * (1.0f / 16.0f);
+            A.cec[ci] = 0.0f;
+          }
+        }
+        res2 = res1;
+        res1 = rr;
       }
-      res2 = res1;
-      res1 = rr;
+      out2 = out1;
+      out1 = x;
     }
-    out2 = out1;
-    out1 = x;
   }
 }
 
