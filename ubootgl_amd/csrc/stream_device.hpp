@@ -59,6 +59,9 @@ struct StreamArgs {
 struct P2 {
   float a, b;
 };
+struct M2 {
+  unsigned a, b;
+};
 
 template <int NS, int MODE, bool ZG, bool AL>
 __device__ __forceinline__ void mg_stream_unit(const StreamArgs &A, int unit) {
@@ -99,16 +102,13 @@ __device__ __forceinline__ void mg_stream_unit(const StreamArgs &A, int unit) {
     }
     return P2{row[c0], row[c1]};
   };
-  auto ldm2 = [&](int yy, unsigned &ma, unsigned &mb) {
+  // a row's mask pair as loaded (AL: both bytes in one 16-bit value); it is
+  // split into the two cells' masks only when the row enters the queue, a
+  // trip after its load was issued (splitting at the load would wait for it)
+  auto ldm2 = [&](int yy) -> M2 {
     const uint8_t *row = A.m + rowoff(yy);
-    if (AL) {
-      uchar2 q = *(const uchar2 *)(row + cp);
-      ma = q.x;
-      mb = q.y;
-    } else {
-      ma = row[c0];
-      mb = row[c1];
-    }
+    if (AL) return M2{(unsigned)*(const uint16_t *)(row + cp), 0u};
+    return M2{row[c0], row[c1]};
   };
   auto relax_a = [&](P2 v, float dn, float up, float fv, unsigned mk) {
     float na = smooth(v.a, from_left(v.b), v.b, dn, up, fv, mk, A.hh, A.alpha);
@@ -156,17 +156,19 @@ __device__ __forceinline__ void mg_stream_unit(const StreamArgs &A, int unit) {
 #pragma unroll
   for (int k = 0; k < NW; k++) E[k] = P2{0.0f, 0.0f};
 
-  // two-row prefetch of the input stream (p, f, mask)
-  P2 px1 = ld2(A.src, L0), px2 = ld2(A.src, L0 + 1);
-  P2 pf1 = ld2(A.f, L0), pf2 = ld2(A.f, L0 + 1);
-  unsigned pm1a, pm1b, pm2a, pm2b;
-  ldm2(L0, pm1a, pm1b);
-  ldm2(L0 + 1, pm2a, pm2b);
-
   // Two rows per trip: L0 is even (R is even), so each half of a trip
   // has a constant row parity and the register queues shift by renaming
-  // inside the trip instead of by moves every row.
+  // inside the trip instead of by moves every row.  The input rows (p, f,
+  // mask) of a trip are loaded one trip ahead: a trip first issues the loads
+  // of the next trip's two rows and hands them over at its end, so the wait
+  // for them falls a whole trip after they were issued.
+  P2 px[2] = {ld2(A.src, L0), ld2(A.src, L0 + 1)};
+  P2 pf[2] = {ld2(A.f, L0), ld2(A.f, L0 + 1)};
+  M2 pm[2] = {ldm2(L0), ldm2(L0 + 1)};
   for (int t2 = L0; t2 <= TEND; t2 += 2) {
+    const P2 lx0 = ld2(A.src, t2 + 2), lx1 = ld2(A.src, t2 + 3);
+    const P2 lf0 = ld2(A.f, t2 + 2), lf1 = ld2(A.f, t2 + 3);
+    const M2 lm0 = ldm2(t2 + 2), lm1 = ldm2(t2 + 3);
 #pragma unroll
     for (int par = 0; par < 2; par++) {
       const int t = t2 + par;
@@ -178,17 +180,10 @@ __device__ __forceinline__ void mg_stream_unit(const StreamArgs &A, int unit) {
         MQa[q] = MQa[q - 1];
         MQb[q] = MQb[q - 1];
       }
-      FQ[0] = pf1;
-      MQa[0] = pm1a;
-      MQb[0] = pm1b;
-      P2 x = px1;
-      px1 = px2;
-      pf1 = pf2;
-      pm1a = pm2a;
-      pm1b = pm2b;
-      px2 = ld2(A.src, t + 2);
-      pf2 = ld2(A.f, t + 2);
-      ldm2(t + 2, pm2a, pm2b);
+      FQ[0] = pf[par];
+      MQa[0] = AL ? pm[par].a & 0xffu : pm[par].a;
+      MQb[0] = AL ? pm[par].a >> 8 : pm[par].b;
+      P2 x = px[par];
       if (MODE == MODE_PC) {
         // prolongate + correct (pressure_solver.cpp:134-181) on interior cells
         if (t >= 1 && t <= h - 2) {
@@ -302,6 +297,12 @@ __device__ __forceinline__ void mg_stream_unit(const StreamArgs &A, int unit) {
       out2 = out1;
       out1 = x;
     }
+    px[0] = lx0;
+    px[1] = lx1;
+    pf[0] = lf0;
+    pf[1] = lf1;
+    pm[0] = lm0;
+    pm[1] = lm1;
   }
 }
 
@@ -349,21 +350,25 @@ __device__ __forceinline__ void rbgs_pair_unit(const float *__restrict__ src,
       b = row[c1];
     }
   };
-  auto ldfm = [&](int yy, float &fa, float &fb, unsigned &ma, unsigned &mb) {
+  // f pair and mask pair of a row; the mask pair as loaded (AL: both bytes
+  // in one 16-bit value), split by `split` one row later, so the split does
+  // not wait for the load issued in the same iteration
+  auto ldfm = [&](int yy, float &fa, float &fb, M2 &mq) {
     const size_t o = rowoff(yy);
     if (AL) {
       float2 v = *(const float2 *)(f + o + cp);
-      uchar2 q = *(const uchar2 *)(m + o + cp);
       fa = v.x;
       fb = v.y;
-      ma = q.x;
-      mb = q.y;
+      mq = M2{(unsigned)*(const uint16_t *)(m + o + cp), 0u};
     } else {
       fa = f[o + c0];
       fb = f[o + c1];
-      ma = m[o + c0];
-      mb = m[o + c1];
+      mq = M2{m[o + c0], m[o + c1]};
     }
+  };
+  auto split = [&](M2 q, unsigned &ma, unsigned &mb) {
+    ma = AL ? q.a & 0xffu : q.a;
+    mb = AL ? q.a >> 8 : q.b;
   };
   // New value of cell a (x0) or cell b (x1) of the pair in an interior row;
   // the horizontal neighbour outside the pair comes from the adjacent lane.
@@ -386,24 +391,29 @@ __device__ __forceinline__ void rbgs_pair_unit(const float *__restrict__ src,
 
   float pm1a, pm1b, p0a, p0b, p1a, p1b, p2a, p2b, p3a, p3b, pm2a, pm2b;
   float fm1a, fm1b, f0a, f0b, f1a, f1b, f2a, f2b;
-  unsigned mm1a, mm1b, m0a, m0b, m1a, m1b, m2a, m2b;
+  unsigned mm1a, mm1b, m0a, m0b, m1a, m1b;
+  M2 mq;
   ldp(ya - 2, pm2a, pm2b);
   ldp(ya - 1, pm1a, pm1b);
   ldp(ya, p0a, p0b);
   ldp(ya + 1, p1a, p1b);
   ldp(ya + 2, p2a, p2b);
   ldp(ya + 3, p3a, p3b);
-  ldfm(ya - 1, fm1a, fm1b, mm1a, mm1b);
-  ldfm(ya, f0a, f0b, m0a, m0b);
-  ldfm(ya + 1, f1a, f1b, m1a, m1b);
-  ldfm(ya + 2, f2a, f2b, m2a, m2b);
+  ldfm(ya - 1, fm1a, fm1b, mq);
+  split(mq, mm1a, mm1b);
+  ldfm(ya, f0a, f0b, mq);
+  split(mq, m0a, m0b);
+  ldfm(ya + 1, f1a, f1b, mq);
+  split(mq, m1a, m1b);
+  M2 m2; // row y + 2 (raw)
+  ldfm(ya + 2, f2a, f2b, m2);
   if (ya - 1 >= 1) RELAX(pm1a, pm1b, pm2a, pm2b, p0a, p0b, fm1a, fm1b, mm1a, mm1b, ya - 1, 1);
   RELAX(p0a, p0b, pm1a, pm1b, p1a, p1b, f0a, f0b, m0a, m0b, ya, 1);
   for (int y = ya; y <= yb; y++) {
     float p4a, p4b, f3a, f3b; // two rows of lead
-    unsigned m3a, m3b;
+    M2 m3;
     ldp(y + 4, p4a, p4b);
-    ldfm(y + 3, f3a, f3b, m3a, m3b);
+    ldfm(y + 3, f3a, f3b, m3);
     if (y + 1 <= h - 2) RELAX(p1a, p1b, p0a, p0b, p2a, p2b, f1a, f1b, m1a, m1b, y + 1, 1);
     RELAX(p0a, p0b, pm1a, pm1b, p1a, p1b, f0a, f0b, m0a, m0b, y, 0); // row y final
     float oa = p0a, ob = p0b;
@@ -437,8 +447,8 @@ __device__ __forceinline__ void rbgs_pair_unit(const float *__restrict__ src,
     f1a = f2a; f1b = f2b;
     f2a = f3a; f2b = f3b;
     m0a = m1a; m0b = m1b;
-    m1a = m2a; m1b = m2b;
-    m2a = m3a; m2b = m3b;
+    split(m2, m1a, m1b);
+    m2 = m3;
   }
 #undef RELAX
 }
