@@ -10,6 +10,7 @@ namespace ubg {
 
 constexpr int MAXL = 16;
 constexpr int TPB_C = 1024;
+constexpr int WAVES_C = TPB_C / 64;
 constexpr size_t LDS_BUDGET = 152 * 1024;
 
 struct CoarseArgs {
@@ -45,8 +46,8 @@ __device__ __forceinline__ RowGroups row_groups(int count) {
   RowGroups r;
   r.L = 1 << lg;
   r.lr = tx & (r.L - 1);
-  r.r0 = ty + 16 * (tx >> lg);
-  r.step = 16 << (6 - lg);
+  r.r0 = ty + WAVES_C * (tx >> lg);
+  r.step = WAVES_C << (6 - lg);
   return r;
 }
 
