@@ -265,19 +265,40 @@ __device__ __forceinline__ float cubic_hermite(float t, float A, float B, float 
 }
 
 // bicubicSample for one lane (interpolators.hpp:92-206): clamp to [3, n-3],
-// truncate, Catmull-Rom down each of four columns, then across.
+// truncate, Catmull-Rom down each of four columns, then across.  BUF (grids
+// under 2 GiB): the four 4-float rows are buffer loads, a 32-bit byte offset
+// for the first row and the row pitch as the (scalar) offset of the others,
+// instead of 64-bit address arithmetic per row; the clamp keeps every row in
+// the grid.  The descriptor's third word is the gfx9 raw-buffer format.
+template <bool BUF>
 __device__ __forceinline__ float bicubic(const float *__restrict__ g, int gw, int gh, float cx,
                                          float cy) {
   cx = fmaxf(fminf(cx, gw - 3.0f), 3.0f);
   cy = fmaxf(fminf(cy, gh - 3.0f), 3.0f);
   int icx = (int)cx, icy = (int)cy;
   float stx = cx - truncf(cx), sty = cy - truncf(cy);
-  const float *b0 = g + (ptrdiff_t)(icy - 1) * gw + (icx - 1);
-  const float *b1 = b0 + gw, *b2 = b1 + gw, *b3 = b2 + gw;
-  float c0 = cubic_hermite(sty, b0[0], b1[0], b2[0], b3[0]);
-  float c1 = cubic_hermite(sty, b0[1], b1[1], b2[1], b3[1]);
-  float c2 = cubic_hermite(sty, b0[2], b1[2], b2[2], b3[2]);
-  float c3 = cubic_hermite(sty, b0[3], b1[3], b2[3], b3[3]);
+  float r[4][4];
+  if (BUF) {
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)g, (short)0, gw * gh * 4, 0x00020000);
+    const int o = ((icy - 1) * gw + (icx - 1)) * 4;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, o, k * gw * 4, 0);
+#pragma unroll
+      for (int j = 0; j < 4; j++) r[k][j] = __uint_as_float(v[j]);
+    }
+  } else {
+    const float *b0 = g + (ptrdiff_t)(icy - 1) * gw + (icx - 1);
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) r[k][j] = b0[(ptrdiff_t)k * gw + j];
+  }
+  float c0 = cubic_hermite(sty, r[0][0], r[1][0], r[2][0], r[3][0]);
+  float c1 = cubic_hermite(sty, r[0][1], r[1][1], r[2][1], r[3][1]);
+  float c2 = cubic_hermite(sty, r[0][2], r[1][2], r[2][2], r[3][2]);
+  float c3 = cubic_hermite(sty, r[0][3], r[1][3], r[2][3], r[3][3]);
   return cubic_hermite(stx, c0, c1, c2, c3);
 }
 
@@ -287,7 +308,7 @@ __device__ __forceinline__ float bicubic(const float *__restrict__ g, int gw, in
 // flags of the skip test set; the skip test and write mask differ as in the
 // reference.  Lanes never reached (tail columns, skipped chunks) keep the back
 // buffer.
-template <bool IS_VY>
+template <bool IS_VY, bool BUF>
 __device__ __forceinline__ void advect_face(const float *__restrict__ xf,
                                             const float *__restrict__ yf,
                                             float *__restrict__ xb, float *__restrict__ yb,
@@ -313,10 +334,10 @@ __device__ __forceinline__ void advect_face(const float *__restrict__ xf,
     float vy1 = ((yf[x + (size_t)y * W] + yf[x + (size_t)(y - 1) * W]) +
                  (yf[x + 1 + (size_t)y * W] + yf[x + 1 + (size_t)(y - 1) * W])) * 0.25f;
     float midx = posx - vx1 * hdt, midy = posy - vy1 * hdt;
-    float vx2 = bicubic(xf, Wx, H, midx - 0.5f, midy);
-    float vy2 = bicubic(yf, W, Hy, midx, midy - 0.5f);
+    float vx2 = bicubic<BUF>(xf, Wx, H, midx - 0.5f, midy);
+    float vy2 = bicubic<BUF>(yf, W, Hy, midx, midy - 0.5f);
     float endx = posx - vx2 * fdt, endy = posy - vy2 * fdt;
-    float v = bicubic(xf, Wx, H, endx - 0.5f, endy);
+    float v = bicubic<BUF>(xf, Wx, H, endx - 0.5f, endy);
     v = v * (float)frow[x] * (float)frow[x + 1];
     xb[(size_t)Wx * y + x] = v;
   } else {
@@ -325,10 +346,10 @@ __device__ __forceinline__ void advect_face(const float *__restrict__ xf,
     float vx1 = ((xf[x + (size_t)y * Wx] + xf[x + (size_t)(y - 1) * Wx]) +
                  (xf[x + 1 + (size_t)y * Wx] + xf[x + 1 + (size_t)(y - 1) * Wx])) * 0.25f;
     float midx = posx - vx1 * hdt, midy = posy - vy1 * hdt;
-    float vx2 = bicubic(xf, Wx, H, midx - 0.5f, midy);
-    float vy2 = bicubic(yf, W, Hy, midx, midy - 0.5f);
+    float vx2 = bicubic<BUF>(xf, Wx, H, midx - 0.5f, midy);
+    float vy2 = bicubic<BUF>(yf, W, Hy, midx, midy - 0.5f);
     float endx = posx - vx2 * fdt, endy = posy - vy2 * fdt;
-    float v = bicubic(yf, W, Hy, endx, endy - 0.5f);
+    float v = bicubic<BUF>(yf, W, Hy, endx, endy - 0.5f);
     v = v * (float)frow[x] * (float)frow[x + W];
     yb[(size_t)W * y + x] = v;
   }
@@ -337,14 +358,15 @@ __device__ __forceinline__ void advect_face(const float *__restrict__ xf,
 // One launch for both sweeps: a thread advects its vx face and its vy face,
 // two independent dependency chains for the scheduler to interleave
 // (measured faster on gfx950 than one launch with a vx and a vy half).
+template <bool BUF>
 __global__ void k_advect(const float *__restrict__ xf, const float *__restrict__ yf,
                          float *__restrict__ xb, float *__restrict__ yb,
                          const uint8_t *__restrict__ fl, int W, int H, int nck_x, int nck_y,
                          float hdt, float fdt) {
   const int lane = blockIdx.x * blockDim.x + threadIdx.x; // 0-based: x = lane + 1
   const int y = blockIdx.y + 1;
-  if (lane < 8 * nck_x) advect_face<false>(xf, yf, xb, yb, fl, W, H, lane, y, hdt, fdt);
-  if (lane < 8 * nck_y) advect_face<true>(xf, yf, xb, yb, fl, W, H, lane, y, hdt, fdt);
+  if (lane < 8 * nck_x) advect_face<false, BUF>(xf, yf, xb, yb, fl, W, H, lane, y, hdt, fdt);
+  if (lane < 8 * nck_y) advect_face<true, BUF>(xf, yf, xb, yb, fl, W, H, lane, y, hdt, fdt);
 }
 
 // Collocated velocity for rendering (interp_shader.cs:14-33, dispatched by
@@ -759,8 +781,11 @@ void launch_advect(const VelView &v, const uint8_t *flag, float hdt, float fdt, 
   int nck_y = v.W > 9 ? (v.W - 2) / 8 : 0;
   int lanes = 8 * (nck_x > nck_y ? nck_x : nck_y);
   if (lanes == 0 || v.H < 3) return;
-  launch_k(k_advect, dim3(ceil_div(lanes, TPB), v.H - 2), dim3(TPB), 0,
-           s, v.vxf, v.vyf, v.vxb, v.vyb, flag, v.W, v.H, nck_x, nck_y, hdt, fdt);
+  // buffer-load gathers while every byte offset of a velocity grid fits in
+  // 31 bits
+  const bool buf = (size_t)v.W * v.H * 4 < ((size_t)1 << 31);
+  launch_k(buf ? k_advect<true> : k_advect<false>, dim3(ceil_div(lanes, TPB), v.H - 2),
+           dim3(TPB), 0, s, v.vxf, v.vyf, v.vxb, v.vyb, flag, v.W, v.H, nck_x, nck_y, hdt, fdt);
 }
 void launch_divergence(const VelView &v, float *f, float ih, hipStream_t s) {
   hipLaunchKernelGGL(k_divergence, dim3(ceil_div(v.W - 2, TPB), v.H - 2), dim3(TPB), 0, s, v.vxf,
