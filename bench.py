@@ -53,8 +53,11 @@ def main():
     ap.add_argument("--height", type=int, default=2048)
     ap.add_argument("--tracers", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-steps", type=int, default=120)
+    ap.add_argument("--cpu-steps", type=int, default=40,
+                    help="timesteps per CPU-baseline run (3 runs, the median reported)")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-hbm-sweep", action="store_true",
+                    help="skip the level-0 sweep timed on 8192x4096 (working set > 256 MB)")
     ap.add_argument("--copies", type=int, default=0,
                     help="step: after the run, N saveCurrentVelocityFields stages (plain "
                          "33.5 MB device copies; counter calibration)")
@@ -166,13 +169,20 @@ def main():
         with open(tpath) as fh:
             prof = json.load(fh)
     traffic = prof.get(dom, {}).get("bytes_per_launch")
+    # these counter figures come from the committed profile, not from this run
+    prof_source = None if not prof else (
+        f"not measured in this run: profiles/step_4096x2048_traffic.json, rocprofv3 --pmc "
+        f"FETCH_SIZE / WRITE_SIZE of the same workload at commit {prof.get('commit', '?')}, "
+        f"scaled by the counters' reading of plain device copies")
     valu = None
     vi = prof.get(dom, {}).get("valu_wave_instructions_per_launch")
     if vi:
         rate = vi / (K["avg_us"] * 1e-6) / 1e9
         valu = {"wave_instructions_per_launch": vi, "achieved": round(rate, 1),
                 "peak": VALU_PEAK_GINST, "unit": "G wave-instructions/s",
-                "frac": round(rate / VALU_PEAK_GINST, 4)}
+                "frac": round(rate / VALU_PEAK_GINST, 4),
+                "source": "instructions per launch from the same profile (SQ_INSTS_VALU), "
+                          "divided by this run's launch time"}
     sw = kern.get("sweep")
     smoother = None if sw is None else {
         "kernel": KERNEL_NAMES["sweep"], "achieved": round(sw["pass_GBps"], 1),
@@ -185,6 +195,8 @@ def main():
     sim.sync()
     vxg = sim.get("vx")
     finite = bool(np.isfinite(vxg).all())
+    sim.close()
+    smoother_big = None if args.no_hbm_sweep else smoother_out_of_cache()
 
     cpu = None
     if not args.no_cpu:
@@ -221,17 +233,65 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "traffic_source": prof_source,
             "avg_launch_us": round(K["avg_us"], 3),
             "launches": K["launches"],
             "bytes_per_launch": K["pass_bytes"],
             "unfused_equiv_GBps": round(K["unfused_equiv_GBps"], 1),
             "valu": valu,
+            "step_achieved": round(step_gb / (ms_per_step * 1e-3), 1),
+            "step_frac": round(step_gb / (ms_per_step * 1e-3) / HBM_PEAK_GBS, 4),
             "smoother": smoother,
+            "smoother_8192x4096": smoother_big,
             "other_kernels_us": {k: round(v["avg_us"], 3) for k, v in kern.items() if k != dom},
         },
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def smoother_out_of_cache(nsweeps=40):
+    """The level-0 red/black sweep on an 8192x4096 grid: p, p_alt and f
+    (134 MB each) plus the mask byte make a 436 MB working set, above the
+    256 MB Infinity Cache, so the achieved rate is HBM's.  Timed with the
+    dispatch-attached events over `nsweeps` back-to-back sweeps."""
+    import numpy as np
+
+    import ubootgl_amd as U
+    from ubootgl_amd import _lib as LL
+
+    W, H = 8192, 4096
+    g = np.random.default_rng(5)
+    flag = np.ones((H, W), np.float32)
+    flag[0, :] = flag[-1, :] = flag[:, 0] = flag[:, -1] = 0
+    for _ in range(200):  # disks, ~3% solid
+        cx, cy, r = g.uniform(64, W - 64), g.uniform(64, H - 64), g.uniform(8, 48)
+        y0, y1, x0, x1 = int(cy - r), int(cy + r) + 1, int(cx - r), int(cx + r) + 1
+        yy, xx = np.mgrid[y0:y1, x0:x1]
+        flag[y0:y1, x0:x1][(xx + 0.5 - cx) ** 2 + (yy + 0.5 - cy) ** 2 <= r * r] = 0
+    sim = U.Simulation(flag, (W - 1) / W, 0.0)
+    sim.mg.set_level(0, LL.LEVEL_P, (g.standard_normal((H, W)) * flag).astype(np.float32))
+    sim.mg.set_level(0, LL.LEVEL_F, g.standard_normal((H, W)).astype(np.float32))
+    h = float(np.float32(1.0 / W))
+    sim.mg.rbgs(h, 1.0, 4)  # warm-up
+    sim.sync()
+    sim.timer_enable(LL.TIMER_SMOOTH_L0, True)
+    t = time.perf_counter()
+    sim.mg.rbgs(h, 1.0, nsweeps)
+    sim.sync()
+    wall = time.perf_counter() - t
+    ms, n = sim.timer_read(LL.TIMER_SMOOTH_L0)
+    sim.timer_enable(LL.TIMER_SMOOTH_L0, False)
+    sim.close()
+    avg = ms / max(n, 1)
+    gbs = SMOOTH_BYTES_PER_CELL * W * H / (avg * 1e-3) / 1e9
+    log(f"sweep 8192x4096: {n} launches, avg {avg * 1e3:.2f} us -> {gbs:.0f} GB/s "
+        f"({wall * 1e3 / nsweeps:.2f} us per sweep by wall clock)")
+    return {"kernel": KERNEL_NAMES["sweep"], "grid": f"{W}x{H}",
+            "working_set_MB": round(13 * W * H / 1e6, 1),  # p, p_alt, f + mask
+            "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "avg_launch_us": round(avg * 1e3, 3),
+            "launches": n, "wall_us_per_sweep": round(wall * 1e6 / nsweeps, 3)}
 
 
 def bench_ir(args):
@@ -444,14 +504,73 @@ def bench_serve(args):
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(d, xy, nsteps):
-    """The CPU oracle (oracle/_build/libubg_oracle.so, OpenMP) on the same
-    workload: `nsteps` steps after one warm-up step, with the tracers."""
+def host_cpu():
+    """The host as lscpu reports it, plus the CPUs this process may run on."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket":
+                "cores_per_socket", "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys and keys[k.strip()] not in info:
+                v = v.strip()
+                info[keys[k.strip()]] = int(v) if v.isdigit() else v
+    except Exception as e:  # lscpu missing: report what Python knows
+        info["lscpu_error"] = str(e)
+    info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    return info
+
+
+def timing_build(host):
+    """oracle/'s timing build (the reference's -Ofast -march=native flags,
+    eight-wide advect) for THIS host: rebuilt when it was built on another CPU
+    model (-march=native)."""
+    import subprocess
+
+    from oracle import pyoracle as O
+    stamp = os.path.join(ROOT, "oracle", "_build", "native.cpu")
+    model = str(host.get("model", "unknown"))
+    fresh = os.path.exists(O.NATIVE_LIB_PATH) and os.path.exists(stamp) and \
+        open(stamp).read().strip() == model
+    if not fresh:
+        t = time.time()
+        out = subprocess.run(["make", "-B", "-C", os.path.join(ROOT, "oracle"), "native"],
+                             capture_output=True, text=True)
+        if out.returncode != 0:
+            raise RuntimeError("oracle timing build failed:\n" + out.stderr[-2000:])
+        with open(stamp, "w") as fh:
+            fh.write(model + "\n")
+        log(f"cpu baseline: built the timing oracle for {model} in {time.time() - t:.1f}s")
+    return O.NATIVE_LIB_PATH
+
+
+def cpu_baseline(d, xy, nsteps, runs=3):
+    """The CPU restatement of the step (oracle/, OpenMP) in its timing build
+    -- the reference's compiler flags -std=c++17 -Ofast -march=native
+    -fopenmp -DNDEBUG (Makefile:12) and an eight-wide AVX2 advect as the
+    reference's -- on the same workload with the tracers: `runs` runs of
+    `nsteps` steps after one warm-up step; value = the median run.  Threads:
+    every CPU this process may use, capped by OMP_NUM_THREADS when set (the
+    GPU box gives each GPU a share of the host's cores), bound close, one per
+    core."""
     import numpy as np
 
     from oracle import pyoracle as O
 
-    threads = int(os.environ.get("UBG_CPU_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    host = host_cpu()
+    lib = timing_build(host)
+    threads = int(os.environ.get("UBG_CPU_THREADS", "0"))
+    if not threads:
+        threads = host["affinity_cpus"]
+        if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+            threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+    # read by the OpenMP runtime when the library loads (nothing in this
+    # process has loaded it before)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    O.use_timing_build(lib)
     O.set_threads(threads)
     vx, vy = d.init_simulation_fields()
     ref = O.OracleSim(d.flag, float(d.pwidth), float(d.mu), nu_pre=2, nu_post=2, nu_coarse=5,
@@ -462,17 +581,29 @@ def cpu_baseline(d, xy, nsteps):
     age = np.zeros(pts.shape[0], np.float32)
     dt = float(d.dt)
     ref.step(dt)
-    t = time.perf_counter()
-    for k in range(nsteps):
-        ref.step(dt)
-        if pts.shape[0]:
-            ref.advect_tracers(pts, age, dt, k + 1)
-    el = time.perf_counter() - t
-    log(f"cpu baseline: {nsteps} steps in {el:.2f}s on {threads} threads")
-    return {"value": round(nsteps / el, 4), "unit": "timesteps/s", "cores": threads,
+    times = []
+    k = 0
+    for _ in range(runs):
+        t = time.perf_counter()
+        for _ in range(nsteps):
+            ref.step(dt)
+            if pts.shape[0]:
+                ref.advect_tracers(pts, age, dt, k + 1)
+            k += 1
+        times.append(time.perf_counter() - t)
+    med = float(np.median(times))
+    log(f"cpu baseline: {runs} x {nsteps} steps, {', '.join(f'{t:.2f}' for t in times)} s "
+        f"on {threads} threads ({host.get('model')})")
+    return {"value": round(nsteps / med, 4), "unit": "timesteps/s", "cores": threads,
             "kind": "port",
-            "sample": f"{nsteps} timesteps (+1 warm-up) of the same {d.W}x{d.H} workload "
-                      f"with tracers, oracle/ OpenMP build, {el:.1f}s"}
+            "sample": f"median of {runs} runs of {nsteps} timesteps (+1 warm-up) of the same "
+                      f"{d.W}x{d.H} workload with its {pts.shape[0]} tracers; oracle/ timing "
+                      f"build (g++ -std=c++17 -Ofast -march=native -fopenmp -DNDEBUG, the "
+                      f"reference's flags; eight-wide AVX2 advect); OMP_PROC_BIND="
+                      f"{os.environ.get('OMP_PROC_BIND')}, OMP_PLACES="
+                      f"{os.environ.get('OMP_PLACES')}",
+            "runs_s": [round(t, 3) for t in times],
+            "host": host}
 
 
 if __name__ == "__main__":

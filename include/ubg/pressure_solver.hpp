@@ -8,8 +8,8 @@
 //   updateFields(flag)      coarse classification on the device        (hpp:34-57)
 //   solve(p, f, flag, h, zg) one V-cycle; p, f, flag are host grids;
 //                           p is uploaded, solved and downloaded       (hpp:59-62)
-// Free functions rbgs() and calculateResidualField() (hpp:3-11) run on the
-// GPU through a per-thread cached MG of the grid's size.
+// Free functions gs(), rbgs() and calculateResidualField() (hpp:3-11) run on
+// the GPU through a per-thread cached MG of the grid's size.
 //
 // Errors: the reference has none; here a failing call throws
 // ubg_error (status code + ubg_last_error() text).  There is no class named
@@ -103,6 +103,17 @@ inline MG &cached_mg(int w, int h) {
   return *m;
 }
 } // namespace ubg_detail
+
+// gs(p, f, flag, h, alpha): one lexicographic Gauss-Seidel sweep
+// (pressure_solver.cpp:26-33)
+inline void gs(Single2DGrid &p, Single2DGrid &f, Single2DGrid &flag, float h, float alpha) {
+  MG &m = ubg_detail::cached_mg(p.width, p.height);
+  ubg_check(ubg_mg_update_fields(m.handle(), flag.data()));
+  ubg_check(ubg_mg_set_level_field(m.handle(), 0, UBG_LEVEL_P, p.data()));
+  ubg_check(ubg_mg_set_level_field(m.handle(), 0, UBG_LEVEL_F, f.data()));
+  ubg_check(ubg_mg_gs(m.handle(), h, alpha, 1));
+  ubg_check(ubg_mg_get_level_field(m.handle(), 0, UBG_LEVEL_P, p.data()));
+}
 
 // rbgs(p, f, flag, h, alpha): one red+black sweep (pressure_solver.cpp:49-89)
 inline void rbgs(Single2DGrid &p, Single2DGrid &f, Single2DGrid &flag, float h, float alpha) {

@@ -20,9 +20,17 @@
 
 #include <array>
 #include <mutex>
+#include <random>
 #include <sstream>
 #include <type_traits>
 #include <vector>
+
+// The game's component types (components.hpp:6-42), viewed by
+// advectFloatingItems / advectFloatingItemsSimple; complete wherever those
+// are called.
+struct CoItem;
+struct CoKinematics;
+struct CoKinematicsSimple;
 
 class Simulation {
 public:
@@ -31,9 +39,30 @@ public:
   struct Vec2 {
     float x, y;
   };
+  // a sink (x, y, strength), as the reference's glm::vec3; converts from any
+  // type with float members x, y, z, so sinks.push_back(glm::vec3(...))
+  // (explosion.cpp:33) compiles unchanged
   struct Vec3 {
     float x, y, z;
+    Vec3() = default;
+    Vec3(float x, float y, float z) : x(x), y(y), z(z) {}
+    template <class V, class = decltype(std::declval<const V &>().z)>
+    Vec3(const V &v) : x((float)v.x), y((float)v.y), z((float)v.z) {}
   };
+
+  // simulation.hpp:20-30: every field zero (the flag too), MG(width, height)
+  // with all coarse flags 1 until mg.updateFields(flag).  The reference leaves
+  // the boundary conditions uninitialised; here they start as the other
+  // constructor sets them.
+  Simulation(float pwidth, float mu, int width, int height)
+      : pwidth(pwidth), mu(mu), width(width), height(height), vx(width - 1, height),
+        vy(width, height - 1), vx_accum(width - 1, height), vy_accum(width, height - 1),
+        vx_current(width - 1, height), vy_current(width, height - 1), p(width, height),
+        f(width, height), flag(width, height), r(width, height), disx(0.0f, 1.0f),
+        disy(0.0f, (float)height / width) {
+    ubg_check(ubg_sim_create_blank(width, height, pwidth, mu, &h_));
+    attach();
+  }
 
   // simulation.hpp:32-67
   Simulation(const Single2DGrid &flagInput, float pwidth, float mu)
@@ -41,16 +70,9 @@ public:
         vx(width - 1, height), vy(width, height - 1), vx_accum(width - 1, height),
         vy_accum(width, height - 1), vx_current(width - 1, height),
         vy_current(width, height - 1), p(width, height), f(width, height), flag(flagInput),
-        r(width, height) {
+        r(width, height), disx(0.0f, 1.0f), disy(0.0f, (float)height / width) {
     ubg_check(ubg_sim_create(flagInput.data(), width, height, pwidth, mu, &h_));
-    float hh = 0;
-    ubg_check(ubg_sim_dims(h_, nullptr, nullptr, &hh, nullptr, nullptr));
-    h = hh;
-    ubg_mg *m = nullptr;
-    ubg_check(ubg_sim_get_mg(h_, &m));
-    mg = MG::borrow(m, width, height);
-    pullBCs();
-    syncToHost();
+    attach();
   }
   Simulation(const Simulation &) = delete;
   Simulation &operator=(const Simulation &) = delete;
@@ -83,14 +105,62 @@ public:
   }
   void saveCurrentVelocityFields() { stage(UBG_STAGE_SAVE_CURRENT); }
 
-  // advect_floating_items.cpp:16-274.  The reference takes an entt::registry
-  // and views CoItem + CoKinematics (or CoKinematicsSimple); this header does
-  // not depend on entt or the game's components, so the game keeps two
-  // forwarders:
-  //   void advectFloatingItems(entt::registry &r, float dt)
-  //     { advectFloatingItemsView<CoItem, CoKinematics>(r, dt, false); }
-  //   void advectFloatingItemsSimple(entt::registry &r, float dt)
-  //     { advectFloatingItemsView<CoItem, CoKinematicsSimple>(r, dt, true); }
+  // simulation.cpp:23-34, 50-78: the boundary rules as value functions
+  float singlePBC(BC bc, float a) { return bc == BC::OUTFLOW_ZERO_PRESSURE ? -a : a; }
+  float VBCPar(BC bc, float a, float b) {
+    if (bc == BC::INFLOW) return b;
+    if (bc == BC::OUTFLOW || bc == BC::OUTFLOW_ZERO_PRESSURE) return a > 0.0f ? a : 0.0f;
+    return 0.0f;
+  }
+  float VBCPer(BC bc, float a, float b) {
+    if (bc == BC::INFLOW) return b;
+    if (bc == BC::OUTFLOW || bc == BC::OUTFLOW_ZERO_PRESSURE) return a > 0.0f ? a : 0.0f;
+    return -a;
+  }
+
+  // simulation.cpp:226-238 (on the device fields; the diagnostic line goes
+  // to diag as in the reference)
+  float getDT() {
+    float rdt = 0.0f;
+    ubg_check(ubg_sim_get_dt(h_, &rdt));
+    diag << "SET_DT: dt=" << rdt << "\n";
+    return rdt;
+  }
+  // simulation.cpp:210-224 (on the device pressure)
+  void centerP() { ubg_check(ubg_sim_center_p(h_)); }
+
+  // simulation.cpp:398-425: flag samplers at a physical position (any type
+  // with float members x and y, e.g. glm::vec2)
+  template <typename V> float psampleFlagNearest(const V &pc) {
+    return sampleFlag1(UBG_FLAG_NEAREST, pc);
+  }
+  template <typename V> float psampleFlagLinear(const V &pc) {
+    return sampleFlag1(UBG_FLAG_LINEAR, pc);
+  }
+  template <typename V> V psampleFlagNormal(const V &pc) {
+    float in[2] = {(float)pc.x, (float)pc.y}, out[2];
+    ubg_check(ubg_sim_sample_flag(h_, UBG_FLAG_NORMAL, in, out, 1));
+    V r = pc;
+    r.x = out[0];
+    r.y = out[1];
+    return r;
+  }
+  // n positions at once (one device call): pc = (x0, y0, x1, y1, ...)
+  void psampleFlags(int mode, const float *pc, float *out, int n) {
+    ubg_check(ubg_sim_sample_flag(h_, mode, pc, out, n));
+  }
+
+  // advect_floating_items.cpp:16-138 and :140-274 with the reference's
+  // signatures (simulation.hpp:116-117): the registry (entt::registry in the
+  // game) views the game's CoItem + CoKinematics / CoKinematicsSimple.  This
+  // header does not include entt; the registry type is a template parameter,
+  // so the call in ubootgl_app.cpp:129-130 compiles unchanged.
+  template <class Registry> void advectFloatingItems(Registry &registry, float gameDT) {
+    advectFloatingItemsView<CoItem, CoKinematics>(registry, gameDT, false);
+  }
+  template <class Registry> void advectFloatingItemsSimple(Registry &registry, float gameDT) {
+    advectFloatingItemsView<CoItem, CoKinematicsSimple>(registry, gameDT, true);
+  }
   // The view's items are packed in visiting order, integrated on the device
   // and written back; their reaction lands in the device-side accumulation
   // fields, which the next step applies (the host mirrors vx_accum / vy_accum
@@ -207,9 +277,28 @@ public:
   Single2DGrid p, f, flag, r;
   MG mg;
   float h = 0.0f;
+  // simulation.hpp:136-138 (seeded and drawn by the game, not by the step)
+  std::default_random_engine gen;
+  std::uniform_real_distribution<float> disx;
+  std::uniform_real_distribution<float> disy;
   std::vector<Vec3> sinks;
 
 private:
+  void attach() {
+    float hh = 0;
+    ubg_check(ubg_sim_dims(h_, nullptr, nullptr, &hh, nullptr, nullptr));
+    h = hh;
+    ubg_mg *m = nullptr;
+    ubg_check(ubg_sim_get_mg(h_, &m));
+    mg = MG::borrow(m, width, height);
+    pullBCs();
+    syncToHost();
+  }
+  template <typename V> float sampleFlag1(int mode, const V &pc) {
+    float in[2] = {(float)pc.x, (float)pc.y}, out = 0.0f;
+    ubg_check(ubg_sim_sample_flag(h_, mode, in, &out, 1));
+    return out;
+  }
   void stage(int st) {
     pushBCs();
     pushSinks();

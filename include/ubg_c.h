@@ -138,6 +138,10 @@ UBG_API int ubg_mg_solve_host(ubg_mg *mg, float *p, const float *f, const float 
                               float h, int zero_gradient_bc);
 /* rbgs(p, f, flag, h, alpha) (pressure_solver.hpp:5-6), n sweeps, level 0. */
 UBG_API int ubg_mg_rbgs(ubg_mg *mg, float h, float alpha, int nsweeps);
+/* gs(p, f, flag, h, alpha) (pressure_solver.hpp:3-4, pressure_solver.cpp:
+ * 26-33): n lexicographic Gauss-Seidel sweeps on level 0 (x fastest, then
+ * y), the values of the sequential sweep. */
+UBG_API int ubg_mg_gs(ubg_mg *mg, float h, float alpha, int nsweeps);
 /* calculateResidualField (pressure_solver.hpp:9-10) on level 0: writes r
  * (if dst != NULL) and the L2 norm accumulated in double. */
 UBG_API int ubg_mg_residual(ubg_mg *mg, float h, float *r_dst, double *norm);
@@ -300,6 +304,12 @@ UBG_API double ubg_server_clock(ubg_server *srv);
  * MG(flag); h = pwidth/(W-1). */
 UBG_API int ubg_sim_create(const float *flag, int width, int height, float pwidth,
                            float mu, ubg_sim **out);
+/* Simulation(pwidth, mu, width, height) (simulation.hpp:20-30): every field
+ * zero, the flag too, no inflow column; MG(width, height) keeps its coarse
+ * flags at 1 until the flag is set (which runs updateFields).  The
+ * reference leaves the boundary conditions uninitialised; here they start as
+ * the other constructor sets them. */
+UBG_API int ubg_sim_create_blank(int width, int height, float pwidth, float mu, ubg_sim **out);
 UBG_API int ubg_sim_destroy(ubg_sim *sim);
 UBG_API int ubg_sim_set_params(ubg_sim *sim, const ubg_params *prm);
 UBG_API int ubg_sim_get_params(const ubg_sim *sim, ubg_params *out);
@@ -335,6 +345,26 @@ UBG_API int ubg_sim_get_mg(ubg_sim *sim, ubg_mg **mg);
 /* bilinearVel(c) (advect_floating_items.cpp:11-14) for n grid-space points
  * c = (x0,y0,x1,y1,...); host arrays. */
 UBG_API int ubg_sim_bilinear_vel(ubg_sim *sim, const float *c, float *out, int n);
+/* Flag samplers at n physical positions pc = (x0,y0,x1,y1,...), host arrays:
+ *   UBG_FLAG_NEAREST psampleFlagNearest (simulation.cpp:422-425): the flag of
+ *                    the cell at the truncated position pc / (pwidth / W);
+ *                    positions whose cell is outside the grid give 0 (the
+ *                    reference indexes out of bounds there);
+ *   UBG_FLAG_LINEAR  psampleFlagLinear (simulation.cpp:398-411);
+ *   UBG_FLAG_NORMAL  psampleFlagNormal (simulation.cpp:413-420), two values
+ *                    (x, y) per position in out.
+ * The game calls psampleFlagLinear at ubootgl_app.cpp:144,166. */
+enum { UBG_FLAG_NEAREST = 0, UBG_FLAG_LINEAR = 1, UBG_FLAG_NORMAL = 2 };
+UBG_API int ubg_sim_sample_flag(ubg_sim *sim, int mode, const float *pc, float *out, int n);
+/* getDT() (simulation.cpp:226-238): pwidth / (W-1) / max_vel * 2.5 with
+ * max_vel the largest of 1e-7 and vx(x,y), vy(x,y) over x in [1, W-2],
+ * y in [1, H-2] (signed values, as the reference takes them). */
+UBG_API int ubg_sim_get_dt(ubg_sim *sim, float *dt);
+/* centerP() (simulation.cpp:210-224): p -= (sum of interior p) / W / H on
+ * interior cells.  The sum is taken in float64 in a fixed order (the
+ * reference's float OpenMP reduction has no fixed order) and rounded to
+ * float before the two divisions. */
+UBG_API int ubg_sim_center_p(ubg_sim *sim);
 /* Floating items (components.hpp:6-42: CoItem + CoKinematics /
  * CoKinematicsSimple), physical coordinates. */
 typedef struct ubg_item {

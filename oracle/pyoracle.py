@@ -25,16 +25,33 @@ FIELD_NAMES = ["vx", "vx_back", "vy", "vy_back", "vx_accum", "vy_accum",
 (ST_ACCUM, ST_DIFFUSE, ST_ADVECT, ST_SET_VBCS, ST_PROJECT, ST_SET_PBC,
  ST_SAVE_CURRENT, ST_DIVERGENCE, ST_CORRECT_VELOCITY, ST_MG_SOLVE) = range(10)
 
+# the timing build of the same source (reference flags -Ofast -march=native,
+# eight-wide advect), loaded only by bench.py's cpu_baseline leg
+NATIVE_LIB_PATH = os.path.join(_HERE, "_build", "libubg_oracle_native.so")
+
 _lib = None
+_lib_path = LIB_PATH
+
+
+def use_timing_build(path=NATIVE_LIB_PATH):
+    """Switch this process to the timing build (bench.py's cpu_baseline only:
+    its -Ofast numerics make it no checker)."""
+    global _lib, _lib_path
+    _lib, _lib_path = None, path
 
 
 def lib():
     global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
-        raise RuntimeError(f"oracle library not built: {LIB_PATH} (run make -C oracle)")
-    L = C.CDLL(LIB_PATH)
+    if _lib is None:
+        _lib = load(_lib_path)
+    return _lib
+
+
+def load(path):
+    """A ctypes handle of one build of the oracle with every signature set."""
+    if not os.path.exists(path):
+        raise RuntimeError(f"oracle library not built: {path} (run make -C oracle)")
+    L = C.CDLL(path)
     sig = {
         "oracle_set_threads": (C.c_int, [C.c_int]),
         "oracle_mg_level_sizes": (C.c_int, [C.c_int, C.c_int, _i32p, _i32p, C.c_int]),
@@ -74,6 +91,12 @@ def lib():
         "oracle_sim_stage": (C.c_int, [C.c_void_p, C.c_int, C.c_float]),
         "oracle_sim_bilinear_vel": (None, [C.c_void_p, _f32p, _f32p, C.c_int]),
         "oracle_flag_linear": (C.c_float, [_f32p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float]),
+        "oracle_flag_nearest": (C.c_float, [_f32p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float]),
+        "oracle_flag_normal": (None, [_f32p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                      C.c_float, _f32p]),
+        "oracle_gs_f32": (None, [_f32p, _f32p, _f32p, C.c_int, C.c_int, C.c_float, C.c_float]),
+        "oracle_sim_center_p": (None, [C.c_void_p]),
+        "oracle_sim_get_dt": (C.c_float, [C.c_void_p]),
         "oracle_sim_advect_tracers": (None, [C.c_void_p, _f32p, _f32p, C.c_int, C.c_float, C.c_uint32]),
         "oracle_sim_advect_items": (None, [C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_int]),
         "oracle_cephes_sin": (C.c_float, [C.c_float]),
@@ -83,7 +106,6 @@ def lib():
         fn = getattr(L, name)
         fn.restype = res
         fn.argtypes = args
-    _lib = L
     return L
 
 
@@ -126,6 +148,31 @@ def residual(p, f, flag, h):
         r = np.zeros((H, W), np.float32)
         n = lib().oracle_residual_f32(f32(p), f32(f), f32(flag), r, W, H, h)
     return r, n
+
+
+def gs(p, f, flag, h, alpha=1.0):
+    """pressure_solver.cpp:26-33 (lexicographic sweep); float32 p in place."""
+    H, W = p.shape
+    lib().oracle_gs_f32(p, f32(f), f32(flag), W, H, h, alpha)
+    return p
+
+
+def flag_sample(flag, pwidth, h, mode, pts):
+    """psampleFlagNearest (mode 0), psampleFlagLinear (1), psampleFlagNormal
+    (2) (simulation.cpp:398-425) at physical positions pts (n, 2); returns
+    (n,) or (n, 2) float32."""
+    flag = f32(flag)
+    H, W = flag.shape
+    pts = np.asarray(pts, np.float32).reshape(-1, 2)
+    if mode == 2:
+        out = np.zeros((len(pts), 2), np.float32)
+        tmp = np.zeros(2, np.float32)
+        for i, (x, y) in enumerate(pts):
+            lib().oracle_flag_normal(flag, W, H, pwidth, h, float(x), float(y), tmp)
+            out[i] = tmp
+        return out
+    fn = lib().oracle_flag_nearest if mode == 0 else lib().oracle_flag_linear
+    return np.array([fn(flag, W, H, pwidth, float(x), float(y)) for x, y in pts], np.float32)
 
 
 def restrict(r, Wc, Hc):
@@ -241,6 +288,12 @@ class OracleSim:
 
     def step(self, dt):
         lib().oracle_sim_step(self.h_, dt)
+
+    def center_p(self):
+        lib().oracle_sim_center_p(self.h_)
+
+    def get_dt(self):
+        return lib().oracle_sim_get_dt(self.h_)
 
     def stage(self, st, dt):
         assert lib().oracle_sim_stage(self.h_, st, dt) == 0

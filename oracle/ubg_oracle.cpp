@@ -27,6 +27,9 @@
 //   * project                     discrete identity div(v_new) = div(v) - L p
 //   * tracers                     bilinear exact on linear fields; Wang hash and
 //                                 the Numerical Recipes LCG as integer closed forms
+//   * gs, centerP, getDT, flag    closed forms (lexicographic order on a point
+//     samplers                    source, the W*H divisor, signed maximum,
+//                                 linear ramps and the clamp / fract rules)
 // Parity unpinned: the composition of advect's quirks under the reference's
 // own -Ofast build (the reference cannot be compiled here: it needs glm and
 // GLEW headers the image lacks), and glm::mix's exact rounding (x*(1-a)+y*a is
@@ -40,6 +43,9 @@
 #include <cstring>
 #include <vector>
 #include <omp.h>
+#if defined(UBG_ORACLE_TIMING) && defined(__AVX2__)
+#include <immintrin.h>
+#endif
 
 #define ORACLE_API extern "C" __attribute__((visibility("default")))
 
@@ -79,6 +85,13 @@ void rbgs(T *p, const T *f, const T *fl, int W, int H, T h, T alpha) {
   for (int y = 1; y < H - 1; y++)
     for (int x = 1 + ((y + 1) % 2); x < W - 1; x += 2)
       p[(size_t)y * W + x] = smooth_point(p, f, fl, W, x, y, h, alpha);
+}
+
+// pressure_solver.cpp:26-33 (gs): one lexicographic Gauss-Seidel sweep
+template <typename T>
+void gs(T *p, const T *f, const T *fl, int W, int H, T h, T alpha) {
+  for (int y = 1; y < H - 1; y++)
+    for (int x = 1; x < W - 1; x++) p[(size_t)y * W + x] = smooth_point(p, f, fl, W, x, y, h, alpha);
 }
 
 // pressure_solver.cpp:91-116 (calculateResidualField); the norm is summed in
@@ -325,6 +338,82 @@ inline float bicubic(const float *g, int gw, int gh, float cx, float cy) {
   return cubic_hermite(stx, col[0], col[1], col[2], col[3]);
 }
 
+#if defined(UBG_ORACLE_TIMING) && defined(__AVX2__)
+// TIMING BUILD ONLY (bench.py's CPU baseline, compiled with the reference's
+// -Ofast -march=native, never used for parity): eight lanes of an advect
+// chunk at once, as the reference's AVX2 advect processes them
+// (simulation.cpp:241-354, interpolators.hpp:78-206).  Each bicubic sample
+// gathers the four rows of one stencil column for all lanes, runs the
+// vertical Catmull-Rom passes eight wide, then the horizontal one.
+inline __m256 cubic_hermite8(__m256 t, __m256 A, __m256 B, __m256 C, __m256 D) {
+  __m256 a = -A / 2.0f + (3.0f * B) / 2.0f - (3.0f * C) / 2.0f + D / 2.0f;
+  __m256 b = A - (5.0f * B) / 2.0f + 2.0f * C - D / 2.0f;
+  __m256 c = -A / 2.0f + C / 2.0f;
+  return a * t * t * t + b * t * t + c * t + B;
+}
+inline __m256 bicubic8(const float *g, int gw, int gh, __m256 cx, __m256 cy) {
+  cx = _mm256_max_ps(_mm256_min_ps(cx, _mm256_set1_ps(gw - 3.0f)), _mm256_set1_ps(3.0f));
+  cy = _mm256_max_ps(_mm256_min_ps(cy, _mm256_set1_ps(gh - 3.0f)), _mm256_set1_ps(3.0f));
+  const __m256i icx = _mm256_cvttps_epi32(cx), icy = _mm256_cvttps_epi32(cy);
+  const __m256 stx = cx - _mm256_round_ps(cx, _MM_FROUND_TO_ZERO | _MM_FROUND_NO_EXC);
+  const __m256 sty = cy - _mm256_round_ps(cy, _MM_FROUND_TO_ZERO | _MM_FROUND_NO_EXC);
+  const __m256i one = _mm256_set1_epi32(1), vw = _mm256_set1_epi32(gw);
+  const __m256i i0 = _mm256_add_epi32(_mm256_sub_epi32(icx, one),
+                                      _mm256_mullo_epi32(_mm256_sub_epi32(icy, one), vw));
+  __m256 col[4];
+  for (int m = 0; m < 4; m++) {
+    const __m256i r0 = _mm256_add_epi32(i0, _mm256_set1_epi32(m));
+    const __m256i r1 = _mm256_add_epi32(r0, vw), r2 = _mm256_add_epi32(r1, vw),
+                  r3 = _mm256_add_epi32(r2, vw);
+    col[m] = cubic_hermite8(sty, _mm256_i32gather_ps(g, r0, 4), _mm256_i32gather_ps(g, r1, 4),
+                            _mm256_i32gather_ps(g, r2, 4), _mm256_i32gather_ps(g, r3, 4));
+  }
+  return cubic_hermite8(stx, col[0], col[1], col[2], col[3]);
+}
+inline __m256 lanes_x(int x0) {
+  return _mm256_add_ps(_mm256_set1_ps((float)x0), _mm256_setr_ps(0, 1, 2, 3, 4, 5, 6, 7));
+}
+// the vx chunk (IS_VY false) or vy chunk starting at x0 in row y
+template <bool IS_VY>
+inline void advect_chunk8(const float *xf, const float *yf, float *xb, float *yb, const float *F,
+                          int W, int H, int x0, int y, float hdt, float fdt) {
+  const int Wx = W - 1, Hy = H - 1;
+  const __m256 two = _mm256_set1_ps(2.0f), half = _mm256_set1_ps(0.5f);
+  const size_t fr = (size_t)y * W + x0;
+  const __m256 fa = _mm256_loadu_ps(F + fr - (IS_VY ? (size_t)W : 1)), fb = _mm256_loadu_ps(F + fr);
+  if (_mm256_movemask_ps(_mm256_cmp_ps(_mm256_add_ps(fa, fb), two, _CMP_EQ_OQ)) == 0) return;
+  const __m256 vh = _mm256_set1_ps(hdt), vf = _mm256_set1_ps(fdt);
+  if (!IS_VY) {
+    const __m256 posx = lanes_x(x0) + half, posy = _mm256_set1_ps((float)y);
+    const __m256 u = _mm256_loadu_ps(xf + (size_t)y * Wx + x0);
+    const __m256 w = ((_mm256_loadu_ps(yf + (size_t)y * W + x0) +
+                       _mm256_loadu_ps(yf + (size_t)(y - 1) * W + x0)) +
+                      (_mm256_loadu_ps(yf + (size_t)y * W + x0 + 1) +
+                       _mm256_loadu_ps(yf + (size_t)(y - 1) * W + x0 + 1))) * 0.25f;
+    const __m256 mx = posx - u * vh, my = posy - w * vh;
+    const __m256 u2 = bicubic8(xf, Wx, H, mx - half, my), w2 = bicubic8(yf, W, Hy, mx, my - half);
+    const __m256 ex = posx - u2 * vf, ey = posy - w2 * vf;
+    __m256 v = bicubic8(xf, Wx, H, ex - half, ey);
+    v = v * _mm256_loadu_ps(F + fr) * _mm256_loadu_ps(F + fr + 1);
+    _mm256_storeu_ps(xb + (size_t)Wx * y + x0, v);
+  } else {
+    const __m256 posx = lanes_x(x0), posy = _mm256_set1_ps((float)y + 0.5f);
+    const __m256 w = _mm256_loadu_ps(yf + (size_t)y * W + x0);
+    const __m256 u = ((_mm256_loadu_ps(xf + (size_t)y * Wx + x0) +
+                       _mm256_loadu_ps(xf + (size_t)(y - 1) * Wx + x0)) +
+                      (_mm256_loadu_ps(xf + (size_t)y * Wx + x0 + 1) +
+                       _mm256_loadu_ps(xf + (size_t)(y - 1) * Wx + x0 + 1))) * 0.25f;
+    const __m256 mx = posx - u * vh, my = posy - w * vh;
+    const __m256 u2 = bicubic8(xf, Wx, H, mx - half, my), w2 = bicubic8(yf, W, Hy, mx, my - half);
+    const __m256 ex = posx - u2 * vf, ey = posy - w2 * vf;
+    __m256 v = bicubic8(yf, W, Hy, ex, ey - half);
+    v = v * _mm256_loadu_ps(F + fr) * _mm256_loadu_ps(F + fr + W);
+    _mm256_storeu_ps(yb + (size_t)W * y + x0, v);
+  }
+}
+#define UBG_ORACLE_AVX2_ADVECT 1
+#endif
+
 // interpolators.hpp:11-27 (scalar bilinearSample with clamp to [0, n-1.1])
 inline float bilinear(const float *g, int gw, int gh, float cx, float cy) {
   cx = std::min(std::max(cx, 0.0f), gw - 1.1f);
@@ -518,6 +607,18 @@ struct Sim {
     const float *F = flag.data();
     const float *xf = vxf(), *yf = vyf();
     float *xb = vxb(), *yb = vyb();
+#ifdef UBG_ORACLE_AVX2_ADVECT
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int y = 1; y < H - 1; y++) {
+      for (int x0 = 1; x0 < Wx - 8; x0 += 8)
+        advect_chunk8<false>(xf, yf, xb, yb, F, W, H, x0, y, hdt, fdt);
+      for (int x0 = 1; x0 < W - 8; x0 += 8)
+        advect_chunk8<true>(xf, yf, xb, yb, F, W, H, x0, y, hdt, fdt);
+    }
+    fx = 1 - fx;
+    fy = 1 - fy;
+    return;
+#endif
 #pragma omp parallel for schedule(dynamic, 16)
     for (int y = 1; y < H - 1; y++) {
       for (int x0 = 1; x0 < Wx - 8; x0 += 8) {
@@ -628,6 +729,31 @@ struct Sim {
   void saveCurrent() {
     std::copy(vx[fx].begin(), vx[fx].end(), vx_cur.begin());
     std::copy(vy[fy].begin(), vy[fy].end(), vy_cur.begin());
+  }
+
+  // simulation.cpp:210-224 (centerP).  The reference's float OpenMP
+  // reduction has no fixed order; this sums row by row in float, the order
+  // of a one-thread run.
+  void centerP() {
+    float avgP = 0;
+    for (int y = 1; y < H - 1; y++)
+      for (int x = 1; x < W - 1; x++) avgP += p[(size_t)y * W + x];
+    for (int y = 1; y < H - 1; y++)
+      for (int x = 1; x < W - 1; x++) p[(size_t)y * W + x] -= avgP / W / H;
+  }
+
+  // simulation.cpp:226-238 (getDT): signed maximum of vx(x, y) and vy(x, y)
+  // over x in [1, W-2], y in [1, H-2], starting at 1e-7
+  float getDT() {
+    float max_vel = 1.0e-7f;
+    const float *xf = vxf(), *yf = vyf();
+    for (int y = 1; y < H - 1; y++)
+      for (int x = 1; x < W - 1; x++) {
+        const float a = xf[(size_t)y * Wx + x], b = yf[(size_t)y * W + x];
+        const float m = (a < b) ? b : a;
+        max_vel = (max_vel < m) ? m : max_vel;
+      }
+    return pwidth / (W - 1.0f) / max_vel * 2.5f;
   }
 
   // simulation.cpp:356-374
@@ -1170,6 +1296,32 @@ ORACLE_API void oracle_sim_bilinear_vel(void *sp, const float *c, float *out, in
     out[2 * i + 1] = bilinear(s->vyf(), s->W, s->Hy, c[2 * i], c[2 * i + 1] - 0.5f);
   }
 }
+
+// simulation.cpp:422-425 (psampleFlagNearest): the flag at the truncated
+// grid position; a cell outside the grid gives 0 (the reference indexes out
+// of bounds there)
+ORACLE_API float oracle_flag_nearest(const float *F, int W, int H, float pwidth, float px,
+                                     float py) {
+  const float gx = px / (pwidth / W), gy = py / (pwidth / W);
+  if (!(gx > -1.0f && gx < (float)W && gy > -1.0f && gy < (float)H)) return 0.0f;
+  return F[(size_t)(int)gy * W + (int)gx];
+}
+// simulation.cpp:413-420 (psampleFlagNormal)
+ORACLE_API void oracle_flag_normal(const float *F, int W, int H, float pwidth, float h, float px,
+                                   float py, float *out) {
+  const float p01 = flag_linear(F, W, H, pwidth, px - h, py + h);
+  const float p11 = flag_linear(F, W, H, pwidth, px + h, py + h);
+  const float p00 = flag_linear(F, W, H, pwidth, px - h, py - h);
+  const float p10 = flag_linear(F, W, H, pwidth, px + h, py - h);
+  out[0] = p11 + p10 - p01 - p00;
+  out[1] = p01 + p11 - p00 - p10;
+}
+ORACLE_API void oracle_gs_f32(float *p, const float *f, const float *fl, int W, int H, float h,
+                              float alpha) {
+  gs(p, f, fl, W, H, h, alpha);
+}
+ORACLE_API void oracle_sim_center_p(void *sp) { ((Sim *)sp)->centerP(); }
+ORACLE_API float oracle_sim_get_dt(void *sp) { return ((Sim *)sp)->getDT(); }
 
 ORACLE_API float oracle_flag_linear(const float *F, int W, int H, float pwidth, float px,
                                     float py) {

@@ -18,6 +18,13 @@ int oracle_sim_get_field(void *s, int id, float *dst);
 void oracle_sim_step(void *s, float dt);
 void oracle_sim_advect_items(void *s, void *items, int n, float gameDT, int simple);
 void *oracle_mg_create(int W, int H, int fp64);
+float oracle_flag_linear(const float *F, int W, int H, float pwidth, float px, float py);
+float oracle_flag_nearest(const float *F, int W, int H, float pwidth, float px, float py);
+void oracle_flag_normal(const float *F, int W, int H, float pwidth, float h, float px, float py,
+                        float *out);
+void oracle_gs_f32(float *p, const float *f, const float *fl, int W, int H, float h, float alpha);
+void oracle_sim_center_p(void *s);
+float oracle_sim_get_dt(void *s);
 void oracle_mg_update_fields(void *m, const float *flag);
 void oracle_mg_solve(void *m, float *p, const float *f, const float *flag, float h, int zg);
 }
@@ -47,11 +54,21 @@ struct CoKinematics {
   float angVel, angForce;
   int bumpCount;
 };
-struct MiniRegistry {
+struct CoKinematicsSimple {
+  float mass;
+  V2 vel, force;
+  float angVel, angForce;
+  int bumpCount;
+};
+// glm::vec3 stand-in for the sinks list (explosion.cpp:33 pushes glm::vec3)
+struct V3 {
+  float x, y, z;
+};
+template <class Kin> struct MiniRegistryT {
   std::vector<CoItem> items;
-  std::vector<CoKinematics> kins;
+  std::vector<Kin> kins;
   struct View {
-    MiniRegistry *r;
+    MiniRegistryT *r;
     struct It {
       int i;
       int operator*() const { return i; }
@@ -70,6 +87,7 @@ struct MiniRegistry {
   };
   template <class A, class B> View view() { return View{this}; }
 };
+using MiniRegistry = MiniRegistryT<CoKinematics>;
 
 static int fails = 0;
 static void expect(bool ok, const char *what, double v) {
@@ -186,7 +204,109 @@ int main() {
     sim.setGrids(Simulation::Vec2{11.0f, 10.0f}, 0.0f);
     sim.syncToHost();
     expect(sim.flag(11, 10) == 0.0f && sim.vx.f(10, 10) == 0.0f, "setGrids / updateFields", 0.0);
+    // the rest of the reference's public Simulation interface
+    // (simulation.hpp:20-140), called as the game calls it
+    {
+      sim.syncToHost();
+      oracle_sim_set_field(os, 0, sim.vx.data());
+      oracle_sim_set_field(os, 2, sim.vy.data());
+      oracle_sim_set_field(os, 8, sim.p.data());
+      float dtg = sim.getDT(), dto = oracle_sim_get_dt(os);
+      expect(dtg == dto, "getDT matches the oracle", std::fabs(dtg - dto));
+      int bad = 0;
+      for (int i = 0; i < 200; i++) {
+        V2 pc{(i * 0.37f - 3.0f) / W, (i * 0.21f - 2.0f) / W};
+        float fl = sim.psampleFlagLinear(pc), fn = sim.psampleFlagNearest(pc);
+        V2 nrm = sim.psampleFlagNormal(pc);
+        float on[2];
+        oracle_flag_normal(sim.flag.data(), W, H, sim.pwidth, sim.h, pc.x, pc.y, on);
+        bad += fl != oracle_flag_linear(sim.flag.data(), W, H, sim.pwidth, pc.x, pc.y);
+        bad += fn != oracle_flag_nearest(sim.flag.data(), W, H, sim.pwidth, pc.x, pc.y);
+        bad += nrm.x != on[0] || nrm.y != on[1];
+        // ubootgl_app.cpp:144 / :166
+        if (sim.psampleFlagLinear(pc) < 0.001) bad += fl >= 0.001f;
+      }
+      expect(bad == 0, "psampleFlagLinear / Nearest / Normal match the oracle", bad);
+      sim.centerP();
+      oracle_sim_center_p(os);
+      sim.syncToHost();
+      std::vector<float> op2((size_t)W * H);
+      oracle_sim_get_field(os, 8, op2.data());
+      double rc = rel(sim.p.data(), op2.data(), op2.size());
+      expect(rc < 1e-6, "centerP matches the oracle", rc);
+      expect(sim.singlePBC(Simulation::BC::OUTFLOW_ZERO_PRESSURE, 2.0f) == -2.0f &&
+                 sim.singlePBC(Simulation::BC::NOSLIP, 2.0f) == 2.0f &&
+                 sim.VBCPar(Simulation::BC::OUTFLOW, -1.0f, 5.0f) == 0.0f &&
+                 sim.VBCPar(Simulation::BC::INFLOW, -1.0f, 5.0f) == 5.0f &&
+                 sim.VBCPer(Simulation::BC::NOSLIP, 3.0f, 5.0f) == -3.0f,
+             "singlePBC / VBCPar / VBCPer", 0.0);
+      sim.sinks.push_back(V3{0.4f, 0.25f, 120.0f}); // explosion.cpp:33
+      sim.project();
+      expect(sim.sinks.size() == 1 && sim.sinks[0].z < 120.0f, "sinks list (glm::vec3 push)",
+             sim.sinks.empty() ? 0.0 : sim.sinks[0].z);
+      sim.sinks.clear();
+      // advect_floating_items.cpp:16, :148 with the reference's signatures:
+      // the same items as the explicit view calls (the item physics reads
+      // only the fluid fields, which the calls do not change)
+      MiniRegistry reg, reg2;
+      MiniRegistryT<CoKinematicsSimple> regs, regs2;
+      for (int i = 0; i < 8; i++) {
+        reg.items.push_back(CoItem{{0.01f, 0.003f}, {(20.0f + 5 * i) / W, 20.0f / W}, 0.0f});
+        reg.kins.push_back(CoKinematics{1.0f, {0.01f, -0.004f * i}, {0.0f, 0.0f}, 0.1f, 0.0f, 0});
+        regs.items.push_back(reg.items.back());
+        regs.kins.push_back(
+            CoKinematicsSimple{1.0f, {0.01f, -0.004f * i}, {0.0f, 0.0f}, 0.1f, 0.0f, 0});
+      }
+      reg2 = reg;
+      regs2 = regs;
+      sim.advectFloatingItems(reg, 1.0f / 60);
+      sim.advectFloatingItemsView<CoItem, CoKinematics>(reg2, 1.0f / 60, false);
+      sim.advectFloatingItemsSimple(regs, 1.0f / 60);
+      sim.advectFloatingItemsView<CoItem, CoKinematicsSimple>(regs2, 1.0f / 60, true);
+      int same = 0;
+      for (int i = 0; i < 8; i++) {
+        same += reg.items[i].pos.x == reg2.items[i].pos.x && reg.items[i].pos.y == reg2.items[i].pos.y;
+        same += regs.items[i].pos.x == regs2.items[i].pos.x && regs.items[i].pos.y == regs2.items[i].pos.y;
+      }
+      expect(same == 16 && reg.items[0].pos.x != 20.0f / W,
+             "advectFloatingItems / Simple (registry)", same);
+      float u = sim.disx(sim.gen), v = sim.disy(sim.gen);
+      expect(u >= 0.0f && u <= 1.0f && v >= 0.0f && v <= (float)H / W, "gen / disx / disy", u);
+      float *fp = sim.getFlag(), *pp = sim.getP(), *rp = sim.getR();
+      expect(fp == sim.flag.data() && pp == sim.p.data() && rp == sim.r.data(),
+             "getFlag / getP / getR", 0.0);
+      sim.applyAccumulatedVelocity();
+      sim.diffuse();
+      sim.advect();
+      sim.setVBCs();
+      sim.setPBC();
+      sim.saveCurrentVelocityFields();
+    }
     oracle_sim_destroy(os);
+  }
+  // 3) Simulation(pwidth, mu, width, height) (simulation.hpp:20-30) and the
+  //    free gs (pressure_solver.hpp:3-4)
+  {
+    const int W = 40, H = 24;
+    Simulation blank((W - 1.0f) / W, 0.001f, W, H);
+    blank.syncToHost();
+    double s = 0;
+    for (int i = 0; i < (W - 1) * H; i++) s += std::fabs(blank.vx.data()[i]);
+    expect(s == 0.0 && blank.flag(3, 3) == 0.0f, "Simulation(pwidth, mu, W, H) starts zero", s);
+    Single2DGrid p(W, H), f(W, H), flag(W, H);
+    for (int y = 0; y < H; y++)
+      for (int x = 0; x < W; x++) {
+        flag(x, y) = (x == 0 || y == 0 || x == W - 1 || y == H - 1 || (x * 7 + y * 3) % 11 == 0)
+                         ? 0.0f
+                         : 1.0f;
+        p(x, y) = std::sin(0.3f * x) + std::cos(0.2f * y);
+        f(x, y) = 0.01f * ((x + 2 * y) % 5);
+      }
+    std::vector<float> po(p.data(), p.data() + W * H);
+    gs(p, f, flag, 1.0f / W, 1.0f);
+    oracle_gs_f32(po.data(), f.data(), flag.data(), W, H, 1.0f / W, 1.0f);
+    expect(rel(p.data(), po.data(), po.size()) == 0.0, "gs matches the oracle bit for bit",
+           rel(p.data(), po.data(), po.size()));
   }
   std::printf("%s\n", fails ? "FAILED" : "ALL OK");
   return fails ? 1 : 0;

@@ -387,3 +387,41 @@ def test_error_codes(gpu):
     with pytest.raises(U.UbgError) as e:
         U.Simulation(bad, 0.9, 0.001)
     assert e.value.code == L.E_FLAG_VALUE
+
+
+def test_advect_flat_loads_bit_exact(gpu, monkeypatch):
+    """advect's 64-bit flat-load path (used for velocity grids of 2 GiB or
+    more) forced on a small grid: bit-identical to the oracle, as the
+    buffer-load path is."""
+    monkeypatch.setenv("UBG_ADVECT_FLAT_LOADS", "1")
+    d = workloads.small_domain(96, 48, seed=51, ndisks=3)
+    sim, ref = make_pair(d)
+    for _ in range(2):
+        sim.step(float(d.dt))
+        ref.step(float(d.dt))
+    sim.stage(L.STAGE_ADVECT, float(d.dt))
+    ref.stage(L.STAGE_ADVECT, float(d.dt))
+    compare(sim, ref)
+
+
+@pytest.mark.parametrize("rows", list(range(8, 33, 2)))
+def test_tile_pass_row_heights(gpu, monkeypatch, rows):
+    """Every tile row height the LDS level passes can pick (even, 8..32),
+    forced on an odd-sized hierarchy whose levels all run as tiles: three
+    V-cycles bit-identical to the oracle's."""
+    monkeypatch.setenv("UBG_TILE_ROWS", str(rows))
+    Wd, Hd = 1001, 603
+    g = np.random.default_rng(52)
+    flag = rand_flag(Wd, Hd, 53, 0.1)
+    f = g.standard_normal((Hd, Wd)).astype(np.float32)
+    h = float(np.float32(1.0 / Wd))
+    mg = U.MG(flag)
+    om = O.OracleMG(Wd, Hd)
+    om.update_fields(flag)
+    pg = np.zeros((Hd, Wd), np.float32)
+    pr = pg.copy()
+    for _ in range(3):
+        mg.solve(pg, f, flag, h, False)
+        om.solve(pr, f, flag, h, False)
+    np.testing.assert_array_equal(pg, pr)
+    mg.close()

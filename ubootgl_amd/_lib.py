@@ -35,6 +35,7 @@ FIELD_NAMES = ["vx", "vx_back", "vy", "vy_back", "vx_accum", "vy_accum", "vx_cur
 MGOP_RBGS, MGOP_ZERO_GRADIENT, MGOP_RESIDUAL, MGOP_RESTRICT_RESIDUAL, MGOP_PROLONG_CORRECT = range(5)
 LEVEL_P, LEVEL_F, LEVEL_R = range(3)
 TIMER_SMOOTH_L0, TIMER_L0_PRE, TIMER_L0_POST, TIMER_ADVECT = range(4)
+FLAG_NEAREST, FLAG_LINEAR, FLAG_NORMAL = range(3)
 
 
 # ubg_item (components.hpp:6-42): CoItem + CoKinematics
@@ -119,6 +120,7 @@ SIGNATURES = {
     "ubg_mg_solve": (C.c_int, [_vp, C.c_float, C.c_int, C.c_int]),
     "ubg_mg_solve_host": (C.c_int, [_vp, _fp, _fp, _fp, C.c_float, C.c_int]),
     "ubg_mg_rbgs": (C.c_int, [_vp, C.c_float, C.c_float, C.c_int]),
+    "ubg_mg_gs": (C.c_int, [_vp, C.c_float, C.c_float, C.c_int]),
     "ubg_mg_residual": (C.c_int, [_vp, C.c_float, _fp, _dp]),
     "ubg_mg_sync": (C.c_int, [_vp]),
     "ubg_ir_params_default": (None, [C.POINTER(IRParams)]),
@@ -169,6 +171,10 @@ SIGNATURES = {
     "ubg_sim_get_sinks": (C.c_int, [_vp, _fp, C.c_int, _ip]),
     "ubg_sim_get_mg": (C.c_int, [_vp, C.POINTER(_vp)]),
     "ubg_sim_bilinear_vel": (C.c_int, [_vp, _fp, _fp, C.c_int]),
+    "ubg_sim_create_blank": (C.c_int, [C.c_int, C.c_int, C.c_float, C.c_float, C.POINTER(_vp)]),
+    "ubg_sim_sample_flag": (C.c_int, [_vp, C.c_int, _fp, _fp, C.c_int]),
+    "ubg_sim_get_dt": (C.c_int, [_vp, _fp]),
+    "ubg_sim_center_p": (C.c_int, [_vp]),
     "ubg_sim_collocate": (C.c_int, [_vp, _fp, _fp, C.c_int]),
     "ubg_sim_shift_map": (C.c_int, [_vp, _fp]),
     "ubg_sim_advect_items": (C.c_int, [_vp, _vp, C.c_int, C.c_float, C.c_int]),

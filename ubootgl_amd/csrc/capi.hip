@@ -10,6 +10,8 @@
 // once per (front-buffer parity, dt) and replayed as a hipGraph.
 #include "capi_common.hpp"
 
+#include <unordered_map>
+
 namespace ubg {
 
 int mg_create(int W, int H, hipStream_t stream, float *p0, float *p0_alt, float *f0,
@@ -277,6 +279,16 @@ UBG_API int ubg_mg_rbgs(ubg_mg *m, float h, float alpha, int nsweeps) {
   return UBG_OK;
 }
 
+UBG_API int ubg_mg_gs(ubg_mg *m, float h, float alpha, int nsweeps) {
+  if (!m) return fail(UBG_E_INVALID, "null mg");
+  if (nsweeps < 0) return fail(UBG_E_INVALID, "negative sweep count");
+  std::lock_guard<std::recursive_mutex> g(m->mu);
+  const Level &L = m->lv[0];
+  for (int i = 0; i < nsweeps; i++) launch_gs(L.p, L.f, L.mask, L.w, L.h, h, alpha, m->stream);
+  UBG_TRY(check_launch());
+  return UBG_OK;
+}
+
 UBG_API int ubg_mg_residual(ubg_mg *m, float h, float *r_dst, double *norm) {
   if (!m) return fail(UBG_E_INVALID, "null mg");
   std::lock_guard<std::recursive_mutex> g(m->mu);
@@ -447,17 +459,30 @@ struct ubg_sim {
     vy_bc_fresh = false;
   }
 
-  // sinks (simulation.cpp:173-187): stamps resolved on the host
+  // sinks (simulation.cpp:173-187), stamped in list order: where the 3x3
+  // patches of several sinks overlap the last one wins, so each cell's final
+  // value is settled here and the device writes every cell once.  A patch
+  // cell is (int)(g + d) of the float centre g, as the reference indexes it.
   int apply_sinks() {
     if (sinks.empty()) return UBG_OK;
-    std::vector<int> xy;
+    std::vector<int> cells;
     std::vector<float> val;
+    std::unordered_map<long long, size_t> slot;
     for (auto &s : sinks) {
       float gx = s.x / h + 0.5f, gy = s.y / h + 0.5f;
       if (gx <= 3 || gx > W - 3 || gy <= 3 || gy > H - 3) continue;
-      xy.push_back((int)gx);
-      xy.push_back((int)gy);
-      val.push_back(s.z);
+      for (int yy = -1; yy <= 1; yy++)
+        for (int xx = -1; xx <= 1; xx++) {
+          const long long c = (long long)(int)(gy + yy) * W + (int)(gx + xx);
+          auto it = slot.find(c);
+          if (it == slot.end()) {
+            slot.emplace(c, val.size());
+            cells.push_back((int)c);
+            val.push_back(s.z);
+          } else {
+            val[it->second] = s.z;
+          }
+        }
       s.z = (float)(s.z * std::pow(0.000001, (double)(dt * 50)));
     }
     sinks.erase(std::remove_if(sinks.begin(), sinks.end(),
@@ -468,13 +493,13 @@ struct ubg_sim {
     if (n > sink_cap) {
       if (d_sink_xy) (void)hipFree(d_sink_xy);
       if (d_sink_val) (void)hipFree(d_sink_val);
-      sink_cap = std::max(n, 64);
-      UBG_HIP(hipMalloc((void **)&d_sink_xy, sizeof(int) * 2 * sink_cap));
+      sink_cap = std::max(n, 576);
+      UBG_HIP(hipMalloc((void **)&d_sink_xy, sizeof(int) * sink_cap));
       UBG_HIP(hipMalloc((void **)&d_sink_val, sizeof(float) * sink_cap));
     }
-    UBG_HIP(hipMemcpyAsync(d_sink_xy, xy.data(), sizeof(int) * 2 * n, hipMemcpyHostToDevice, stream));
+    UBG_HIP(hipMemcpyAsync(d_sink_xy, cells.data(), sizeof(int) * n, hipMemcpyHostToDevice, stream));
     UBG_HIP(hipMemcpyAsync(d_sink_val, val.data(), sizeof(float) * n, hipMemcpyHostToDevice, stream));
-    launch_stamp_sinks(f, W, d_sink_xy, d_sink_val, n, stream);
+    launch_stamp_sinks(f, d_sink_xy, d_sink_val, n, stream);
     // the host arrays must outlive the async copies
     UBG_HIP(hipStreamSynchronize(stream));
     return UBG_OK;
@@ -599,18 +624,21 @@ struct ubg_sim {
   }
 };
 
-extern "C" {
-
-UBG_API int ubg_sim_create(const float *flag, int width, int height, float pwidth, float mu,
-                           ubg_sim **out) {
-  if (!flag || !out) return fail(UBG_E_INVALID, "null argument");
+// Simulation(flag, pwidth, mu) (simulation.hpp:32-67), or with blank = true
+// Simulation(pwidth, mu, width, height) (simulation.hpp:20-30): every field
+// zero (the flag too), no inflow column, and MG(width, height), whose coarse
+// flags stay 1 until updateFields.
+static int sim_create(const float *flag, int width, int height, float pwidth, float mu,
+                      bool blank, ubg_sim **out) {
+  if ((!flag && !blank) || !out) return fail(UBG_E_INVALID, "null argument");
   if (width < 8 || height < 8)
     return fail(UBG_E_DOMAIN_TOO_SMALL, "Simulation needs at least 8 cells in each dimension");
   int ndev = 0;
   ubg_device_count(&ndev);
   if (ndev <= 0) return fail(UBG_E_NO_DEVICE, "no HIP device visible");
   std::vector<uint8_t> f8;
-  UBG_TRY(validate_flag(flag, (size_t)width * height, f8));
+  if (blank) f8.assign((size_t)width * height, 0);
+  else UBG_TRY(validate_flag(flag, (size_t)width * height, f8));
   auto *s = new ubg_sim();
   s->W = width;
   s->H = height;
@@ -647,11 +675,14 @@ UBG_API int ubg_sim_create(const float *flag, int width, int height, float pwidt
     delete s;
     return fail(UBG_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
-  // simulation.hpp:55-60: vy(x,0) = 0 and vx(0,y) = 1 in both buffers
-  std::vector<float> col((size_t)(width - 1) * height, 0.0f);
-  for (int y = 0; y < height; y++) col[(size_t)y * (width - 1)] = 1.0f;
-  for (int b = 0; b < 2; b++)
-    (void)hipMemcpyAsync(s->vx[b], col.data(), col.size() * 4, hipMemcpyHostToDevice, s->stream);
+  // simulation.hpp:55-60: vy(x,0) = 0 and vx(0,y) = 1 in both buffers (the
+  // pool starts zeroed)
+  if (!blank) {
+    std::vector<float> col((size_t)(width - 1) * height, 0.0f);
+    for (int y = 0; y < height; y++) col[(size_t)y * (width - 1)] = 1.0f;
+    for (int b = 0; b < 2; b++)
+      (void)hipMemcpyAsync(s->vx[b], col.data(), col.size() * 4, hipMemcpyHostToDevice, s->stream);
+  }
   (void)hipMemcpyAsync(s->flag, f8.data(), nc, hipMemcpyHostToDevice, s->stream);
   (void)hipStreamSynchronize(s->stream);
   rc = mg_create(width, height, s->stream, s->p_buf[0], s->p_buf[1], s->f, s->flag, s->mask,
@@ -663,13 +694,27 @@ UBG_API int ubg_sim_create(const float *flag, int width, int height, float pwidt
     return rc;
   }
   s->mg->timers = s->timers;
-  s->mg->classify(); // MG(flag) -> updateFields (simulation.hpp:62)
+  if (blank) // MG(w, h): the level-0 mask of the zero flag, coarse flags 1
+    launch_build_mask(s->flag, s->mask, width, height, s->stream);
+  else
+    s->mg->classify(); // MG(flag) -> updateFields (simulation.hpp:62)
   rc = check_launch();
   if (rc) return rc;
-  s->h = pwidth / (width - 1.0f); // simulation.hpp:63
+  s->h = pwidth / (width - 1.0f); // simulation.hpp:29, :63
   UBG_HIP(hipStreamSynchronize(s->stream));
   *out = s;
   return UBG_OK;
+}
+
+extern "C" {
+
+UBG_API int ubg_sim_create(const float *flag, int width, int height, float pwidth, float mu,
+                           ubg_sim **out) {
+  return sim_create(flag, width, height, pwidth, mu, false, out);
+}
+
+UBG_API int ubg_sim_create_blank(int width, int height, float pwidth, float mu, ubg_sim **out) {
+  return sim_create(nullptr, width, height, pwidth, mu, true, out);
 }
 
 UBG_API int ubg_sim_destroy(ubg_sim *s) {
@@ -922,6 +967,69 @@ UBG_API int ubg_sim_bilinear_vel(ubg_sim *s, const float *c, float *out, int n) 
   if (e == hipSuccess) e = es;
   if (rc) return rc;
   if (e != hipSuccess) return fail(UBG_E_HIP, hipGetErrorString(e));
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_sample_flag(ubg_sim *s, int mode, const float *pc, float *out, int n) {
+  if (!s || !pc || !out || n < 0) return fail(UBG_E_INVALID, "bad argument");
+  if (mode < UBG_FLAG_NEAREST || mode > UBG_FLAG_NORMAL) return fail(UBG_E_INVALID, "unknown mode");
+  if (n == 0) return UBG_OK;
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  const int nout = mode == UBG_FLAG_NORMAL ? 2 * n : n;
+  float *d = nullptr;
+  UBG_TRY(s->staging(sizeof(float) * (2 * (size_t)n + nout), (void **)&d));
+  UBG_HIP(hipMemcpyAsync(d, pc, sizeof(float) * 2 * n, hipMemcpyHostToDevice, s->stream));
+  launch_sample_flag(s->flag, s->W, s->H, s->pwidth, s->h, mode, d, d + 2 * n, n, s->stream);
+  int rc = check_launch();
+  hipError_t e = hipMemcpyAsync(out, d + 2 * n, sizeof(float) * nout, hipMemcpyDeviceToHost,
+                                s->stream);
+  const hipError_t es = hipStreamSynchronize(s->stream);
+  if (e == hipSuccess) e = es;
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(UBG_E_HIP, hipGetErrorString(e));
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_get_dt(ubg_sim *s, float *dt) {
+  if (!s || !dt) return fail(UBG_E_INVALID, "null argument");
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  const size_t n = (size_t)(s->W - 2) * (s->H - 2);
+  const int nb = (int)std::min<size_t>(1024, (n + 255) / 256);
+  float *d = nullptr;
+  UBG_TRY(s->staging(sizeof(float) * nb, (void **)&d));
+  launch_max_vel(s->vel(), d, nb, s->stream);
+  UBG_TRY(check_launch());
+  std::vector<float> part(nb);
+  UBG_HIP(hipMemcpyAsync(part.data(), d, sizeof(float) * nb, hipMemcpyDeviceToHost, s->stream));
+  UBG_HIP(hipStreamSynchronize(s->stream));
+  // simulation.cpp:227-234: the scan starts at 1e-7; glm::max keeps the
+  // larger value, so the order of the partial maxima does not matter
+  float max_vel = 1.0e-7f;
+  for (float v : part)
+    if (max_vel < v) max_vel = v;
+  *dt = s->pwidth / (s->W - 1.0f) / max_vel * 2.5f; // simulation.cpp:236
+  return UBG_OK;
+}
+
+UBG_API int ubg_sim_center_p(ubg_sim *s) {
+  if (!s) return fail(UBG_E_INVALID, "null sim");
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  const size_t n = (size_t)(s->W - 2) * (s->H - 2);
+  const int nb = (int)std::min<size_t>(1024, (n + 255) / 256);
+  double *d = nullptr;
+  UBG_TRY(s->staging(sizeof(double) * nb, (void **)&d));
+  launch_sum_interior(s->P(), s->W, s->H, d, nb, s->stream);
+  UBG_TRY(check_launch());
+  std::vector<double> part(nb);
+  UBG_HIP(hipMemcpyAsync(part.data(), d, sizeof(double) * nb, hipMemcpyDeviceToHost, s->stream));
+  UBG_HIP(hipStreamSynchronize(s->stream));
+  double sum = 0.0;
+  for (double v : part) sum += v;
+  // simulation.cpp:218-223: the float sum divided by width, then height
+  const float avgP = (float)sum;
+  launch_sub_interior(s->P(), s->W, s->H, avgP / s->W / s->H, s->stream);
+  UBG_TRY(check_launch());
+  UBG_HIP(hipStreamSynchronize(s->stream));
   return UBG_OK;
 }
 

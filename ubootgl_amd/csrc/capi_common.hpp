@@ -65,6 +65,9 @@ struct Pool {
     size = bytes;
     UBG_HIP(hipMalloc((void **)&base, bytes));
     UBG_HIP(hipMemset(base, 0, bytes));
+    // hipMemset runs on the null stream, which the handles' non-blocking
+    // streams do not wait for: finish it before any of them writes the pool
+    UBG_HIP(hipStreamSynchronize(nullptr));
     return UBG_OK;
   }
   template <typename T> T *take(size_t n) {

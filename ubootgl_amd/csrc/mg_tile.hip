@@ -3,6 +3,8 @@
 // per tile (tile bodies in tile_device.hpp).
 #include "tile_device.hpp"
 
+#include <cstdlib>
+
 namespace ubg {
 namespace {
 
@@ -48,11 +50,17 @@ struct TileGrid {
 // one residency wave of `target` workgroups (a second, partial wave costs
 // a whole tile lifetime: 2048x1024 with 20 rows, 936 tiles, beats 16 rows,
 // 1152 tiles, by ~6% of the level's time).
+// UBG_TILE_ROWS (an even number 8..32) overrides the rule: a test switch, so
+// every row height the rule can pick is checked on one level.
 TileGrid tile_grid(int w, int h, int nu, int target) {
   TileGrid t;
   tile_cols(w, nu, t.otx, t.ntx);
   int oty = 8;
   while (oty < 32 && (long)t.ntx * ((h + oty - 1) / oty) > target) oty += 2;
+  if (const char *e = std::getenv("UBG_TILE_ROWS")) {
+    const int r = std::atoi(e);
+    if (r >= 8 && r <= 32 && r % 2 == 0) oty = r;
+  }
   t.oty = oty;
   t.nty = (h + oty - 1) / oty;
   return t;

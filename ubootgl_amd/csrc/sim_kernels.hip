@@ -3,7 +3,9 @@
 // semi-Lagrangian RK2 advection with Catmull-Rom sampling, divergence,
 // velocity correction, bilinear velocity sampling and tracer advection.
 // Memory-bound gathers and stencils; one thread per face / cell / point.
-#include "ubg_internal.hpp"
+#include "mg_device.hpp"
+
+#include <cstdlib>
 
 namespace ubg {
 
@@ -256,11 +258,26 @@ __global__ void k_diffuse_both(const float *__restrict__ sx, float *__restrict__
   }
 }
 
-// CubicHermite (interpolators.hpp:78-85), Catmull-Rom
+// CubicHermite (interpolators.hpp:78-85), Catmull-Rom, evaluated as
+//   a = -A/2 + (3B)/2 - (3C)/2 + D/2,  b = A - (5B)/2 + 2C - D/2,
+//   c = -A/2 + C/2,  a t t t + b t t + c t + B,
+// each operation rounded as written, left to right.  A halving or a
+// doubling is exact (a power-of-two scaling of a normal number), and fma(x,
+// k, s) rounds x*k + s once, which for an exact product x*k is the rounding
+// of the written sum: so each "(...)/2 +" and "2C +" term is one fma with the
+// written result.  (Only a product that leaves the normal range -- a
+// nonzero input below 2^-125 or above 2^126 in magnitude -- could round
+// differently; velocities never come near either.)  The polynomial's
+// products are inexact and stay separate.  20 instead of 26 operations.
 __device__ __forceinline__ float cubic_hermite(float t, float A, float B, float C, float D) {
-  float a = -A / 2.0f + (3.0f * B) / 2.0f - (3.0f * C) / 2.0f + D / 2.0f;
-  float b = A - (5.0f * B) / 2.0f + 2.0f * C - D / 2.0f;
-  float c = -A / 2.0f + C / 2.0f;
+  const float mA = A * -0.5f;                       // -A / 2
+  float a = __builtin_fmaf(3.0f * B, 0.5f, mA);     // + (3B) / 2
+  a = __builtin_fmaf(3.0f * C, -0.5f, a);           // - (3C) / 2
+  a = __builtin_fmaf(D, 0.5f, a);                   // + D / 2
+  float b = __builtin_fmaf(5.0f * B, -0.5f, A);     // A - (5B) / 2
+  b = __builtin_fmaf(C, 2.0f, b);                   // + 2C
+  b = __builtin_fmaf(D, -0.5f, b);                  // - D / 2
+  const float c = __builtin_fmaf(C, 0.5f, mA);      // -A / 2 + C / 2
   return a * t * t * t + b * t * t + c * t + B;
 }
 
@@ -268,8 +285,11 @@ __device__ __forceinline__ float cubic_hermite(float t, float A, float B, float 
 // truncate, Catmull-Rom down each of four columns, then across.  BUF (grids
 // under 2 GiB): the four 4-float rows are buffer loads, a 32-bit byte offset
 // for the first row and the row pitch as the (scalar) offset of the others,
-// instead of 64-bit address arithmetic per row; the clamp keeps every row in
-// the grid.  The descriptor's third word is the gfx9 raw-buffer format.
+// instead of 64-bit address arithmetic per row.  The clamp keeps the 4 x 4
+// stencil (columns icx-1 .. icx+2, rows icy-1 .. icy+2) inside the grid for
+// gw, gh >= 6 (icx <= gw-3); a simulation is at least 8 x 8 cells, so its
+// velocity grids are at least 7 x 8.  The descriptor's third word is the gfx9
+// raw-buffer format.
 template <bool BUF>
 __device__ __forceinline__ float bicubic(const float *__restrict__ g, int gw, int gh, float cx,
                                          float cy) {
@@ -316,18 +336,15 @@ __device__ __forceinline__ void advect_face(const float *__restrict__ xf,
                                             int lane, int y, float hdt, float fdt) {
   const int Wx = W - 1, Hy = H - 1;
   const int x = lane + 1;
-  const int x0 = 1 + (lane & ~7);
   const uint8_t *frow = fl + (size_t)y * W;
-  bool any = false;
-  if (!IS_VY) {
-#pragma unroll
-    for (int k = 0; k < 8; k++) any |= (frow[x0 + k - 1] & frow[x0 + k]) != 0;
-  } else {
-    const uint8_t *fdn = frow - W;
-#pragma unroll
-    for (int k = 0; k < 8; k++) any |= (frow[x0 + k] & fdn[x0 + k]) != 0;
-  }
-  if (!any) return;
+  // The chunk of lanes x0 .. x0+7 (x0 = 1 + 8k) runs if any of its lanes has
+  // both flags of the skip test set.  A chunk is 8 consecutive threads of one
+  // wave, 8-aligned in it, and either all or none of them are active here
+  // (the caller's bound is a multiple of 8), so one ballot of the lanes' own
+  // tests answers it for every chunk of the wave.
+  const bool mine = IS_VY ? (frow[x] & frow[x - W]) != 0 : (frow[x - 1] & frow[x]) != 0;
+  const unsigned long long votes = __ballot(mine);
+  if (((votes >> (threadIdx.x & 56)) & 0xffull) == 0) return;
   if (!IS_VY) {
     float posx = (float)x + 0.5f, posy = (float)y;
     float vx1 = xf[(size_t)x + (size_t)y * Wx];
@@ -504,14 +521,13 @@ __global__ void k_divergence(const float *__restrict__ xf, const float *__restri
   f[iy] = -ih * (xf[ix] - xf[ix - 1] + yf[iy] - yf[iy - W]);
 }
 
-// 3x3 sink stamps (simulation.cpp:173-182); positions resolved on the host.
-__global__ void k_stamp_sinks(float *__restrict__ f, int W, const int *__restrict__ xy,
+// sink stamps (simulation.cpp:173-182): the cells of the 3x3 patches and
+// their final values, settled on the host in sink order (one write per cell)
+__global__ void k_stamp_sinks(float *__restrict__ f, const int *__restrict__ cell,
                               const float *__restrict__ val, int n) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 9 * n) return;
-  int s = t / 9, k = t % 9;
-  int gx = xy[2 * s] + (k % 3) - 1, gy = xy[2 * s + 1] + (k / 3) - 1;
-  f[(size_t)gy * W + gx] = val[s];
+  if (t >= n) return;
+  f[cell[t]] = val[t];
 }
 
 // project's velocity correction (simulation.cpp:196-207)
@@ -673,11 +689,17 @@ __device__ __forceinline__ float randf(uint32_t &s) {
   return (float)(s % 100000u) / 100000.0f;
 }
 
-// psampleFlagLinear (simulation.cpp:398-411); mix(x,y,a) = x*(1-a) + y*a
+// psampleFlagLinear (simulation.cpp:398-411); mix(x,y,a) = x*(1-a) + y*a.
+// The float-to-int conversion of glm::ivec2(vec2) is x86's truncating
+// conversion, which gives INT_MIN for NaN and out-of-range values; the
+// clamp then picks column / row 0 for them, as it does on the reference.
+__device__ __forceinline__ int trunc_i32_x86(float v) {
+  return (v >= -2147483648.0f && v < 2147483648.0f) ? (int)v : (int)0x80000000u;
+}
 __device__ __forceinline__ float flag_linear(const uint8_t *__restrict__ F, int W, int H,
                                              float pwidth, float px, float py) {
   float cx = px / (pwidth / W) - 0.5f, cy = py / (pwidth / W) - 0.5f;
-  int ix = (int)cx, iy = (int)cy;
+  int ix = trunc_i32_x86(cx), iy = trunc_i32_x86(cy);
   ix = max(0, min(W - 2, ix));
   iy = max(0, min(H - 2, iy));
   float sx = cx - floorf(cx), sy = cy - floorf(cy);
@@ -724,6 +746,95 @@ __global__ void k_tracers(const float *__restrict__ xf, const float *__restrict_
   }
   xy[j] = make_float2(nx, ny);
   age[j] = a;
+}
+
+// psampleFlagNearest / psampleFlagLinear / psampleFlagNormal
+// (simulation.cpp:398-425) at n physical positions pc = (x0, y0, x1, ...).
+// Nearest reads the flag at the truncated grid position pc / (pwidth / W);
+// a position whose cell lies outside the grid (the reference indexes out of
+// bounds there) reads as solid, 0.  Normal writes two values per point from
+// the four linear samples at pc -+ (h, h), summed left to right.
+__global__ void k_sample_flag(const uint8_t *__restrict__ F, int W, int H, float pwidth, float h,
+                              int mode, const float *__restrict__ pc, float *__restrict__ out,
+                              int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float px = pc[2 * i], py = pc[2 * i + 1];
+  if (mode == 0) {
+    const float cell = pwidth / W;
+    const float gx = px / cell, gy = py / cell;
+    const bool in = gx > -1.0f && gx < (float)W && gy > -1.0f && gy < (float)H;
+    out[i] = in ? (float)F[(size_t)(int)gy * W + (int)gx] : 0.0f;
+  } else if (mode == 1) {
+    out[i] = flag_linear(F, W, H, pwidth, px, py);
+  } else {
+    const float p01 = flag_linear(F, W, H, pwidth, px - h, py + h);
+    const float p11 = flag_linear(F, W, H, pwidth, px + h, py + h);
+    const float p00 = flag_linear(F, W, H, pwidth, px - h, py - h);
+    const float p10 = flag_linear(F, W, H, pwidth, px + h, py - h);
+    out[2 * i] = p11 + p10 - p01 - p00;
+    out[2 * i + 1] = p01 + p11 - p00 - p10;
+  }
+}
+
+// getDT's scan (simulation.cpp:226-238): max over interior cells x in
+// [1, W-2], y in [1, H-2] of max(vx(x, y), vy(x, y)), one partial per
+// block (a maximum does not depend on the order it is taken in).
+__global__ void __launch_bounds__(256) k_max_vel(const float *__restrict__ xf,
+                                                 const float *__restrict__ yf, int W, int H,
+                                                 float *__restrict__ part) {
+  __shared__ float red[4];
+  const int iw = W - 2;
+  const size_t n = (size_t)iw * (H - 2);
+  float m = -INFINITY;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x) {
+    const int x = 1 + (int)(k % iw), y = 1 + (int)(k / iw);
+    m = fmaxf(m, fmaxf(xf[(size_t)y * (W - 1) + x], yf[(size_t)y * W + x]));
+  }
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+
+// centerP's sum (simulation.cpp:210-217) over interior cells, in float64
+// per block (a fixed order, so the result does not vary between calls)
+__global__ void __launch_bounds__(256) k_sum_interior(const float *__restrict__ p, int W, int H,
+                                                      double *__restrict__ part) {
+  __shared__ double red[4];
+  const int iw = W - 2;
+  const size_t n = (size_t)iw * (H - 2);
+  double s = 0.0;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x)
+    s += (double)p[(size_t)(1 + k / iw) * W + 1 + k % iw];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+// centerP's update (simulation.cpp:218-223): p -= v on interior cells
+__global__ void k_sub_interior(float *__restrict__ p, int W, int H, float v) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x + 1, y = blockIdx.y + 1;
+  if (x >= W - 1) return;
+  p[(size_t)y * W + x] -= v;
+}
+
+// gs (pressure_solver.cpp:26-33): a lexicographic Gauss-Seidel sweep.  The
+// cell (x, y) reads the new values of (x-1, y) and (x, y-1) and the old ones
+// of (x+1, y) and (x, y+1), so the cells of one anti-diagonal x + y = d
+// depend only on diagonal d-1 (already updated) and d+1 (not yet): one launch
+// per diagonal, in increasing d, gives the sequential sweep's values.
+__global__ void k_gs_diagonal(float *__restrict__ p, const float *__restrict__ f,
+                              const uint8_t *__restrict__ mask, int W, int H, float h, float alpha,
+                              int d) {
+  const int y0 = max(1, d - (W - 2));
+  const int y = y0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (y > min(H - 2, d - 1)) return;
+  const int x = d - y;
+  const size_t c = (size_t)y * W + x;
+  p[c] = smooth(p[c], p[c - 1], p[c + 1], p[c - W], p[c + W], f[c], mask[c], h, alpha);
 }
 
 } // namespace
@@ -782,8 +893,10 @@ void launch_advect(const VelView &v, const uint8_t *flag, float hdt, float fdt, 
   int lanes = 8 * (nck_x > nck_y ? nck_x : nck_y);
   if (lanes == 0 || v.H < 3) return;
   // buffer-load gathers while every byte offset of a velocity grid fits in
-  // 31 bits
-  const bool buf = (size_t)v.W * v.H * 4 < ((size_t)1 << 31);
+  // 31 bits; UBG_ADVECT_FLAT_LOADS=1 forces the 64-bit flat loads of larger
+  // grids (a test switch: both paths give the same values)
+  const char *flat = std::getenv("UBG_ADVECT_FLAT_LOADS");
+  const bool buf = (size_t)v.W * v.H * 4 < ((size_t)1 << 31) && !(flat && flat[0] == '1');
   launch_k(buf ? k_advect<true> : k_advect<false>, dim3(ceil_div(lanes, TPB), v.H - 2),
            dim3(TPB), 0, s, v.vxf, v.vyf, v.vxb, v.vyb, flag, v.W, v.H, nck_x, nck_y, hdt, fdt);
 }
@@ -791,9 +904,9 @@ void launch_divergence(const VelView &v, float *f, float ih, hipStream_t s) {
   hipLaunchKernelGGL(k_divergence, dim3(ceil_div(v.W - 2, TPB), v.H - 2), dim3(TPB), 0, s, v.vxf,
                      v.vyf, f, v.W, v.H, ih);
 }
-void launch_stamp_sinks(float *f, int W, const int *xy, const float *val, int n, hipStream_t s) {
+void launch_stamp_sinks(float *f, const int *cell, const float *val, int n, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_stamp_sinks, dim3(ceil_div(9 * n, TPB)), dim3(TPB), 0, s, f, W, xy, val, n);
+  hipLaunchKernelGGL(k_stamp_sinks, dim3(ceil_div(n, TPB)), dim3(TPB), 0, s, f, cell, val, n);
 }
 void launch_correct_velocity(const VelView &v, const float *p, const uint8_t *flag, float ih,
                              hipStream_t s) {
@@ -826,6 +939,31 @@ void launch_bilinear_vel(const VelView &v, const float *c, float *out, int n, hi
   if (n <= 0) return;
   hipLaunchKernelGGL(k_bilinear_vel, dim3(ceil_div(n, TPB)), dim3(TPB), 0, s, v.vxf, v.vyf, v.W,
                      v.H, c, out, n);
+}
+void launch_sample_flag(const uint8_t *flag, int W, int H, float pwidth, float h, int mode,
+                        const float *pc, float *out, int n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_sample_flag, dim3(ceil_div(n, TPB)), dim3(TPB), 0, s, flag, W, H, pwidth, h,
+                     mode, pc, out, n);
+}
+void launch_max_vel(const VelView &v, float *part, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_max_vel, dim3(nblocks), dim3(256), 0, s, v.vxf, v.vyf, v.W, v.H, part);
+}
+void launch_sum_interior(const float *p, int W, int H, double *part, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_interior, dim3(nblocks), dim3(256), 0, s, p, W, H, part);
+}
+void launch_sub_interior(float *p, int W, int H, float v, hipStream_t s) {
+  hipLaunchKernelGGL(k_sub_interior, dim3(ceil_div(W - 2, TPB), H - 2), dim3(TPB), 0, s, p, W, H,
+                     v);
+}
+void launch_gs(float *p, const float *f, const uint8_t *mask, int W, int H, float h, float alpha,
+               hipStream_t s) {
+  for (int d = 2; d <= (W - 2) + (H - 2); d++) {
+    const int y0 = d - (W - 2) > 1 ? d - (W - 2) : 1;
+    const int y1 = (H - 2) < d - 1 ? (H - 2) : d - 1;
+    hipLaunchKernelGGL(k_gs_diagonal, dim3(ceil_div(y1 - y0 + 1, TPB)), dim3(TPB), 0, s, p, f, mask,
+                       W, H, h, alpha, d);
+  }
 }
 void launch_tracers(const VelView &v, const uint8_t *flag, float h, float pwidth, float dt,
                     uint32_t seed, float *xy, float *age, const int *perm, int n, hipStream_t s) {

@@ -128,7 +128,8 @@ void launch_correct_tail(const VelView &v, float *p, const uint16_t *fmask, floa
 void launch_accumulate(const VelView &v, float *vx_acc, float *vy_acc, hipStream_t s);
 void launch_advect(const VelView &v, const uint8_t *flag, float hdt, float fdt, hipStream_t s);
 void launch_divergence(const VelView &v, float *f, float ih, hipStream_t s);
-void launch_stamp_sinks(float *f, int W, const int *xy, const float *val, int n, hipStream_t s);
+// f[cell[i]] = val[i] for n distinct cells
+void launch_stamp_sinks(float *f, const int *cell, const float *val, int n, hipStream_t s);
 void launch_correct_velocity(const VelView &v, const float *p, const uint8_t *flag, float ih,
                              hipStream_t s);
 void launch_bilinear_vel(const VelView &v, const float *c, float *out, int n, hipStream_t s);
@@ -148,6 +149,18 @@ void launch_items(ubg_item *items, const float *snap, int n, const float *vx, co
 // (2W-1) x (2H-1) collocated (vx, vy) pairs and magnitudes from vx, vy
 void launch_collocate(const float *vx, const float *vy, float *vxy, float *mag, int W, int H,
                       hipStream_t s);
+// psampleFlagNearest (mode 0), psampleFlagLinear (1), psampleFlagNormal (2)
+// at n physical positions (x, y pairs); out holds n values, 2n for mode 2
+void launch_sample_flag(const uint8_t *flag, int W, int H, float pwidth, float h, int mode,
+                        const float *pc, float *out, int n, hipStream_t s);
+// getDT's maximum of max(vx, vy) over interior cells: one partial per block
+void launch_max_vel(const VelView &v, float *part, int nblocks, hipStream_t s);
+// centerP: float64 partial sums of the interior cells, then p -= v there
+void launch_sum_interior(const float *p, int W, int H, double *part, int nblocks, hipStream_t s);
+void launch_sub_interior(float *p, int W, int H, float v, hipStream_t s);
+// one lexicographic Gauss-Seidel sweep (gs), one launch per anti-diagonal
+void launch_gs(float *p, const float *f, const uint8_t *mask, int W, int H, float h, float alpha,
+               hipStream_t s);
 void launch_tracers(const VelView &v, const uint8_t *flag, float h, float pwidth, float dt,
                     uint32_t seed, float *xy, float *age, const int *perm, int n,
                     hipStream_t s);

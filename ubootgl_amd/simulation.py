@@ -122,6 +122,10 @@ class MG:
     def solve_resident(self, h, zeroGradientBC=False, ncycles=1):
         check(lib().ubg_mg_solve(self._h, h, int(bool(zeroGradientBC)), ncycles))
 
+    def gs(self, h, alpha=1.0, nsweeps=1):
+        """gs (pressure_solver.cpp:26-33) on the resident level-0 fields."""
+        check(lib().ubg_mg_gs(self._h, h, alpha, nsweeps))
+
     def rbgs(self, h, alpha=1.0, nsweeps=1):
         check(lib().ubg_mg_rbgs(self._h, h, alpha, nsweeps))
 
@@ -180,13 +184,20 @@ class MG:
 
 
 class Simulation:
-    """simulation.hpp:18-141: Simulation(flag, pwidth, mu)."""
+    """simulation.hpp:18-141: Simulation(flag, pwidth, mu), or
+    Simulation.blank(pwidth, mu, width, height) for the reference's
+    Simulation(pwidth, mu, width, height) (simulation.hpp:20-30)."""
 
-    def __init__(self, flag, pwidth, mu, params=None):
-        flag = _f32(flag)
-        self.height, self.width = flag.shape
+    def __init__(self, flag, pwidth, mu, params=None, _blank_size=None):
         h = C.c_void_p()
-        check(lib().ubg_sim_create(fptr(flag), self.width, self.height, pwidth, mu, C.byref(h)))
+        if _blank_size is not None:
+            self.width, self.height = _blank_size
+            check(lib().ubg_sim_create_blank(self.width, self.height, pwidth, mu, C.byref(h)))
+        else:
+            flag = _f32(flag)
+            self.height, self.width = flag.shape
+            check(lib().ubg_sim_create(fptr(flag), self.width, self.height, pwidth, mu,
+                                       C.byref(h)))
         self._h = h
         self.pwidth, self.mu = float(pwidth), float(mu)
         hh = C.c_float()
@@ -199,6 +210,12 @@ class Simulation:
         self.mg = MG(None, _handle=mgh, _owner=self)
         self.mg.set_sizes(self.width, self.height)
         self.dt = 0.0
+
+    @classmethod
+    def blank(cls, pwidth, mu, width, height, params=None):
+        """Simulation(pwidth, mu, width, height): all fields zero, the flag
+        too; the MG's coarse flags stay 1 until the flag is set."""
+        return cls(None, pwidth, mu, params=params, _blank_size=(int(width), int(height)))
 
     def set_params(self, params):
         check(lib().ubg_sim_set_params(self._h, C.byref(params._p)))
@@ -296,6 +313,36 @@ class Simulation:
         check(lib().ubg_sim_collocate(self._h, vxy.ctypes.data_as(C.POINTER(C.c_float)),
                                       fptr(mag), 0))
         return vxy, mag
+
+    def _sample_flag(self, mode, pc):
+        pc = _f32(pc).reshape(-1, 2)
+        n = pc.shape[0]
+        out = np.zeros((n, 2) if mode == L.FLAG_NORMAL else (n,), np.float32)
+        check(lib().ubg_sim_sample_flag(self._h, mode, fptr(pc), fptr(out), n))
+        return out
+
+    def psampleFlagNearest(self, pc):
+        """simulation.cpp:422-425 at physical positions pc (n, 2); cells
+        outside the grid give 0."""
+        return self._sample_flag(L.FLAG_NEAREST, pc)
+
+    def psampleFlagLinear(self, pc):
+        """simulation.cpp:398-411 at physical positions pc (n, 2)."""
+        return self._sample_flag(L.FLAG_LINEAR, pc)
+
+    def psampleFlagNormal(self, pc):
+        """simulation.cpp:413-420 at physical positions pc (n, 2) -> (n, 2)."""
+        return self._sample_flag(L.FLAG_NORMAL, pc)
+
+    def getDT(self):
+        """simulation.cpp:226-238."""
+        out = C.c_float()
+        check(lib().ubg_sim_get_dt(self._h, C.byref(out)))
+        return out.value
+
+    def centerP(self):
+        """simulation.cpp:210-224."""
+        check(lib().ubg_sim_center_p(self._h))
 
     def bilinearVel(self, c):
         c = _f32(c).reshape(-1, 2)
