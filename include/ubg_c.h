@@ -147,6 +147,29 @@ UBG_API int ubg_mg_gs(ubg_mg *mg, float h, float alpha, int nsweeps);
 UBG_API int ubg_mg_residual(ubg_mg *mg, float h, float *r_dst, double *norm);
 UBG_API int ubg_mg_sync(ubg_mg *mg);
 
+/* ---- allocation-free solve ------------------------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  ncycles x MG::solve(p, f, flag, h, zeroGradientBC)
+ * (pressure_solver.hpp:59-62) on caller-owned DEVICE arrays: p (W x H fp32,
+ * updated in place), f (W x H fp32, read only) and flag (W x H bytes, 0 solid
+ * / nonzero fluid), with the hierarchy built from the flag on every call (MG
+ * (flag): updateFields, pressure_solver.hpp:32-57).  The results equal
+ * ubg_mg_solve_host's on a new MG of that flag.  prm: NULL for the reference's
+ * smoothing counts (3/3/5, alpha 1).  stream: a hipStream_t, or NULL for the
+ * default stream; the call only enqueues work there (no allocation, memset
+ * or synchronisation), so it may be captured into a graph.
+ * Every intermediate buffer lives in one static device array of
+ * UBG_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; a solve
+ * needs about 10.7 bytes per cell (W*H up to about 3.1 M cells, e.g.
+ * 2048 x 1024); larger sizes fail with UBG_E_BUDGET.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams or
+ * host threads) race and must be ordered by the caller. */
+#define UBG_NOALLOC_SCRATCH_BYTES (32u << 20)
+UBG_API int ubg_mg_solve_device_noalloc(float *p, const float *f, const uint8_t *flag, int width,
+                                        int height, float h, int zero_gradient_bc, int ncycles,
+                                        const ubg_params *prm, void *stream);
+
 /* ---- FP64 iterative refinement around FP32 V-cycles ---------------------
  * No counterpart in the reference (an extension of MG::solve,
  * pressure_solver.hpp:59-62).  Solves the flag-masked Poisson problem of

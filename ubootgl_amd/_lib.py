@@ -121,6 +121,8 @@ SIGNATURES = {
     "ubg_mg_solve_host": (C.c_int, [_vp, _fp, _fp, _fp, C.c_float, C.c_int]),
     "ubg_mg_rbgs": (C.c_int, [_vp, C.c_float, C.c_float, C.c_int]),
     "ubg_mg_gs": (C.c_int, [_vp, C.c_float, C.c_float, C.c_int]),
+    "ubg_mg_solve_device_noalloc": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_float,
+                                              C.c_int, C.c_int, C.POINTER(Params), _vp]),
     "ubg_mg_residual": (C.c_int, [_vp, C.c_float, _fp, _dp]),
     "ubg_mg_sync": (C.c_int, [_vp]),
     "ubg_ir_params_default": (None, [C.POINTER(IRParams)]),
