@@ -286,7 +286,7 @@ def smoother_out_of_cache(nsweeps=40):
     avg = ms / max(n, 1)
     gbs = SMOOTH_BYTES_PER_CELL * W * H / (avg * 1e-3) / 1e9
     log(f"sweep 8192x4096: {n} launches, avg {avg * 1e3:.2f} us -> {gbs:.0f} GB/s "
-        f"({wall * 1e3 / nsweeps:.2f} us per sweep by wall clock)")
+        f"({wall * 1e6 / nsweeps:.2f} us per sweep by wall clock)")
     return {"kernel": KERNEL_NAMES["sweep"], "grid": f"{W}x{H}",
             "working_set_MB": round(13 * W * H / 1e6, 1),  # p, p_alt, f + mask
             "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

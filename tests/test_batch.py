@@ -15,6 +15,7 @@ import ubootgl_amd as U
 from ubootgl_amd import _lib as L
 from ubootgl_amd import batch as B
 from ubootgl_amd import workloads as W
+from oracle import pyoracle as O
 
 MIXED_SIZES = [(64, 64), (96, 32), (128, 128), (160, 64), (256, 96), (320, 320),
                (512, 128), (384, 224), (16, 16), (33, 47), (1024, 256), (200, 130)]
@@ -80,6 +81,11 @@ def test_batch_equals_single_domain(gpu):
         assert rb["converged"] and rb["rel_residual"] <= 1e-6
         assert rb == rs, f"domain {d} {P.W}x{P.H}: {rb} vs {rs}"
         assert np.array_equal(xb, xs), f"domain {d} {P.W}x{P.H}"
+        # independent of the GPU: the oracle's float64 residual of the
+        # returned solution against ||r_0|| = ||F f|| of the zero start
+        _, r0 = O.residual(np.zeros((P.H, P.W)), P.f, P.flag, float(P.h))
+        _, rn = O.residual(xb, P.f, P.flag, float(P.h))
+        assert rn <= 1.0001e-6 * r0, f"domain {d} {P.W}x{P.H}: {rn / r0:.3e}"
     bt.close()
 
 
