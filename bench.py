@@ -61,10 +61,13 @@ def main():
     ap.add_argument("--copies", type=int, default=0,
                     help="step: after the run, N saveCurrentVelocityFields stages (plain "
                          "33.5 MB device copies; counter calibration)")
-    ap.add_argument("--workload", choices=["step", "ir", "batch", "serve"], default="step",
+    ap.add_argument("--workload", choices=["step", "ir", "batch", "serve", "sims"],
+                    default="step",
                     help="step: the 4096x2048 fluid step (default); ir: the 8192x4096 "
                          "FP64-refined solve to 1e-10; batch: 256 uneven domains to 1e-6; "
-                         "serve: Poisson arrivals at --rate of 3-timestep solve requests")
+                         "serve: Poisson arrivals at --rate of 3-timestep solve requests; "
+                         "sims: --sims simulations stepped with their solves batched")
+    ap.add_argument("--sims", type=int, default=64)
     ap.add_argument("--rate", type=float, default=2000.0)
     ap.add_argument("--duration", type=float, default=1.0)
     ap.add_argument("--templates", type=int, default=48)
@@ -94,6 +97,8 @@ def main():
         return bench_batch(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "sims":
+        return bench_sims(args)
 
     t0 = time.time()
     if (args.width, args.height) == (4096, 2048):
@@ -406,6 +411,77 @@ def bench_batch(args):
                    "sequential_solves_per_s": round(sequential, 3),
                    "speedup_vs_sequential": round(batched / sequential, 2),
                    "algorithmic_GBps": round(alg / (el / steps) / 1e9, 1)},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_sims(args):
+    """--sims independent simulations (W log-uniform in 128..1024, multiples
+    of 32, aspect 1..4, walls + disks, own 4-mode field; MG 2/2/5, 2 V-cycles
+    per step as the 4096x2048 workload): timesteps of all of them with their
+    pressure solves batched (one launch per multigrid stage for all, the rest
+    of each step on its own stream) against the same simulations stepped one
+    after another (each a graph replay).  value = simulation timesteps/s."""
+    import numpy as np
+
+    import ubootgl_amd as U
+    from ubootgl_amd import batch as B
+    from ubootgl_amd import workloads
+
+    sizes = workloads.batch_sizes(args.sims, seed=11, wmin=128, wmax=1024)
+    P = U.Params(nu_pre=2, nu_post=2, nu_coarse=5, vcycles_per_step=2)
+    pairs, dts = [], []
+    for i, (w, h) in enumerate(sizes):
+        d = workloads.small_domain(w, h, seed=500 + i, ndisks=max(1, w * h // 8192))
+        vx, vy = d.init_simulation_fields()
+        two = []
+        for _ in range(2):
+            s = U.Simulation(d.flag, d.pwidth, d.mu, params=P)
+            for name, a in (("vx", vx), ("vx_back", vx), ("vy", vy), ("vy_back", vy)):
+                s.set(name, a)
+            two.append(s)
+        pairs.append(two)
+        dts.append(float(d.dt))
+    cells = sum(w * h for w, h in sizes)
+    alone = [a for a, _ in pairs]
+    sb = B.SimBatch([b for _, b in pairs])
+    steps, warm = max(1, args.steps // 5), max(1, args.warmup)
+
+    def run(k, batched):
+        for _ in range(k):
+            if batched:
+                sb.step(dts)
+            else:
+                for s, dt in zip(alone, dts):
+                    s.step(dt)
+        for s in (b for _, b in pairs) if batched else alone:
+            s.sync()
+
+    run(warm, False)
+    run(warm, True)
+    t1 = time.perf_counter()
+    run(steps, False)
+    t_alone = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    run(steps, True)
+    t_batch = time.perf_counter() - t1
+    same = all(np.array_equal(a.get("p"), b.get("p")) and np.array_equal(a.get("vx"), b.get("vx"))
+               for a, b in pairs)
+    rate_b = len(pairs) * steps / t_batch
+    rate_a = len(pairs) * steps / t_alone
+    log(f"sims: {len(pairs)} simulations, {cells / 1e6:.2f} M cells; {steps} steps: batched "
+        f"{rate_b:.0f} sim-steps/s, one by one {rate_a:.0f} sim-steps/s; fields equal: {same}")
+    out = {
+        "metric": "simulation timesteps/s, pressure solves of several simulations batched",
+        "value": round(rate_b, 1), "unit": "sim-timesteps/s", "n_gpus": 1, "steps": steps,
+        "warmup": warm, "ms_per_step": round(t_batch * 1e3 / steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{len(pairs)} simulations W 128..1024 (log-uniform, x32), "
+                               f"aspect 1..4, walls + disks, MG 2/2/5, 2 V-cycles per step",
+                   "cells": cells, "one_by_one_sim_steps_per_s": round(rate_a, 1),
+                   "speedup_vs_one_by_one": round(rate_b / rate_a, 2),
+                   "fields_bit_identical": bool(same)},
     }
     print(json.dumps(out), flush=True)
 

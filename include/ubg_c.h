@@ -257,6 +257,39 @@ UBG_API int ubg_batch_autotune(ubg_batch *b, const char *db_path, ubg_batch_tuni
 UBG_API int ubg_batch_set_tuning(ubg_batch *b, const ubg_batch_tuning *t);
 UBG_API int ubg_batch_get_tuning(ubg_batch *b, ubg_batch_tuning *t);
 
+/* ---- Batched projections --------------------------------------------------
+ * The simulation's own pressure solve for many domains at once: MG::solve
+ * (pressure_solver.cpp:250-257 -> solveLevel :201-248) repeated ncycles times
+ * on every domain's float32 pressure, with setZeroGradientBC on level 0 as
+ * Simulation::project calls it (simulation.cpp:189-190), one launch per
+ * multigrid stage over all domains.  The reference steps one Simulation at a
+ * time; each domain's result is bit-identical to ubg_mg_solve /
+ * Simulation::step on that domain alone.  Needs 1..3 pre- and post-smoothing
+ * sweeps (the reference's 3/3 included).
+ *
+ * a projection domain: any flag (the simulation's walls included), float32
+ * rhs (the divergence), h; ubg_batch_solve refuses such a domain unless its
+ * outer ring is fluid */
+UBG_API int ubg_batch_set_projection_domain(ubg_batch *b, int d, const float *flag,
+                                            const float *rhs, float h);
+/* the float32 pressure the V-cycles start from (W x H host array; NULL = 0),
+ * and the one they leave */
+UBG_API int ubg_batch_set_pressure(ubg_batch *b, int d, const float *p);
+UBG_API int ubg_batch_get_pressure(ubg_batch *b, int d, float *p);
+/* ncycles V-cycles of every domain on its pressure and rhs; zero_gradient_bc
+ * as MG::solve's argument */
+UBG_API int ubg_batch_vcycles(ubg_batch *b, int ncycles, int zero_gradient_bc);
+/* A batch whose domain i is simulation i's pressure solve (size, flag, h;
+ * the simulations must share their multigrid parameters) ... */
+UBG_API int ubg_batch_create_for_sims(ubg_sim *const *sims, int n, ubg_batch **out);
+/* ... and one Simulation::step (simulation.cpp:356-374) of each of the n
+ * simulations with timestep dts[i], their pressure solves batched: each
+ * simulation runs up to its solve on its own stream, the V-cycles of all run
+ * on the batch's stream, then each finishes its step.  Equal, bit for bit, to
+ * ubg_sim_step on each simulation.  A simulation whose flags changed since the
+ * last call (or another simulation at slot i) gets its domain rebuilt. */
+UBG_API int ubg_sims_step(ubg_batch *b, ubg_sim *const *sims, int n, const float *dts);
+
 /* ---- Served solves ----------------------------------------------------------
  * No counterpart in the reference: a request loop on its own host thread.
  * A request is `timesteps` consecutive refinement solves on one registered

@@ -143,6 +143,12 @@ SIGNATURES = {
     "ubg_batch_autotune": (C.c_int, [_vp, C.c_char_p, C.POINTER(BatchTuning)]),
     "ubg_batch_set_tuning": (C.c_int, [_vp, C.POINTER(BatchTuning)]),
     "ubg_batch_get_tuning": (C.c_int, [_vp, C.POINTER(BatchTuning)]),
+    "ubg_batch_set_projection_domain": (C.c_int, [_vp, C.c_int, _fp, _fp, C.c_float]),
+    "ubg_batch_set_pressure": (C.c_int, [_vp, C.c_int, _fp]),
+    "ubg_batch_get_pressure": (C.c_int, [_vp, C.c_int, _fp]),
+    "ubg_batch_vcycles": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "ubg_batch_create_for_sims": (C.c_int, [C.POINTER(_vp), C.c_int, C.POINTER(_vp)]),
+    "ubg_sims_step": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, _fp]),
     "ubg_server_params_default": (None, [C.POINTER(ServerParams)]),
     "ubg_server_create": (C.c_int, [C.POINTER(ServerParams), C.POINTER(Params), C.POINTER(_vp)]),
     "ubg_server_destroy": (C.c_int, [_vp]),

@@ -1,11 +1,16 @@
-"""Batched refinement solves over libubg.so (ubg_batch_*).
+"""Batched solves over libubg.so (ubg_batch_*, ubg_sims_step).
 
-n independent Poisson problems (mgtest form: fluid outer ring held as
-Dirichlet data, residual operator of calculateResidualField,
+Refinement solves: n independent Poisson problems (mgtest form: fluid outer
+ring held as Dirichlet data, residual operator of calculateResidualField,
 pressure_solver.cpp:91-116) of uneven sizes, each solved by FP64 iterative
-refinement around FP32 V-cycles exactly as MG.ir_solve does for one domain;
-every multigrid stage of all domains is one launch.  No counterpart in the
-reference.
+refinement around FP32 V-cycles exactly as MG.ir_solve does for one domain.
+
+Projections: the simulation's own pressure solve (MG::solve with the
+zero-gradient BC, pressure_solver.cpp:250-257) on n domains, and whole
+Simulation::step calls of n simulations with their solves batched.
+
+Every multigrid stage of all domains is one launch.  No counterpart in the
+reference, which steps one Simulation at a time.
 """
 import ctypes as C
 
@@ -102,6 +107,34 @@ class Batch:
         check(lib().ubg_batch_get_tuning(self._h, C.byref(t)))
         return dict(tile_w=t.tile_w, tile_h=t.tile_h, blocks=t.blocks, pass_rows=t.pass_rows)
 
+    # ---- projections (the simulation's pressure solve) ----
+    def set_projection_domain(self, d, flag, rhs, h):
+        """Domain d for projections: any 0/1 flag (walls included)."""
+        flag = np.ascontiguousarray(flag, np.float32)
+        rhs = np.ascontiguousarray(rhs, np.float32)
+        W, H = self.sizes[d]
+        assert flag.shape == (H, W) and rhs.shape == (H, W)
+        check(lib().ubg_batch_set_projection_domain(self._h, d, fptr(flag), fptr(rhs), float(h)))
+
+    def set_pressure(self, d, p=None):
+        if p is None:
+            check(lib().ubg_batch_set_pressure(self._h, d, None))
+            return
+        p = np.ascontiguousarray(p, np.float32)
+        W, H = self.sizes[d]
+        assert p.shape == (H, W)
+        check(lib().ubg_batch_set_pressure(self._h, d, fptr(p)))
+
+    def pressure(self, d):
+        W, H = self.sizes[d]
+        out = np.zeros((H, W), np.float32)
+        check(lib().ubg_batch_get_pressure(self._h, d, fptr(out)))
+        return out
+
+    def vcycles(self, ncycles, zero_gradient_bc=True):
+        """ncycles V-cycles (MG::solve) of every domain, batched."""
+        check(lib().ubg_batch_vcycles(self._h, int(ncycles), 1 if zero_gradient_bc else 0))
+
     def close(self):
         if getattr(self, "_h", None) is not None:
             lib().ubg_batch_destroy(self._h)
@@ -112,3 +145,27 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class SimBatch(Batch):
+    """The pressure solves of several Simulations, stepped together:
+    step(dts) is one Simulation.step of each, bit-identical to stepping them
+    one by one, with one launch per multigrid stage for all of them."""
+
+    def __init__(self, sims):
+        self.sims = list(sims)
+        n = len(self.sims)
+        hs = (C.c_void_p * n)(*[s._h.value for s in self.sims])
+        h = C.c_void_p()
+        check(lib().ubg_batch_create_for_sims(hs, n, C.byref(h)))
+        self._h = h
+        self.sizes = [(s.width, s.height) for s in self.sims]
+
+    def step(self, dts):
+        n = len(self.sims)
+        if np.isscalar(dts):
+            dts = [dts] * n
+        d = np.ascontiguousarray(dts, np.float32)
+        assert d.shape == (n,)
+        hs = (C.c_void_p * n)(*[s._h.value for s in self.sims])
+        check(lib().ubg_sims_step(self._h, hs, n, fptr(d)))

@@ -42,6 +42,7 @@ struct StageArgs {
   const uint8_t *ib; // per-domain index row b
   float alpha;
   int oty; // tile rows of the fused level passes (their widths are per domain)
+  int zg;  // row-streamed level-0 passes apply setZeroGradientBC (projection solves)
 };
 
 __device__ __forceinline__ bool dom_active(const BDom &D) {
@@ -335,12 +336,21 @@ __global__ void __launch_bounds__(TPB) kb_stream_sweep(StageArgs a) {
   if (!dom_active(a.dom[u.d])) return;
   const BLev &F = a.lev[u.d * BMAXL + a.level];
   const int s = a.ia[u.d];
-  if ((F.w & 1) == 0)
+  const bool al = (F.w & 1) == 0;
+  if (a.zg) {
+    if (al)
+      rbgs_pair_unit<true, true>(F.p[s], F.p[s ^ 1], F.f, F.mask, F.w, F.h, F.hh, a.alpha,
+                                 u.nstrips, u.R, u.unit);
+    else
+      rbgs_pair_unit<true, false>(F.p[s], F.p[s ^ 1], F.f, F.mask, F.w, F.h, F.hh, a.alpha,
+                                  u.nstrips, u.R, u.unit);
+  } else if (al) {
     rbgs_pair_unit<false, true>(F.p[s], F.p[s ^ 1], F.f, F.mask, F.w, F.h, F.hh, a.alpha,
                                 u.nstrips, u.R, u.unit);
-  else
+  } else {
     rbgs_pair_unit<false, false>(F.p[s], F.p[s ^ 1], F.f, F.mask, F.w, F.h, F.hh, a.alpha,
                                  u.nstrips, u.R, u.unit);
+  }
 }
 
 // MODE_RR: last pre-sweep + residual + restriction (coarse rhs, zero coarse
@@ -372,8 +382,15 @@ __global__ void __launch_bounds__(TPB) kb_stream_pass(StageArgs a) {
   A.hc = C.h;
   A.nstrips = u.nstrips;
   A.R = u.R;
-  if ((F.w & 1) == 0) mg_stream_unit<1, MODE, false, true>(A, u.unit);
-  else mg_stream_unit<1, MODE, false, false>(A, u.unit);
+  const bool al = (F.w & 1) == 0;
+  if (a.zg) {
+    if (al) mg_stream_unit<1, MODE, true, true>(A, u.unit);
+    else mg_stream_unit<1, MODE, true, false>(A, u.unit);
+  } else if (al) {
+    mg_stream_unit<1, MODE, false, true>(A, u.unit);
+  } else {
+    mg_stream_unit<1, MODE, false, false>(A, u.unit);
+  }
 }
 
 // the coarse levels of each domain in LDS, one workgroup per domain slot;
@@ -726,18 +743,21 @@ int BatchEngine::init(hipStream_t s, const ubg_params &prm, int vcycles_per_step
   return UBG_OK;
 }
 
-int BatchEngine::geom_create(Arena &arena, int W, int H, const float *flag, BGeom **out) {
+int BatchEngine::geom_create(Arena &arena, int W, int H, const float *flag, BGeom **out,
+                             bool need_ring) {
   BShape sh;
   UBG_TRY(batch_shape(W, H, &sh));
   std::vector<uint8_t> f8;
   UBG_TRY(validate_flag(flag, (size_t)W * H, f8));
+  bool ring = true;
   for (int x = 0; x < W; x++)
-    if (!f8[x] || !f8[(size_t)(H - 1) * W + x])
-      return fail(UBG_E_FLAG_VALUE, "the outer ring of a solve domain must be fluid");
+    if (!f8[x] || !f8[(size_t)(H - 1) * W + x]) ring = false;
   for (int y = 0; y < H; y++)
-    if (!f8[(size_t)y * W] || !f8[(size_t)y * W + W - 1])
-      return fail(UBG_E_FLAG_VALUE, "the outer ring of a solve domain must be fluid");
+    if (!f8[(size_t)y * W] || !f8[(size_t)y * W + W - 1]) ring = false;
+  if (need_ring && !ring)
+    return fail(UBG_E_FLAG_VALUE, "the outer ring of a solve domain must be fluid");
   auto *g = new BGeom();
+  g->ring_fluid = ring;
   g->W = W;
   g->H = H;
   g->nlev = sh.nlev;
@@ -883,6 +903,27 @@ int BatchEngine::domain_get_solution(BDomain *d, double *x, hipStream_t s) {
   return UBG_OK;
 }
 
+int BatchEngine::domain_set_level0(BDomain *d, const float *p, const float *f,
+                                   hipMemcpyKind kind, hipStream_t s) {
+  if (!s) s = stream_;
+  const size_t n = (size_t)d->g->W * d->g->H;
+  float *p0 = d->p[0][d->cur[0]];
+  if (p) UBG_HIP(hipMemcpyAsync(p0, p, n * 4, kind, s));
+  else launch_fill_f32(p0, n, 0.0f, s);
+  if (f) UBG_HIP(hipMemcpyAsync(d->f[0], f, n * 4, kind, s));
+  UBG_TRY(check_launch());
+  if (kind != hipMemcpyDeviceToDevice) UBG_HIP(hipStreamSynchronize(s));
+  return UBG_OK;
+}
+
+int BatchEngine::domain_get_pressure(BDomain *d, float *p, hipMemcpyKind kind, hipStream_t s) {
+  if (!s) s = stream_;
+  const size_t n = (size_t)d->g->W * d->g->H;
+  UBG_HIP(hipMemcpyAsync(p, d->p[0][d->cur[0]], n * 4, kind, s));
+  if (kind != hipMemcpyDeviceToDevice) UBG_HIP(hipStreamSynchronize(s));
+  return UBG_OK;
+}
+
 // ===========================================================================
 // member tables, tiles, schedules
 // ===========================================================================
@@ -1009,6 +1050,9 @@ constexpr int kStreamR = 16;
 static_assert(kStreamR % 2 == 0, "streamed passes need an even row-block height");
 
 bool BatchEngine::stream_eligible(const BGeom &G, int l) const {
+  if (l < G.cfirst && tiled() && prm_.nu_pre >= 1 && prm_.nu_post >= 1 && l == 0 &&
+      force_stream0_)
+    return true;
   return l < G.cfirst && tiled() && prm_.nu_pre >= 1 && prm_.nu_post >= 1 &&
          G.w[l] >= kStreamMinW && (size_t)G.w[l] * G.h[l] >= kStreamMinCells;
 }
@@ -1043,7 +1087,7 @@ int BatchEngine::build_tiles() {
       const BGeom &G = *run_[s]->g;
       if (stream_eligible(G, l)) units += (long)cdiv(G.w[l], SW2) * cdiv(G.h[l] - 2, kStreamR);
     }
-    stream_on_[l] = units >= kStreamMinUnits;
+    stream_on_[l] = units >= kStreamMinUnits || (l == 0 && force_stream0_);
   }
   for (int l = 0; l < BMAXL; l++) {
     span(sw_[l], nr, [&](int s, int &nx, int &ny, int &) {
@@ -1144,6 +1188,64 @@ int BatchEngine::build_tiles() {
 
 enum { ST_SWEEP, ST_RR, ST_PC, ST_COARSE, ST_UPDATE, ST_NORM, ST_TPRE, ST_TPOST, ST_SSWEEP, ST_SRR, ST_SPC };
 
+// One V-cycle of the running members in the tiled mode (the per-level
+// passes of solveLevel, pressure_solver.cpp:201-248), simulated on (and
+// advancing) the domains' host index state.
+void BatchEngine::gen_vcycle_tiled(std::vector<StageRec> &st, std::vector<uint8_t> &idx) {
+  const size_t n = all_.size();
+  const size_t nr = run_.size();
+  auto row = [&](int kind, int level) -> std::pair<uint8_t *, uint8_t *> {
+    size_t off = idx.size();
+    idx.resize(off + 2 * n, 0);
+    st.push_back(StageRec{kind, level, off});
+    return {idx.data() + off, idx.data() + off + n};
+  };
+  // per level: a streamed domain takes nu_pre - 1 sweeps and the fused
+  // pre pass (each flipping the level's p), a tiled one the fused tile
+  // pre pass (one flip); mirrored on the way up
+  auto tile_at = [&](size_t d, int l) {
+    return l < run_[d]->g->cfirst && !streamed(*run_[d]->g, l);
+  };
+  auto stream_at = [&](size_t d, int l) { return streamed(*run_[d]->g, l); };
+  auto sweep_rows = [&](int l, int n) {
+    for (int k = 0; k < n; k++) {
+      auto r = row(ST_SSWEEP, l);
+      for (size_t d = 0; d < nr; d++)
+        if (stream_at(d, l)) {
+          r.first[d] = (uint8_t)run_[d]->cur[l];
+          run_[d]->cur[l] ^= 1;
+        }
+    }
+  };
+  auto pass_row = [&](int kind, int l, auto at) {
+    auto r = row(kind, l);
+    for (size_t d = 0; d < nr; d++)
+      if (at(d, l)) {
+        r.first[d] = (uint8_t)run_[d]->cur[l];
+        r.second[d] = (uint8_t)run_[d]->cur[l + 1];
+        run_[d]->cur[l] ^= 1;
+      }
+  };
+  for (int l = 0; l < lmax_; l++) {
+    if (st_[l].n) {
+      sweep_rows(l, prm_.nu_pre - 1);
+      pass_row(ST_SRR, l, stream_at);
+    }
+    if (tp_[l].n) pass_row(ST_TPRE, l, tile_at);
+  }
+  {
+    auto r = row(ST_COARSE, 0);
+    for (size_t d = 0; d < nr; d++) r.first[d] = (uint8_t)run_[d]->cur[run_[d]->g->cfirst];
+  }
+  for (int l = lmax_ - 1; l >= 0; l--) {
+    if (st_[l].n) {
+      pass_row(ST_SPC, l, stream_at);
+      sweep_rows(l, prm_.nu_post - 1);
+    }
+    if (tq_[l].n) pass_row(ST_TPOST, l, tile_at);
+  }
+}
+
 // The stage sequence of nsteps refinement steps and, per stage, the two
 // per-domain index rows, simulated from (and advancing) the domains' host
 // index state: a sweep reads p[cur] and writes p[cur^1]; the update reads
@@ -1160,52 +1262,7 @@ void BatchEngine::gen_steps(int nsteps, std::vector<StageRec> &st, std::vector<u
   };
   for (int step = 0; step < nsteps; step++) {
     const bool with_fresh = step == 0 && (all_fresh || !fresh_.empty());
-    for (int v = 0; v < k_ && nr && tiled(); v++) {
-      // per level: a streamed domain takes nu_pre - 1 sweeps and the fused
-      // pre pass (each flipping the level's p), a tiled one the fused tile
-      // pre pass (one flip); mirrored on the way up
-      auto tile_at = [&](size_t d, int l) {
-        return l < run_[d]->g->cfirst && !streamed(*run_[d]->g, l);
-      };
-      auto stream_at = [&](size_t d, int l) { return streamed(*run_[d]->g, l); };
-      auto sweep_rows = [&](int l, int n) {
-        for (int k = 0; k < n; k++) {
-          auto r = row(ST_SSWEEP, l);
-          for (size_t d = 0; d < nr; d++)
-            if (stream_at(d, l)) {
-              r.first[d] = (uint8_t)run_[d]->cur[l];
-              run_[d]->cur[l] ^= 1;
-            }
-        }
-      };
-      auto pass_row = [&](int kind, int l, auto at) {
-        auto r = row(kind, l);
-        for (size_t d = 0; d < nr; d++)
-          if (at(d, l)) {
-            r.first[d] = (uint8_t)run_[d]->cur[l];
-            r.second[d] = (uint8_t)run_[d]->cur[l + 1];
-            run_[d]->cur[l] ^= 1;
-          }
-      };
-      for (int l = 0; l < lmax_; l++) {
-        if (st_[l].n) {
-          sweep_rows(l, prm_.nu_pre - 1);
-          pass_row(ST_SRR, l, stream_at);
-        }
-        if (tp_[l].n) pass_row(ST_TPRE, l, tile_at);
-      }
-      {
-        auto r = row(ST_COARSE, 0);
-        for (size_t d = 0; d < nr; d++) r.first[d] = (uint8_t)run_[d]->cur[run_[d]->g->cfirst];
-      }
-      for (int l = lmax_ - 1; l >= 0; l--) {
-        if (st_[l].n) {
-          pass_row(ST_SPC, l, stream_at);
-          sweep_rows(l, prm_.nu_post - 1);
-        }
-        if (tq_[l].n) pass_row(ST_TPOST, l, tile_at);
-      }
-    }
+    for (int v = 0; v < k_ && nr && tiled(); v++) gen_vcycle_tiled(st, idx);
     for (int v = 0; v < k_ && nr && !tiled(); v++) {
       for (int l = 0; l < lmax_; l++) {
         if (!sw_[l].n && !rr_[l].n) continue;
@@ -1283,6 +1340,7 @@ int BatchEngine::launch_stages(const std::vector<StageRec> &st, const uint8_t *d
     a.lev = d_lev_;
     a.dom = d_dom_;
     a.level = r.level;
+    a.zg = zg0_ && r.level == 0;
     a.ia = d_idx + r.off;
     a.ib = d_idx + r.off + n;
     a.alpha = prm_.alpha;
@@ -1460,6 +1518,31 @@ int BatchEngine::run_steps_graph(int nsteps) {
   return UBG_OK;
 }
 
+int BatchEngine::set_level0_streamed(bool on) {
+  if (on == force_stream0_) return UBG_OK;
+  force_stream0_ = on;
+  drop_graphs();
+  if (d_lev_) UBG_TRY(build_tiles());
+  return UBG_OK;
+}
+
+int BatchEngine::enqueue_vcycles(int ncycles, bool zg) {
+  if (!tiled() || prm_.nu_pre < 1 || prm_.nu_post < 1)
+    return fail(UBG_E_INVALID, "projection batches need 1..3 pre- and post-smoothing sweeps");
+  if (!fresh_.empty()) return fail(UBG_E_STATE, "projection batches take running members only");
+  if (run_.empty() || ncycles <= 0) return UBG_OK;
+  UBG_TRY(set_level0_streamed(zg));
+  std::vector<StageRec> st;
+  std::vector<uint8_t> idx;
+  for (int v = 0; v < ncycles; v++) gen_vcycle_tiled(st, idx);
+  uint8_t *d_idx = nullptr;
+  UBG_TRY(upload_idx(idx, &d_idx));
+  zg0_ = zg;
+  const int rc = launch_stages(st, d_idx);
+  zg0_ = false;
+  return rc;
+}
+
 std::vector<BDomain *> BatchEngine::active_members() const {
   std::vector<BDomain *> out;
   for (size_t i = 0; i < all_.size(); i++)
@@ -1606,6 +1689,16 @@ struct ubg_batch {
   std::vector<int> ws, hs;
   std::vector<bool> ready;
   std::recursive_mutex mu;
+  // projection solves: every domain's solve state is active (refinement
+  // solves end domains; the next projection re-activates them)
+  bool proj_active = false;
+  // batched simulation steps: the simulation and flag generation each
+  // domain's geometry was built from, and the events that order the
+  // simulations' streams around the batch's
+  std::vector<const ubg_sim *> src;
+  std::vector<unsigned> src_gen;
+  hipEvent_t ev_solved = nullptr;
+  std::vector<hipEvent_t> ev_ready;
 };
 
 namespace {
@@ -1695,31 +1788,49 @@ UBG_API int ubg_batch_destroy(ubg_batch *b) {
     if (d) b->eng.domain_release(b->arena, d);
   b->doms.clear();
   b->arena.release();
+  if (b->ev_solved) (void)hipEventDestroy(b->ev_solved);
+  for (hipEvent_t e : b->ev_ready)
+    if (e) (void)hipEventDestroy(e);
   hipStream_t s = b->stream;
   delete b; // the engine frees its tables first
   if (s) (void)hipStreamDestroy(s);
   return UBG_OK;
 }
 
-UBG_API int ubg_batch_set_domain(ubg_batch *b, int d, const float *flag, const float *rhs, float h) {
+namespace {
+
+int set_domain(ubg_batch *b, int d, const float *flag, const float *rhs, float h, bool need_ring) {
   UBG_TRY(batch_check(b, d));
   if (!flag || !rhs) return fail(UBG_E_INVALID, "null argument");
   if (!(h > 0.0f) || !std::isfinite(h)) return fail(UBG_E_INVALID, "h must be positive");
   std::lock_guard<std::recursive_mutex> g(b->mu);
   if (b->doms[d]) {
+    b->eng.invalidate_members();
     b->eng.domain_release(b->arena, b->doms[d]);
     b->doms[d] = nullptr;
   }
   ubg::BGeom *geo = nullptr;
-  UBG_TRY(b->eng.geom_create(b->arena, b->ws[d], b->hs[d], flag, &geo));
+  UBG_TRY(b->eng.geom_create(b->arena, b->ws[d], b->hs[d], flag, &geo, need_ring));
   ubg::BDomain *dom = nullptr;
   int rc = b->eng.domain_create(b->arena, geo, h, &dom);
   b->eng.geom_release(b->arena, geo); // the domain holds its reference
   if (rc) return rc;
   b->doms[d] = dom;
+  b->proj_active = false;
   rc = b->eng.domain_set_rhs(dom, rhs);
   b->ready[d] = rc == UBG_OK;
   return rc;
+}
+
+} // namespace
+
+UBG_API int ubg_batch_set_domain(ubg_batch *b, int d, const float *flag, const float *rhs, float h) {
+  return set_domain(b, d, flag, rhs, h, true);
+}
+
+UBG_API int ubg_batch_set_projection_domain(ubg_batch *b, int d, const float *flag,
+                                            const float *rhs, float h) {
+  return set_domain(b, d, flag, rhs, h, false);
 }
 
 UBG_API int ubg_batch_set_rhs(ubg_batch *b, int d, const float *rhs) {
@@ -1737,6 +1848,158 @@ UBG_API int ubg_batch_set_guess(ubg_batch *b, int d, const double *x) {
   return b->eng.domain_set_guess(b->doms[d], x);
 }
 
+UBG_API int ubg_batch_set_pressure(ubg_batch *b, int d, const float *p) {
+  UBG_TRY(batch_check(b, d));
+  std::lock_guard<std::recursive_mutex> g(b->mu);
+  if (!b->doms[d]) return fail(UBG_E_STATE, "domain not set");
+  if (p) {
+    const size_t n = (size_t)b->ws[d] * b->hs[d];
+    for (size_t i = 0; i < n; i++)
+      if (!std::isfinite(p[i])) return fail(UBG_E_NONFINITE, "pressure has a non-finite value");
+  }
+  return b->eng.domain_set_level0(b->doms[d], p, nullptr, hipMemcpyHostToDevice);
+}
+
+UBG_API int ubg_batch_get_pressure(ubg_batch *b, int d, float *p) {
+  UBG_TRY(batch_check(b, d));
+  if (!p) return fail(UBG_E_INVALID, "null output");
+  std::lock_guard<std::recursive_mutex> g(b->mu);
+  if (!b->doms[d]) return fail(UBG_E_STATE, "domain not set");
+  return b->eng.domain_get_pressure(b->doms[d], p, hipMemcpyDeviceToHost);
+}
+
+namespace {
+
+// every domain a running member with an active solve state
+int proj_members(ubg_batch *b) {
+  if (!b->proj_active) {
+    for (auto *d : b->doms) UBG_TRY(b->eng.domain_start(d, 0.5, 1));
+    b->proj_active = true;
+  }
+  return b->eng.set_members(b->doms, {});
+}
+
+} // namespace
+
+UBG_API int ubg_batch_vcycles(ubg_batch *b, int ncycles, int zero_gradient_bc) {
+  if (!b) return fail(UBG_E_INVALID, "null batch");
+  if (ncycles < 1 || ncycles > 1000) return fail(UBG_E_INVALID, "ncycles must be in 1..1000");
+  std::lock_guard<std::recursive_mutex> g(b->mu);
+  for (size_t i = 0; i < b->doms.size(); i++)
+    if (!b->ready[i]) return fail(UBG_E_STATE, "domain " + std::to_string(i) + " not set");
+  // the level-0 rhs of the V-cycles is the domain's rhs
+  for (auto *d : b->doms)
+    UBG_HIP(hipMemcpyAsync(d->f[0], d->rhs, (size_t)d->g->W * d->g->H * 4,
+                           hipMemcpyDeviceToDevice, b->stream));
+  UBG_TRY(proj_members(b));
+  UBG_TRY(b->eng.enqueue_vcycles(ncycles, zero_gradient_bc != 0));
+  UBG_HIP(hipStreamSynchronize(b->stream));
+  return check_launch();
+}
+
+namespace {
+
+bool same_solver(const ubg_params &a, const ubg_params &b) {
+  return a.nu_pre == b.nu_pre && a.nu_post == b.nu_post && a.nu_coarse == b.nu_coarse &&
+         a.vcycles_per_step == b.vcycles_per_step && a.alpha == b.alpha;
+}
+
+// (re)build domain d from simulation s's level-0 flag and h
+int domain_from_sim(ubg_batch *b, int d, ubg_sim *s) {
+  const ubg::SimLevel0 q = ubg::sim_level0(s);
+  const size_t n = (size_t)q.W * q.H;
+  std::vector<uint8_t> fl(n);
+  UBG_HIP(hipStreamSynchronize(q.stream));
+  UBG_HIP(hipMemcpy(fl.data(), q.flag, n, hipMemcpyDeviceToHost));
+  std::vector<float> flag(n), zero(n, 0.0f);
+  for (size_t i = 0; i < n; i++) flag[i] = fl[i] ? 1.0f : 0.0f;
+  UBG_TRY(ubg_batch_set_projection_domain(b, d, flag.data(), zero.data(), q.h));
+  b->src[d] = s;
+  b->src_gen[d] = q.flag_gen;
+  b->proj_active = false;
+  return UBG_OK;
+}
+
+} // namespace
+
+UBG_API int ubg_batch_create_for_sims(ubg_sim *const *sims, int n, ubg_batch **out) {
+  if (!sims || !out || n < 1) return fail(UBG_E_INVALID, "null argument or n < 1");
+  std::vector<int> ws(n), hs(n);
+  ubg_params prm{};
+  for (int i = 0; i < n; i++) {
+    if (!sims[i]) return fail(UBG_E_INVALID, "null simulation");
+    const ubg::SimLevel0 q = ubg::sim_level0(sims[i]);
+    if (i == 0) prm = q.prm;
+    else if (!same_solver(prm, q.prm))
+      return fail(UBG_E_INVALID, "the simulations' multigrid parameters differ");
+    ws[i] = q.W;
+    hs[i] = q.H;
+  }
+  ubg_batch *b = nullptr;
+  UBG_TRY(ubg_batch_create(n, ws.data(), hs.data(), &prm, &b));
+  b->src.assign(n, nullptr);
+  b->src_gen.assign(n, 0u);
+  for (int i = 0; i < n; i++) {
+    const int rc = domain_from_sim(b, i, sims[i]);
+    if (rc != UBG_OK) {
+      ubg_batch_destroy(b);
+      return rc;
+    }
+  }
+  *out = b;
+  return UBG_OK;
+}
+
+UBG_API int ubg_sims_step(ubg_batch *b, ubg_sim *const *sims, int n, const float *dts) {
+  if (!b || !sims || !dts) return fail(UBG_E_INVALID, "null argument");
+  std::lock_guard<std::recursive_mutex> g(b->mu);
+  if (n != (int)b->doms.size() || b->src.size() != b->doms.size())
+    return fail(UBG_E_INVALID, "the batch was not made for these simulations");
+  const ubg_params &bp = b->eng.params();
+  std::vector<ubg::SimLevel0> q(n);
+  for (int i = 0; i < n; i++) {
+    if (!sims[i]) return fail(UBG_E_INVALID, "null simulation");
+    q[i] = ubg::sim_level0(sims[i]);
+    if (q[i].W != b->ws[i] || q[i].H != b->hs[i] || !same_solver(bp, q[i].prm))
+      return fail(UBG_E_INVALID, "simulation " + std::to_string(i) + " does not match its domain");
+    if (!std::isfinite(dts[i])) return fail(UBG_E_INVALID, "dt is not finite");
+  }
+  // geometry of a domain whose simulation changed its flags (or is another one)
+  for (int i = 0; i < n; i++)
+    if (b->src[i] != sims[i] || b->src_gen[i] != q[i].flag_gen) {
+      UBG_TRY(domain_from_sim(b, i, sims[i]));
+      q[i] = ubg::sim_level0(sims[i]);
+    }
+  if (!b->ev_solved) UBG_HIP(hipEventCreateWithFlags(&b->ev_solved, hipEventDisableTiming));
+  while ((int)b->ev_ready.size() < n) {
+    hipEvent_t e = nullptr;
+    UBG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    b->ev_ready.push_back(e);
+  }
+  // each simulation up to its solve on its own stream ...
+  for (int i = 0; i < n; i++) {
+    UBG_TRY(ubg::sim_step_before_solve(sims[i], dts[i]));
+    q[i] = ubg::sim_level0(sims[i]);
+    UBG_HIP(hipEventRecord(b->ev_ready[i], q[i].stream));
+    UBG_HIP(hipStreamWaitEvent(b->stream, b->ev_ready[i], 0));
+  }
+  // ... the V-cycles of all of them on the batch's stream ...
+  for (int i = 0; i < n; i++)
+    UBG_TRY(b->eng.domain_set_level0(b->doms[i], q[i].p, q[i].f, hipMemcpyDeviceToDevice,
+                                     b->stream));
+  UBG_TRY(proj_members(b));
+  UBG_TRY(b->eng.enqueue_vcycles(bp.vcycles_per_step, true));
+  for (int i = 0; i < n; i++)
+    UBG_TRY(b->eng.domain_get_pressure(b->doms[i], q[i].p, hipMemcpyDeviceToDevice, b->stream));
+  UBG_HIP(hipEventRecord(b->ev_solved, b->stream));
+  // ... and the rest of each step on its own stream again
+  for (int i = 0; i < n; i++) {
+    UBG_HIP(hipStreamWaitEvent(q[i].stream, b->ev_solved, 0));
+    UBG_TRY(ubg::sim_step_after_solve(sims[i]));
+  }
+  return check_launch();
+}
+
 UBG_API int ubg_batch_solve(ubg_batch *b, const ubg_ir_params *prm, int *graph_replays) {
   if (!b) return fail(UBG_E_INVALID, "null batch");
   ubg_ir_params P;
@@ -1746,9 +2009,15 @@ UBG_API int ubg_batch_solve(ubg_batch *b, const ubg_ir_params *prm, int *graph_r
   if (P.vcycles_per_iter < 1 || P.vcycles_per_iter > 256 || P.max_iter < 1 || P.max_iter > 1000000)
     return fail(UBG_E_INVALID, "vcycles_per_iter or max_iter out of range");
   std::lock_guard<std::recursive_mutex> g(b->mu);
-  for (size_t i = 0; i < b->doms.size(); i++)
+  for (size_t i = 0; i < b->doms.size(); i++) {
     if (!b->ready[i]) return fail(UBG_E_STATE, "domain " + std::to_string(i) + " not set");
+    if (!b->doms[i]->g->ring_fluid)
+      return fail(UBG_E_FLAG_VALUE, "domain " + std::to_string(i) +
+                                        " has a solid outer-ring cell: projection solves only");
+  }
   b->eng.set_vcycles(P.vcycles_per_iter);
+  b->proj_active = false;
+  UBG_TRY(b->eng.set_level0_streamed(false));
   for (auto *d : b->doms) UBG_TRY(b->eng.domain_start(d, P.tol, P.max_iter));
   // first update of every domain (r0), then replayed two-step graphs
   UBG_TRY(b->eng.set_members(b->doms, {}));
@@ -1848,6 +2117,7 @@ UBG_API int ubg_batch_autotune(ubg_batch *b, const char *db_path, ubg_batch_tuni
       }
     }
   }
+  UBG_TRY(b->eng.set_level0_streamed(false));
   UBG_TRY(b->eng.set_members(b->doms, {}));
   double ms = 0.0;
   UBG_TRY(b->eng.autotune(&ms));

@@ -78,6 +78,7 @@ struct BGeom {
   size_t bytes = 0;
   size_t lds_bytes = 0;
   int refs = 0;
+  bool ring_fluid = false; // every outer-ring cell is fluid
 };
 
 struct BDomain {
@@ -124,7 +125,10 @@ public:
   }
 
   // geometry / domain lifetime (memory from `arena`)
-  int geom_create(Arena &arena, int W, int H, const float *flag_host, BGeom **out);
+  // need_ring: the outer ring must be fluid (refinement solves hold it as
+  // Dirichlet data); projection domains take any flag
+  int geom_create(Arena &arena, int W, int H, const float *flag_host, BGeom **out,
+                  bool need_ring = true);
   void geom_release(Arena &arena, BGeom *g);
   // The domain calls take an optional stream (default: the engine's own).
   // A caller that keeps a step in flight on the engine's stream may use
@@ -139,10 +143,24 @@ public:
                    hipStream_t s = nullptr);
   int domain_read_state(BDomain *d, DState *out, hipStream_t s = nullptr);
   int domain_get_solution(BDomain *d, double *x_host, hipStream_t s = nullptr);
+  // level-0 pressure and rhs of the plain V-cycles (projection solves); kind
+  // is a hipMemcpyKind (host or device arrays); p == nullptr clears p
+  int domain_set_level0(BDomain *d, const float *p, const float *f, hipMemcpyKind kind,
+                        hipStream_t s = nullptr);
+  int domain_get_pressure(BDomain *d, float *p, hipMemcpyKind kind, hipStream_t s = nullptr);
 
   // the member set of the following steps: `running` domains take V-cycles
   // and an accumulating update; `fresh` ones only their first update
   int set_members(const std::vector<BDomain *> &running, const std::vector<BDomain *> &fresh);
+  // forget the member set (a member domain was replaced: a new domain may
+  // sit at the old one's address with other buffers), so the next
+  // set_members rebuilds the tables
+  void invalidate_members() {
+    run_.clear();
+    fresh_.clear();
+    all_.clear();
+    drop_graphs();
+  }
   // enqueue `nsteps` refinement steps of the members; the active-domain
   // count lands in *count_host (pinned) when the stream reaches it
   int enqueue_steps(int nsteps);
@@ -150,6 +168,17 @@ public:
   int enqueue_first_update();
   // eager steps, or one captured graph per index state, replayed
   int run_steps_graph(int nsteps);
+  // ncycles plain FP32 V-cycles (MG::solve, pressure_solver.cpp:250-257) of
+  // every running member on its level-0 pressure and rhs; zg: level 0 takes
+  // setZeroGradientBC after every sweep and after the correction (the
+  // simulation's projection).  Level 0 then runs as row-streamed passes for
+  // every domain (they carry the boundary copy); needs 1 <= nu_pre, nu_post
+  // <= 3.
+  int enqueue_vcycles(int ncycles, bool zg);
+  // level 0 as row-streamed passes for every member domain (projection
+  // solves) or by the size rule (refinement solves); rebuilds the tile maps
+  // when it changes
+  int set_level0_streamed(bool on);
   int active_count() const { return *count_host_; }
   // members whose solve was still active at the last completed step
   std::vector<BDomain *> active_members() const;
@@ -171,6 +200,7 @@ private:
   int build_tiles();
   void gen_steps(int nsteps, std::vector<StageRec> &st, std::vector<uint8_t> &idx,
                  bool all_fresh = false);
+  void gen_vcycle_tiled(std::vector<StageRec> &st, std::vector<uint8_t> &idx);
   int launch_stages(const std::vector<StageRec> &st, const uint8_t *d_idx);
   int upload_idx(const std::vector<uint8_t> &idx, uint8_t **d_idx);
   int launch_sweeps_only(int reps);
@@ -204,6 +234,8 @@ private:
   Span tp_[BMAXL], tq_[BMAXL]; // fused level passes (pre, post)
   Span ss_[BMAXL], st_[BMAXL]; // row-streamed levels: sweeps, pre / post passes
   bool stream_on_[BMAXL] = {};  // level l has enough streamed units over the members
+  bool force_stream0_ = false;  // level 0 streamed for every domain (projection solves)
+  bool zg0_ = false;            // the stages being launched apply ZG at level 0
   // levels run as fused tile passes (smoothing counts with a tiled pass)
   bool tiled() const { return tile_pass_ok(prm_.nu_pre) && tile_pass_ok(prm_.nu_post); }
   // level l of a domain of this shape may run as row-streamed passes (wide

@@ -534,15 +534,37 @@ struct ubg_sim {
   // divergence, [sinks,] the V-cycles, setPBC + correction + setVBCs +
   // saveCurrentVelocityFields.  Needs vy_bc_fresh.
   int step_body(bool with_sinks) {
+    UBG_TRY(body_before_solve(with_sinks));
+    for (int c = 0; c < prm.vcycles_per_step; c++) mg_solve();
+    body_after_solve();
+    return UBG_OK;
+  }
+  // the step's stages before and after the pressure solve's V-cycles
+  int body_before_solve(bool with_sinks) {
     diffuse_fused();
     advect();
     launch_vbcs_divergence(vel(), bc, f, 1.0f / h, stream);
     if (with_sinks) UBG_TRY(apply_sinks());
-    for (int c = 0; c < prm.vcycles_per_step; c++) mg_solve();
+    return UBG_OK;
+  }
+  void body_after_solve() {
     launch_correct_tail(vel(), P(), fmask, 1.0f / h, bc, prm.save_current ? vx_cur : nullptr,
                         prm.save_current ? vy_cur : nullptr, stream);
     vy_bc_fresh = true;
-    return UBG_OK;
+  }
+  // step() up to the V-cycles, eager (no graph): applyAccumulatedVelocity,
+  // diffuse, advect, setVBCs + divergence, sinks (simulation.cpp:356-369,
+  // 164-187); the pressure solve is then the caller's
+  int step_before_solve(float timestep) {
+    dt = timestep;
+    if (accum_dirty) {
+      launch_accumulate(vel(), vx_acc, vy_acc, stream);
+      accum_dirty = false;
+      vy_bc_fresh = false;
+    }
+    refresh_vy_bcs();
+    ensure_fmask();
+    return body_before_solve(true);
   }
 
   // simulation.cpp:356-374
@@ -1241,3 +1263,34 @@ UBG_API int ubg_sim_timer_read(ubg_sim *s, int timer, double *total_ms, long *la
 }
 
 } // extern "C"
+
+// the step around its pressure solve, for batched steps (batch.hip)
+namespace ubg {
+
+int sim_step_before_solve(ubg_sim *s, float dt) {
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  UBG_TRY(s->step_before_solve(dt));
+  return check_launch();
+}
+
+int sim_step_after_solve(ubg_sim *s) {
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->body_after_solve();
+  return check_launch();
+}
+
+SimLevel0 sim_level0(ubg_sim *s) {
+  SimLevel0 o;
+  o.W = s->W;
+  o.H = s->H;
+  o.h = s->h;
+  o.p = s->P();
+  o.f = s->f;
+  o.flag = s->flag;
+  o.stream = s->stream;
+  o.prm = s->prm;
+  o.flag_gen = s->mg->flag_gen;
+  return o;
+}
+
+} // namespace ubg

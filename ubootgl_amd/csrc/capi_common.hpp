@@ -247,6 +247,22 @@ struct ubg_mg {
 
 namespace ubg {
 
+// ---- a simulation's step around its pressure solve (batched steps) ------
+struct SimLevel0 {
+  int W = 0, H = 0;
+  float h = 0.0f;
+  float *p = nullptr, *f = nullptr; // current pressure, divergence rhs (device)
+  const uint8_t *flag = nullptr;    // 0/1 level-0 flag (device)
+  hipStream_t stream = nullptr;
+  ubg_params prm{};
+  unsigned flag_gen = 0;            // moves whenever the flag may have changed
+};
+SimLevel0 sim_level0(ubg_sim *s);
+// step() up to the V-cycles (eager, on the simulation's stream) and the
+// stages after them (setPBC + correction + setVBCs + saveCurrent)
+int sim_step_before_solve(ubg_sim *s, float dt);
+int sim_step_after_solve(ubg_sim *s);
+
 // Build an MG hierarchy.  If level-0 buffers are given they are borrowed
 // (the simulation's p, f, flag, mask); otherwise the MG owns them.
 int mg_create(int W, int H, hipStream_t stream, float *p0, float *p0_alt, float *f0,
