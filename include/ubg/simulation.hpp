@@ -261,6 +261,7 @@ public:
     pushSinks();
   }
   ubg_sim *handle() { return h_; }
+  friend class SimulationBatch;
 
   // public data members of the reference (simulation.hpp:119-140)
   float pwidth;
@@ -328,4 +329,38 @@ private:
     if (n) ubg_check(ubg_sim_get_sinks(h_, &sinks[0].x, n, &n));
   }
   ubg_sim *h_ = nullptr;
+};
+
+// Several simulations stepped together, their pressure solves batched into
+// one launch per multigrid stage (ubg_sims_step).  No counterpart in the
+// reference, which steps one Simulation at a time; step(dts) leaves every
+// simulation as sims[i]->step(dts[i]) would, bit for bit.  The simulations
+// must share their multigrid parameters and be at least 16 x 16.
+class SimulationBatch {
+public:
+  explicit SimulationBatch(std::vector<Simulation *> sims) : sims_(std::move(sims)) {
+    for (Simulation *s : sims_) hs_.push_back(s->h_);
+    ubg_check(ubg_batch_create_for_sims(hs_.data(), (int)hs_.size(), &b_));
+  }
+  ~SimulationBatch() {
+    if (b_) ubg_batch_destroy(b_);
+  }
+  SimulationBatch(const SimulationBatch &) = delete;
+  SimulationBatch &operator=(const SimulationBatch &) = delete;
+
+  void step(const std::vector<float> &dts) {
+    if (dts.size() != sims_.size()) throw ubg_error(UBG_E_INVALID, "one dt per simulation");
+    for (size_t i = 0; i < sims_.size(); i++) {
+      sims_[i]->dt = dts[i];
+      sims_[i]->pushBCs();
+      sims_[i]->pushSinks();
+    }
+    ubg_check(ubg_sims_step(b_, hs_.data(), (int)hs_.size(), dts.data()));
+    for (Simulation *s : sims_) s->pullSinks();
+  }
+
+private:
+  std::vector<Simulation *> sims_;
+  std::vector<ubg_sim *> hs_;
+  ubg_batch *b_ = nullptr;
 };
