@@ -536,6 +536,27 @@ __global__ void __launch_bounds__(1024) kb_norm(StageArgs a, int nslots) {
   if (S.iters >= S.max_iter) S.active = 0;
 }
 
+// Level-0 copies of up to kCopyJobs domains in one launch (batched
+// simulation steps): block (x, j) copies a stride of job j's p (and f when
+// its source is set).  The jobs travel as the kernel's argument.
+constexpr int kCopyJobs = 64;
+struct CopyJob {
+  const float *sp, *sf;
+  float *dp, *df;
+  int n;
+};
+struct CopyJobs {
+  CopyJob j[kCopyJobs];
+};
+__global__ void __launch_bounds__(256) kb_copy_level0(CopyJobs jobs, int njobs) {
+  if ((int)blockIdx.y >= njobs) return;
+  const CopyJob &J = jobs.j[blockIdx.y];
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < J.n; i += gridDim.x * 256) {
+    J.dp[i] = J.sp[i];
+    if (J.sf) J.df[i] = J.sf[i];
+  }
+}
+
 __global__ void __launch_bounds__(256) kb_count(const BDom *dom, int n, int *out,
                                                 uint8_t *flags) {
   __shared__ int red[4];
@@ -1532,15 +1553,51 @@ int BatchEngine::enqueue_vcycles(int ncycles, bool zg) {
   if (!fresh_.empty()) return fail(UBG_E_STATE, "projection batches take running members only");
   if (run_.empty() || ncycles <= 0) return UBG_OK;
   UBG_TRY(set_level0_streamed(zg));
+  // one captured graph per index state (the V-cycles of all members, their
+  // index rows in a device buffer of the graph's own), replayed
+  const std::string key =
+      state_key() + "/v" + std::to_string(ncycles) + (zg ? "z" : "n");
+  auto it = graphs_.find(key);
+  if (it == graphs_.end()) {
+    std::vector<std::array<int, BMAXL>> saved;
+    for (BDomain *d : all_) {
+      std::array<int, BMAXL> c;
+      for (int l = 0; l < BMAXL; l++) c[l] = d->cur[l];
+      saved.push_back(c);
+    }
+    std::vector<StageRec> st;
+    std::vector<uint8_t> idx;
+    for (int v = 0; v < ncycles; v++) gen_vcycle_tiled(st, idx);
+    for (size_t i = 0; i < all_.size(); i++)
+      for (int l = 0; l < BMAXL; l++) all_[i]->cur[l] = saved[i][l];
+    Graph g;
+    UBG_HIP(hipMalloc((void **)&g.d_idx, std::max<size_t>(idx.size(), 1)));
+    UBG_HIP(hipMemcpy(g.d_idx, idx.data(), idx.size(), hipMemcpyHostToDevice));
+    hipGraph_t graph = nullptr;
+    UBG_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    zg0_ = zg;
+    int rc = launch_stages(st, g.d_idx);
+    zg0_ = false;
+    hipError_t e = hipStreamEndCapture(stream_, &graph);
+    if (rc != UBG_OK || e != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      (void)hipFree(g.d_idx);
+      return rc != UBG_OK ? rc : fail(UBG_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      (void)hipFree(g.d_idx);
+      return fail(UBG_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    it = graphs_.emplace(key, g).first;
+  }
+  UBG_HIP(hipGraphLaunch(it->second.exec, stream_));
+  // advance the host index state as the schedule does
   std::vector<StageRec> st;
   std::vector<uint8_t> idx;
   for (int v = 0; v < ncycles; v++) gen_vcycle_tiled(st, idx);
-  uint8_t *d_idx = nullptr;
-  UBG_TRY(upload_idx(idx, &d_idx));
-  zg0_ = zg;
-  const int rc = launch_stages(st, d_idx);
-  zg0_ = false;
-  return rc;
+  return UBG_OK;
 }
 
 std::vector<BDomain *> BatchEngine::active_members() const {
@@ -1899,6 +1956,22 @@ UBG_API int ubg_batch_vcycles(ubg_batch *b, int ncycles, int zero_gradient_bc) {
 
 namespace {
 
+// the copies of `jobs`, kCopyJobs per launch
+int launch_copies(const std::vector<ubg::CopyJob> &jobs, hipStream_t s) {
+  for (size_t b = 0; b < jobs.size(); b += ubg::kCopyJobs) {
+    ubg::CopyJobs cj{};
+    const int nj = (int)std::min<size_t>(ubg::kCopyJobs, jobs.size() - b);
+    int maxn = 0;
+    for (int k = 0; k < nj; k++) {
+      cj.j[k] = jobs[b + k];
+      maxn = std::max(maxn, cj.j[k].n);
+    }
+    const int gx = std::max(1, std::min((maxn + 1023) / 1024, 512));
+    hipLaunchKernelGGL(ubg::kb_copy_level0, dim3(gx, nj), dim3(256), 0, s, cj, nj);
+  }
+  return check_launch();
+}
+
 bool same_solver(const ubg_params &a, const ubg_params &b) {
   return a.nu_pre == b.nu_pre && a.nu_post == b.nu_post && a.nu_coarse == b.nu_coarse &&
          a.vcycles_per_step == b.vcycles_per_step && a.alpha == b.alpha;
@@ -1983,14 +2056,22 @@ UBG_API int ubg_sims_step(ubg_batch *b, ubg_sim *const *sims, int n, const float
     UBG_HIP(hipEventRecord(b->ev_ready[i], q[i].stream));
     UBG_HIP(hipStreamWaitEvent(b->stream, b->ev_ready[i], 0));
   }
-  // ... the V-cycles of all of them on the batch's stream ...
-  for (int i = 0; i < n; i++)
-    UBG_TRY(b->eng.domain_set_level0(b->doms[i], q[i].p, q[i].f, hipMemcpyDeviceToDevice,
-                                     b->stream));
+  // ... the V-cycles of all of them on the batch's stream (the level-0
+  // pressure and rhs gathered in one launch, the pressure scattered back in
+  // another) ...
   UBG_TRY(proj_members(b));
+  std::vector<ubg::CopyJob> in(n), out(n);
+  for (int i = 0; i < n; i++) {
+    ubg::BDomain *d = b->doms[i];
+    in[i] = ubg::CopyJob{q[i].p, q[i].f, d->p[0][d->cur[0]], d->f[0], q[i].W * q[i].H};
+  }
+  UBG_TRY(launch_copies(in, b->stream));
   UBG_TRY(b->eng.enqueue_vcycles(bp.vcycles_per_step, true));
-  for (int i = 0; i < n; i++)
-    UBG_TRY(b->eng.domain_get_pressure(b->doms[i], q[i].p, hipMemcpyDeviceToDevice, b->stream));
+  for (int i = 0; i < n; i++) {
+    ubg::BDomain *d = b->doms[i];
+    out[i] = ubg::CopyJob{d->p[0][d->cur[0]], nullptr, q[i].p, nullptr, q[i].W * q[i].H};
+  }
+  UBG_TRY(launch_copies(out, b->stream));
   UBG_HIP(hipEventRecord(b->ev_solved, b->stream));
   // ... and the rest of each step on its own stream again
   for (int i = 0; i < n; i++) {
