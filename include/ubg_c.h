@@ -257,6 +257,25 @@ UBG_API int ubg_batch_autotune(ubg_batch *b, const char *db_path, ubg_batch_tuni
 UBG_API int ubg_batch_set_tuning(ubg_batch *b, const ubg_batch_tuning *t);
 UBG_API int ubg_batch_get_tuning(ubg_batch *b, ubg_batch_tuning *t);
 
+/* ---- Fluid components ----------------------------------------------------
+ * With solid walls each fluid component of the flag is a pure-Neumann
+ * problem; its rhs is compatible only if it sums to zero over the component.
+ * No counterpart in the reference (its solver never makes the rhs
+ * compatible).  Components under 4-neighbour connectivity (the coupling of
+ * calculateResidualField, pressure_solver.cpp:100-108), labelled 0..n-1 in
+ * raster order of each component's first cell, -1 on solid cells; integer
+ * and exact (computed on the GPU by a lock-free union-find).  W x H host
+ * arrays. */
+UBG_API int ubg_flag_components(const float *flag, int width, int height, int32_t *labels,
+                                int *ncomp);
+/* f minus, on every fluid cell, the mean of f over its component: sums in
+ * 64-bit fixed point (f 2^s rounded, s from max|f| and W*H so no sum
+ * overflows; the result does not depend on summation order), means in
+ * float64, f = (float)(f - mean); solid cells unchanged.  *ncomp may be
+ * NULL. */
+UBG_API int ubg_remove_component_means(const float *flag, int width, int height, float *f,
+                                       int *ncomp);
+
 /* ---- Batched projections --------------------------------------------------
  * The simulation's own pressure solve for many domains at once: MG::solve
  * (pressure_solver.cpp:250-257 -> solveLevel :201-248) repeated ncycles times

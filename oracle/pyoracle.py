@@ -101,6 +101,8 @@ def load(path):
         "oracle_sim_advect_items": (None, [C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_int]),
         "oracle_cephes_sin": (C.c_float, [C.c_float]),
         "oracle_cephes_cos": (C.c_float, [C.c_float]),
+        "oracle_components": (C.c_int, [_f32p, C.c_int, C.c_int, _i32p]),
+        "oracle_remove_component_means": (C.c_int, [_f32p, C.c_int, C.c_int, _f32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -450,3 +452,24 @@ def cephes_sin(x):
 
 def cephes_cos(x):
     return lib().oracle_cephes_cos(float(x))
+
+
+def components(flag):
+    """Fluid components under 4-neighbour connectivity: (labels int32, n),
+    labels 0..n-1 in raster order of each component's first cell, -1 on
+    solid cells."""
+    fl = f32(flag)
+    H, W = fl.shape
+    lab = np.zeros((H, W), np.int32)
+    n = lib().oracle_components(fl, W, H, lab)
+    return lab, n
+
+
+def remove_component_means(flag, f):
+    """f with its mean over each fluid component removed (64-bit fixed-point
+    sums, float64 means); returns (f', n)."""
+    fl = f32(flag)
+    out = f32(f).copy()
+    H, W = fl.shape
+    n = lib().oracle_remove_component_means(fl, W, H, out)
+    return out, n

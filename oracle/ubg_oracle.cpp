@@ -1387,3 +1387,67 @@ ORACLE_API void oracle_sim_advect_items(void *sp, void *items, int n, float game
 
 ORACLE_API float oracle_cephes_sin(float x) { return cephes_sin(x); }
 ORACLE_API float oracle_cephes_cos(float x) { return cephes_cos(x); }
+
+// Connected components of the fluid cells (flag 1) under 4-neighbour
+// connectivity, the coupling of calculateResidualField
+// (pressure_solver.cpp:100-108): labels 0..n-1 in raster order of each
+// component's first cell, -1 on solid cells.  Breadth-first search from
+// each unlabelled fluid cell in raster order.  Returns n.
+ORACLE_API int oracle_components(const float *fl, int W, int H, int *lab) {
+  const long n = (long)W * H;
+  for (long i = 0; i < n; i++) lab[i] = -1;
+  std::vector<long> queue;
+  int nc = 0;
+  for (long s = 0; s < n; s++) {
+    if (fl[s] == 0.0f || lab[s] >= 0) continue;
+    queue.clear();
+    queue.push_back(s);
+    lab[s] = nc;
+    for (size_t k = 0; k < queue.size(); k++) {
+      const long i = queue[k];
+      const int x = (int)(i % W), y = (int)(i / W);
+      const long nb[4] = {x > 0 ? i - 1 : -1, x + 1 < W ? i + 1 : -1, y > 0 ? i - W : -1,
+                          y + 1 < H ? i + W : -1};
+      for (long j : nb)
+        if (j >= 0 && fl[j] != 0.0f && lab[j] < 0) {
+          lab[j] = nc;
+          queue.push_back(j);
+        }
+    }
+    nc++;
+  }
+  return nc;
+}
+
+// Subtract from f, on every fluid cell, the mean of f over the cell's
+// component.  The component sums are 64-bit fixed point: each f scaled by
+// 2^s (s = 62 - e - en with max|f| < 2^e and n < 2^en, so no sum overflows)
+// and rounded to the nearest integer, which makes them independent of the
+// summation order; mean = sum 2^-s / count in float64; f = (float)(f - mean).
+ORACLE_API int oracle_remove_component_means(const float *fl, int W, int H, float *f) {
+  const long n = (long)W * H;
+  std::vector<int> lab((size_t)n);
+  const int nc = oracle_components(fl, W, H, lab.data());
+  float amax = 0.0f;
+  for (long i = 0; i < n; i++)
+    if (lab[i] >= 0) amax = std::fmax(amax, std::fabs(f[i]));
+  if (nc == 0 || amax == 0.0f) return nc;
+  int e = 0, en = 0;
+  (void)std::frexp((double)amax, &e);
+  (void)std::frexp((double)n, &en);
+  const int sh = 62 - e - en;
+  const double scale = std::ldexp(1.0, sh), inv = std::ldexp(1.0, -sh);
+  std::vector<long long> sum((size_t)nc, 0);
+  std::vector<int> cnt((size_t)nc, 0);
+  for (long i = 0; i < n; i++)
+    if (lab[i] >= 0) {
+      sum[lab[i]] += std::llrint((double)f[i] * scale);
+      cnt[lab[i]]++;
+    }
+  for (long i = 0; i < n; i++)
+    if (lab[i] >= 0) {
+      const double mean = (double)sum[lab[i]] * inv / (double)cnt[lab[i]];
+      f[i] = (float)((double)f[i] - mean);
+    }
+  return nc;
+}

@@ -149,6 +149,8 @@ SIGNATURES = {
     "ubg_batch_vcycles": (C.c_int, [_vp, C.c_int, C.c_int]),
     "ubg_batch_create_for_sims": (C.c_int, [C.POINTER(_vp), C.c_int, C.POINTER(_vp)]),
     "ubg_sims_step": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, _fp]),
+    "ubg_flag_components": (C.c_int, [_fp, C.c_int, C.c_int, _ip, _ip]),
+    "ubg_remove_component_means": (C.c_int, [_fp, C.c_int, C.c_int, _fp, _ip]),
     "ubg_server_params_default": (None, [C.POINTER(ServerParams)]),
     "ubg_server_create": (C.c_int, [C.POINTER(ServerParams), C.POINTER(Params), C.POINTER(_vp)]),
     "ubg_server_destroy": (C.c_int, [_vp]),

@@ -401,3 +401,28 @@ def terrain_flags(rgba, scale):
     check(lib().ubg_terrain_flags(rgba.ctypes.data_as(C.POINTER(C.c_uint8)), W, H, int(scale),
                                   fptr(full), fptr(sim)))
     return full, sim
+
+
+def flag_components(flag):
+    """Fluid components of a 0/1 flag under 4-neighbour connectivity, on the
+    GPU: (labels int32 H x W, n), labels 0..n-1 in raster order of each
+    component's first cell, -1 on solid cells (ubg_flag_components)."""
+    fl = _f32(flag)
+    H, W = fl.shape
+    lab = np.zeros((H, W), np.int32)
+    n = C.c_int()
+    check(lib().ubg_flag_components(fptr(fl), W, H, lab.ctypes.data_as(C.POINTER(C.c_int)),
+                                    C.byref(n)))
+    return lab, n.value
+
+
+def remove_component_means(flag, f):
+    """f with its mean over each fluid component removed, on the GPU
+    (ubg_remove_component_means); returns (f', n)."""
+    fl = _f32(flag)
+    out = np.array(f, np.float32, copy=True, order="C")
+    H, W = fl.shape
+    assert out.shape == (H, W)
+    n = C.c_int()
+    check(lib().ubg_remove_component_means(fptr(fl), W, H, fptr(out), C.byref(n)))
+    return out, n.value
