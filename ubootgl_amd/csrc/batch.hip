@@ -1754,8 +1754,17 @@ struct ubg_batch {
   // simulations' streams around the batch's
   std::vector<const ubg_sim *> src;
   std::vector<unsigned> src_gen;
-  hipEvent_t ev_solved = nullptr;
   std::vector<hipEvent_t> ev_ready;
+  std::shared_ptr<ubg::ExtEvent> ev_done; // recorded after each batched step
+  // the per-stage simulation tables of a batched step: two slots used in
+  // turn (pinned staging and device copy), each free again once its event
+  // has passed
+  struct TabSlot {
+    ubg::SimRec *host = nullptr, *dev = nullptr;
+    size_t cap = 0;
+    hipEvent_t used = nullptr;
+  } slot[2];
+  int next_slot = 0;
 };
 
 namespace {
@@ -1845,9 +1854,13 @@ UBG_API int ubg_batch_destroy(ubg_batch *b) {
     if (d) b->eng.domain_release(b->arena, d);
   b->doms.clear();
   b->arena.release();
-  if (b->ev_solved) (void)hipEventDestroy(b->ev_solved);
   for (hipEvent_t e : b->ev_ready)
     if (e) (void)hipEventDestroy(e);
+  for (auto &t : b->slot) {
+    if (t.host) (void)hipHostFree(t.host);
+    if (t.dev) (void)hipFree(t.dev);
+    if (t.used) (void)hipEventDestroy(t.used);
+  }
   hipStream_t s = b->stream;
   delete b; // the engine frees its tables first
   if (s) (void)hipStreamDestroy(s);
@@ -2043,22 +2056,76 @@ UBG_API int ubg_sims_step(ubg_batch *b, ubg_sim *const *sims, int n, const float
       UBG_TRY(domain_from_sim(b, i, sims[i]));
       q[i] = ubg::sim_level0(sims[i]);
     }
-  if (!b->ev_solved) UBG_HIP(hipEventCreateWithFlags(&b->ev_solved, hipEventDisableTiming));
+  if (!b->ev_done) {
+    auto e = std::make_shared<ubg::ExtEvent>();
+    UBG_HIP(hipEventCreateWithFlags(&e->ev, hipEventDisableTiming));
+    b->ev_done = e;
+  }
   while ((int)b->ev_ready.size() < n) {
     hipEvent_t e = nullptr;
     UBG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     b->ev_ready.push_back(e);
   }
-  // each simulation up to its solve on its own stream ...
+  // The per-cell stages of every simulation's step as records (their
+  // applyAccumulatedVelocity / boundary / face-factor preludes, when due,
+  // run on the simulations' own streams); the batch's stream then waits for
+  // every simulation whose stream had work of its own since the last step.
+  std::vector<ubg::SimRec> rec((size_t)4 * n);
+  std::vector<char> sinks(n, 0);
   for (int i = 0; i < n; i++) {
-    UBG_TRY(ubg::sim_step_before_solve(sims[i], dts[i]));
-    q[i] = ubg::sim_level0(sims[i]);
-    UBG_HIP(hipEventRecord(b->ev_ready[i], q[i].stream));
-    UBG_HIP(hipStreamWaitEvent(b->stream, b->ev_ready[i], 0));
+    bool has = false;
+    UBG_TRY(ubg::sim_batched_records(sims[i], dts[i], &rec[(size_t)4 * i], &has));
+    sinks[i] = has;
   }
-  // ... the V-cycles of all of them on the batch's stream (the level-0
-  // pressure and rhs gathered in one launch, the pressure scattered back in
-  // another) ...
+  for (int i = 0; i < n; i++)
+    if (ubg::sim_take_own_work(sims[i])) {
+      UBG_HIP(hipEventRecord(b->ev_ready[i], q[i].stream));
+      UBG_HIP(hipStreamWaitEvent(b->stream, b->ev_ready[i], 0));
+    }
+  // stage tables: stage k of simulation i at [k n + i], rows numbered
+  // through the simulations' interiors
+  auto &T = b->slot[b->next_slot];
+  b->next_slot ^= 1;
+  const size_t nt = (size_t)4 * n;
+  if (T.cap < nt) {
+    if (T.host) (void)hipHostFree(T.host);
+    if (T.dev) (void)hipFree(T.dev);
+    T.host = nullptr;
+    T.dev = nullptr;
+    T.cap = 0;
+    UBG_HIP(hipHostMalloc((void **)&T.host, nt * sizeof(ubg::SimRec), hipHostMallocDefault));
+    UBG_HIP(hipMalloc((void **)&T.dev, nt * sizeof(ubg::SimRec)));
+    T.cap = nt;
+    if (!T.used) UBG_HIP(hipEventCreateWithFlags(&T.used, hipEventDisableTiming));
+  } else {
+    UBG_HIP(hipEventSynchronize(T.used)); // the step two calls ago has read it
+  }
+  int rows = 0, maxw = 0;
+  bool buf = true;
+  for (int i = 0; i < n; i++) {
+    for (int k = 0; k < 4; k++) {
+      ubg::SimRec r = rec[(size_t)4 * i + k];
+      r.row0 = rows;
+      T.host[(size_t)k * n + i] = r;
+    }
+    rows += q[i].H - 2;
+    maxw = std::max(maxw, q[i].W);
+    buf = buf && (size_t)q[i].W * q[i].H * 4 < ((size_t)1 << 31);
+  }
+  UBG_HIP(hipMemcpyAsync(T.dev, T.host, nt * sizeof(ubg::SimRec), hipMemcpyHostToDevice, b->stream));
+  const ubg::SimRec *tab = T.dev;
+  // diffuse (two iterations), advect, setVBCs + divergence of all
+  // simulations, one launch each; sinks stamped in list order after the
+  // divergence
+  ubg::launch_sims_diffuse(tab, n, rows, maxw, b->stream);
+  ubg::launch_sims_diffuse(tab + n, n, rows, maxw, b->stream);
+  ubg::launch_sims_advect(tab + 2 * n, n, rows, maxw, buf, b->stream);
+  ubg::launch_sims_vbcs_divergence(tab + 3 * n, n, rows, maxw, b->stream);
+  UBG_TRY(check_launch());
+  for (int i = 0; i < n; i++)
+    if (sinks[i]) UBG_TRY(ubg::sim_apply_sinks_on(sims[i], b->stream));
+  // the V-cycles of all of them (the level-0 pressure and rhs gathered in
+  // one launch, the pressure scattered back in another)
   UBG_TRY(proj_members(b));
   std::vector<ubg::CopyJob> in(n), out(n);
   for (int i = 0; i < n; i++) {
@@ -2072,13 +2139,14 @@ UBG_API int ubg_sims_step(ubg_batch *b, ubg_sim *const *sims, int n, const float
     out[i] = ubg::CopyJob{d->p[0][d->cur[0]], nullptr, q[i].p, nullptr, q[i].W * q[i].H};
   }
   UBG_TRY(launch_copies(out, b->stream));
-  UBG_HIP(hipEventRecord(b->ev_solved, b->stream));
-  // ... and the rest of each step on its own stream again
-  for (int i = 0; i < n; i++) {
-    UBG_HIP(hipStreamWaitEvent(q[i].stream, b->ev_solved, 0));
-    UBG_TRY(ubg::sim_step_after_solve(sims[i]));
-  }
-  return check_launch();
+  // setPBC + correction + setVBCs + saveCurrent of all of them
+  ubg::launch_sims_correct_tail(tab + 3 * n, n, rows, maxw, b->stream);
+  UBG_TRY(check_launch());
+  UBG_HIP(hipEventRecord(T.used, b->stream));
+  // each simulation's next own work waits for this step
+  UBG_HIP(hipEventRecord(b->ev_done->ev, b->stream));
+  for (int i = 0; i < n; i++) ubg::sim_set_ext_event(sims[i], b->ev_done);
+  return UBG_OK;
 }
 
 UBG_API int ubg_batch_solve(ubg_batch *b, const ubg_ir_params *prm, int *graph_replays) {

@@ -143,6 +143,7 @@ UBG_API int ubg_mg_set_params(ubg_mg *m, const ubg_params *prm) {
   if (!m) return fail(UBG_E_INVALID, "null mg");
   UBG_TRY(check_params(prm));
   std::lock_guard<std::recursive_mutex> g(m->mu);
+  m->enter();
   m->prm = *prm;
   return UBG_OK;
 }
@@ -156,6 +157,7 @@ UBG_API int ubg_mg_num_levels(const ubg_mg *m, int *levels) {
 UBG_API int ubg_mg_update_fields(ubg_mg *m, const float *flag) {
   if (!m || !flag) return fail(UBG_E_INVALID, "null argument");
   std::lock_guard<std::recursive_mutex> g(m->mu);
+  m->enter();
   return mg_upload_flag(m, flag);
 }
 
@@ -175,6 +177,7 @@ UBG_API int ubg_mg_set_level_field(ubg_mg *m, int level, int which, const float 
   UBG_TRY(level_ptr(m, level, which, &d));
   if (!src) return fail(UBG_E_INVALID, "null source");
   std::lock_guard<std::recursive_mutex> g(m->mu);
+  m->enter();
   size_t n = (size_t)m->lv[level].w * m->lv[level].h;
   UBG_HIP(hipMemcpyAsync(d, src, n * 4, hipMemcpyHostToDevice, m->stream));
   UBG_HIP(hipStreamSynchronize(m->stream));
@@ -186,6 +189,7 @@ UBG_API int ubg_mg_get_level_field(ubg_mg *m, int level, int which, float *dst) 
   UBG_TRY(level_ptr(m, level, which, &d));
   if (!dst) return fail(UBG_E_INVALID, "null destination");
   std::lock_guard<std::recursive_mutex> g(m->mu);
+  m->enter();
   size_t n = (size_t)m->lv[level].w * m->lv[level].h;
   UBG_HIP(hipMemcpyAsync(dst, d, n * 4, hipMemcpyDeviceToHost, m->stream));
   UBG_HIP(hipStreamSynchronize(m->stream));
@@ -196,6 +200,7 @@ UBG_API int ubg_mg_get_level_flag(ubg_mg *m, int level, float *dst) {
   if (!m || !dst) return fail(UBG_E_INVALID, "null argument");
   if (level < 0 || level >= m->levels) return fail(UBG_E_INVALID, "level out of range");
   std::lock_guard<std::recursive_mutex> g(m->mu);
+  m->enter();
   const Level &L = m->lv[level];
   size_t n = (size_t)L.w * L.h;
   std::vector<uint8_t> tmp(n);
@@ -209,6 +214,7 @@ UBG_API int ubg_mg_level_op(ubg_mg *m, int op, int level, float h, float alpha) 
   if (!m) return fail(UBG_E_INVALID, "null mg");
   if (level < 0 || level >= m->levels) return fail(UBG_E_INVALID, "level out of range");
   std::lock_guard<std::recursive_mutex> g(m->mu);
+  m->enter();
   const Level &L = m->lv[level];
   switch (op) {
   case UBG_MGOP_RBGS:
@@ -241,6 +247,7 @@ UBG_API int ubg_mg_solve(ubg_mg *m, float h, int zg, int ncycles) {
   if (!m) return fail(UBG_E_INVALID, "null mg");
   if (ncycles < 0) return fail(UBG_E_INVALID, "negative cycle count");
   std::lock_guard<std::recursive_mutex> g(m->mu);
+  m->enter();
   for (int c = 0; c < ncycles; c++) m->solve_level(0, h, zg != 0);
   UBG_TRY(check_launch());
   return UBG_OK;
@@ -250,6 +257,7 @@ UBG_API int ubg_mg_solve_host(ubg_mg *m, float *p, const float *f, const float *
                               int zg) {
   if (!m || !p || !f) return fail(UBG_E_INVALID, "null argument");
   std::lock_guard<std::recursive_mutex> g(m->mu);
+  m->enter();
   const Level &L = m->lv[0];
   size_t n = (size_t)L.w * L.h;
   if (flag) {
@@ -271,6 +279,7 @@ UBG_API int ubg_mg_solve_host(ubg_mg *m, float *p, const float *f, const float *
 UBG_API int ubg_mg_rbgs(ubg_mg *m, float h, float alpha, int nsweeps) {
   if (!m) return fail(UBG_E_INVALID, "null mg");
   std::lock_guard<std::recursive_mutex> g(m->mu);
+  m->enter();
   float saved = m->prm.alpha;
   m->prm.alpha = alpha;
   m->sweeps(0, nsweeps, h, false);
@@ -283,6 +292,7 @@ UBG_API int ubg_mg_gs(ubg_mg *m, float h, float alpha, int nsweeps) {
   if (!m) return fail(UBG_E_INVALID, "null mg");
   if (nsweeps < 0) return fail(UBG_E_INVALID, "negative sweep count");
   std::lock_guard<std::recursive_mutex> g(m->mu);
+  m->enter();
   const Level &L = m->lv[0];
   for (int i = 0; i < nsweeps; i++) launch_gs(L.p, L.f, L.mask, L.w, L.h, h, alpha, m->stream);
   UBG_TRY(check_launch());
@@ -292,6 +302,7 @@ UBG_API int ubg_mg_gs(ubg_mg *m, float h, float alpha, int nsweeps) {
 UBG_API int ubg_mg_residual(ubg_mg *m, float h, float *r_dst, double *norm) {
   if (!m) return fail(UBG_E_INVALID, "null mg");
   std::lock_guard<std::recursive_mutex> g(m->mu);
+  m->enter();
   const Level &L = m->lv[0];
   size_t n = (size_t)L.w * L.h;
   launch_fill_f32(L.r, n, 0.0f, m->stream);
@@ -310,6 +321,10 @@ UBG_API int ubg_mg_residual(ubg_mg *m, float h, float *r_dst, double *norm) {
 
 UBG_API int ubg_mg_sync(ubg_mg *m) {
   if (!m) return fail(UBG_E_INVALID, "null mg");
+  {
+    std::lock_guard<std::recursive_mutex> g(m->mu);
+    m->enter();
+  }
   UBG_HIP(hipStreamSynchronize(m->stream));
   return UBG_OK;
 }
@@ -387,6 +402,13 @@ struct ubg_sim {
   };
   std::vector<G> graphs;
   std::recursive_mutex mu_;
+  // batched steps (ubg_sims_step) run this simulation's passes on a batch's
+  // stream: the event the batch recorded after them (waited for before this
+  // handle's next own work), and whether this handle's stream has had work
+  // of its own since the last batched step
+  std::shared_ptr<ExtEvent> ext_ev;
+  bool own_work = true;
+  void enter() { enter_stream(stream, ext_ev, own_work); }
 
   float *P() const { return mg->lv[0].p; }
 
@@ -463,7 +485,8 @@ struct ubg_sim {
   // patches of several sinks overlap the last one wins, so each cell's final
   // value is settled here and the device writes every cell once.  A patch
   // cell is (int)(g + d) of the float centre g, as the reference indexes it.
-  int apply_sinks() {
+  int apply_sinks() { return apply_sinks_on(stream); }
+  int apply_sinks_on(hipStream_t stream) {
     if (sinks.empty()) return UBG_OK;
     std::vector<int> cells;
     std::vector<float> val;
@@ -709,6 +732,10 @@ static int sim_create(const float *flag, int width, int height, float pwidth, fl
   (void)hipStreamSynchronize(s->stream);
   rc = mg_create(width, height, s->stream, s->p_buf[0], s->p_buf[1], s->f, s->flag, s->mask,
                  &s->mg);
+  if (rc == UBG_OK) {
+    s->mg->owner_ext = &s->ext_ev;
+    s->mg->owner_work = &s->own_work;
+  }
   if (rc) {
     (void)hipStreamDestroy(s->stream);
     s->pool.release();
@@ -741,6 +768,7 @@ UBG_API int ubg_sim_create_blank(int width, int height, float pwidth, float mu, 
 
 UBG_API int ubg_sim_destroy(ubg_sim *s) {
   if (!s) return UBG_OK;
+  s->enter(); // batched passes on this simulation's buffers end first
   (void)hipStreamSynchronize(s->stream);
   s->invalidate_graphs();
   for (auto &t : s->timers)
@@ -762,6 +790,7 @@ UBG_API int ubg_sim_set_params(ubg_sim *s, const ubg_params *prm) {
   if (!s) return fail(UBG_E_INVALID, "null sim");
   UBG_TRY(check_params(prm));
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->enter();
   s->prm = *prm;
   s->mg->prm = *prm;
   s->invalidate_graphs();
@@ -795,6 +824,7 @@ UBG_API int ubg_sim_set_field(ubg_sim *s, int field, const float *src) {
   if (!s || !src) return fail(UBG_E_INVALID, "null argument");
   if (field < 0 || field >= UBG_FIELD_COUNT) return fail(UBG_E_INVALID, "unknown field");
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->enter();
   s->vy_bc_fresh = false;
   int w, h;
   s->shape(field, &w, &h);
@@ -813,6 +843,7 @@ UBG_API int ubg_sim_get_field(ubg_sim *s, int field, float *dst) {
   if (!s || !dst) return fail(UBG_E_INVALID, "null argument");
   if (field < 0 || field >= UBG_FIELD_COUNT) return fail(UBG_E_INVALID, "unknown field");
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->enter();
   int w, h;
   s->shape(field, &w, &h);
   size_t n = (size_t)w * h;
@@ -838,12 +869,14 @@ UBG_API int ubg_sim_field_device_ptr(ubg_sim *s, int field, void **dptr) {
 UBG_API int ubg_sim_step(ubg_sim *s, float dt) {
   if (!s) return fail(UBG_E_INVALID, "null sim");
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->enter();
   return s->step(dt);
 }
 
 UBG_API int ubg_sim_advect_tracers(ubg_sim *s, float dt, uint32_t seed) {
   if (!s) return fail(UBG_E_INVALID, "null sim");
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->enter();
   launch_tracers(s->vel(), s->flag, s->h, s->pwidth, dt, seed, s->tr_xy, s->tr_age, s->tr_perm,
                  s->ntr, s->stream);
   return check_launch();
@@ -852,6 +885,7 @@ UBG_API int ubg_sim_advect_tracers(ubg_sim *s, float dt, uint32_t seed) {
 UBG_API int ubg_sim_run(ubg_sim *s, float dt, int nsteps) {
   if (!s) return fail(UBG_E_INVALID, "null sim");
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->enter();
   for (int i = 0; i < nsteps; i++) {
     UBG_TRY(s->step(dt));
     if (s->ntr > 0) {
@@ -865,6 +899,7 @@ UBG_API int ubg_sim_run(ubg_sim *s, float dt, int nsteps) {
 UBG_API int ubg_sim_stage(ubg_sim *s, int stage, float dt) {
   if (!s) return fail(UBG_E_INVALID, "null sim");
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->enter();
   s->dt = dt;
   switch (stage) {
   case UBG_STAGE_ACCUM:
@@ -895,6 +930,7 @@ UBG_API int ubg_sim_set_grids(ubg_sim *s, int x, int y, float val) {
   if (!s) return fail(UBG_E_INVALID, "null sim");
   if (val != 0.0f && val != 1.0f) return fail(UBG_E_FLAG_VALUE, "flag value must be 0 or 1");
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->enter();
   // simulation.hpp:82-98: out-of-range cells are ignored
   if (!(x >= 0 && y >= 0 && x < s->W && y < s->H)) return UBG_OK;
   s->vy_bc_fresh = false;
@@ -922,6 +958,7 @@ UBG_API int ubg_sim_set_bcs(ubg_sim *s, int west, int east, int north, int south
   for (int b : {west, east, north, south})
     if (b < BC_INFLOW || b > BC_NOSLIP) return fail(UBG_E_INVALID, "unknown boundary condition");
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->enter();
   if (s->bc.west == west && s->bc.east == east && s->bc.north == north && s->bc.south == south)
     return UBG_OK;
   s->bc.west = west;
@@ -945,6 +982,7 @@ UBG_API int ubg_sim_get_bcs(const ubg_sim *s, int *west, int *east, int *north, 
 UBG_API int ubg_sim_set_sinks(ubg_sim *s, const float *xyz, int n) {
   if (!s || n < 0 || (n > 0 && !xyz)) return fail(UBG_E_INVALID, "bad argument");
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->enter();
   s->sinks.clear();
   for (int i = 0; i < n; i++) s->sinks.push_back({xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]});
   return UBG_OK;
@@ -953,6 +991,7 @@ UBG_API int ubg_sim_set_sinks(ubg_sim *s, const float *xyz, int n) {
 UBG_API int ubg_sim_add_sink(ubg_sim *s, float x, float y, float z) {
   if (!s) return fail(UBG_E_INVALID, "null sim");
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->enter();
   s->sinks.push_back({x, y, z});
   return UBG_OK;
 }
@@ -978,6 +1017,7 @@ UBG_API int ubg_sim_bilinear_vel(ubg_sim *s, const float *c, float *out, int n) 
   if (!s || !c || !out || n < 0) return fail(UBG_E_INVALID, "bad argument");
   if (n == 0) return UBG_OK;
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->enter();
   float *d = nullptr;
   UBG_TRY(s->staging(sizeof(float) * 4 * n, (void **)&d));
   UBG_HIP(hipMemcpyAsync(d, c, sizeof(float) * 2 * n, hipMemcpyHostToDevice, s->stream));
@@ -997,6 +1037,7 @@ UBG_API int ubg_sim_sample_flag(ubg_sim *s, int mode, const float *pc, float *ou
   if (mode < UBG_FLAG_NEAREST || mode > UBG_FLAG_NORMAL) return fail(UBG_E_INVALID, "unknown mode");
   if (n == 0) return UBG_OK;
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->enter();
   const int nout = mode == UBG_FLAG_NORMAL ? 2 * n : n;
   float *d = nullptr;
   UBG_TRY(s->staging(sizeof(float) * (2 * (size_t)n + nout), (void **)&d));
@@ -1015,6 +1056,7 @@ UBG_API int ubg_sim_sample_flag(ubg_sim *s, int mode, const float *pc, float *ou
 UBG_API int ubg_sim_get_dt(ubg_sim *s, float *dt) {
   if (!s || !dt) return fail(UBG_E_INVALID, "null argument");
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->enter();
   const size_t n = (size_t)(s->W - 2) * (s->H - 2);
   const int nb = (int)std::min<size_t>(1024, (n + 255) / 256);
   float *d = nullptr;
@@ -1036,6 +1078,7 @@ UBG_API int ubg_sim_get_dt(ubg_sim *s, float *dt) {
 UBG_API int ubg_sim_center_p(ubg_sim *s) {
   if (!s) return fail(UBG_E_INVALID, "null sim");
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->enter();
   const size_t n = (size_t)(s->W - 2) * (s->H - 2);
   const int nb = (int)std::min<size_t>(1024, (n + 255) / 256);
   double *d = nullptr;
@@ -1060,6 +1103,7 @@ UBG_API int ubg_sim_advect_items(ubg_sim *s, ubg_item *items, int n, float game_
   if (!std::isfinite(game_dt)) return fail(UBG_E_INVALID, "game_dt must be finite");
   if (n == 0) return UBG_OK;
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->enter();
   const size_t bytes = sizeof(ubg_item) * n + sizeof(float) * 2 * n;
   char *d = nullptr;
   UBG_TRY(s->staging(bytes, (void **)&d));
@@ -1089,6 +1133,7 @@ UBG_API int ubg_sim_shift_map(ubg_sim *s, const float *new_flag) {
   if (!s || !new_flag) return fail(UBG_E_INVALID, "null argument");
   if (s->W > 16384) return fail(UBG_E_INVALID, "map shift needs W <= 16384");
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->enter();
   std::vector<uint8_t> f8;
   UBG_TRY(validate_flag(new_flag, (size_t)s->W * s->H, f8));
   uint8_t *d = nullptr;
@@ -1142,6 +1187,7 @@ UBG_API int ubg_terrain_flags(const uint8_t *rgba, int width, int height, int sc
 UBG_API int ubg_sim_collocate(ubg_sim *s, float *vxy, float *mag, int on_device) {
   if (!s || !vxy || !mag) return fail(UBG_E_INVALID, "null argument");
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->enter();
   const size_t n = (size_t)(2 * s->W - 1) * (2 * s->H - 1);
   if (on_device) {
     launch_collocate(s->vx_cur, s->vy_cur, vxy, mag, s->W, s->H, s->stream);
@@ -1165,6 +1211,7 @@ UBG_API int ubg_sim_collocate(ubg_sim *s, float *vxy, float *mag, int on_device)
 UBG_API int ubg_sim_set_tracers(ubg_sim *s, const float *xy, const float *age, int n) {
   if (!s || n < 0 || (n > 0 && !xy)) return fail(UBG_E_INVALID, "bad argument");
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->enter();
   if (n > s->tr_cap) {
     if (s->tr_xy) (void)hipFree(s->tr_xy);
     if (s->tr_age) (void)hipFree(s->tr_age);
@@ -1208,6 +1255,7 @@ UBG_API int ubg_sim_set_tracers(ubg_sim *s, const float *xy, const float *age, i
 UBG_API int ubg_sim_get_tracers(ubg_sim *s, float *xy, float *age, int n) {
   if (!s || n < 0 || n > s->ntr) return fail(UBG_E_INVALID, "bad argument");
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->enter();
   if (n == 0) return UBG_OK;
   const int N = s->ntr;
   std::vector<float> sxy(2 * (size_t)N), sage((size_t)N);
@@ -1229,6 +1277,10 @@ UBG_API int ubg_sim_get_tracers(ubg_sim *s, float *xy, float *age, int n) {
 
 UBG_API int ubg_sim_sync(ubg_sim *s) {
   if (!s) return fail(UBG_E_INVALID, "null sim");
+  {
+    std::lock_guard<std::recursive_mutex> g(s->mu_);
+    s->enter();
+  }
   UBG_HIP(hipStreamSynchronize(s->stream));
   return UBG_OK;
 }
@@ -1236,6 +1288,7 @@ UBG_API int ubg_sim_sync(ubg_sim *s) {
 UBG_API int ubg_sim_timer_enable(ubg_sim *s, int timer, int on) {
   if (!s || timer < 0 || timer >= UBG_TIMER_COUNT) return fail(UBG_E_INVALID, "bad argument");
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->enter();
   UBG_HIP(hipStreamSynchronize(s->stream));
   Timer &t = s->timers[timer];
   t.on = on != 0;
@@ -1248,6 +1301,7 @@ UBG_API int ubg_sim_timer_enable(ubg_sim *s, int timer, int on) {
 UBG_API int ubg_sim_timer_read(ubg_sim *s, int timer, double *total_ms, long *launches) {
   if (!s || timer < 0 || timer >= UBG_TIMER_COUNT) return fail(UBG_E_INVALID, "bad argument");
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->enter();
   UBG_HIP(hipStreamSynchronize(s->stream));
   Timer &t = s->timers[timer];
   for (size_t i = 0; i + 1 < t.used; i += 2) {
@@ -1269,14 +1323,87 @@ namespace ubg {
 
 int sim_step_before_solve(ubg_sim *s, float dt) {
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->enter();
   UBG_TRY(s->step_before_solve(dt));
   return check_launch();
 }
 
 int sim_step_after_solve(ubg_sim *s) {
   std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->enter();
   s->body_after_solve();
   return check_launch();
+}
+
+int sim_batched_records(ubg_sim *s, float dt, SimRec rec[4], bool *has_sinks) {
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  if (s->accum_dirty || !s->vy_bc_fresh || s->fmask_gen != s->mg->flag_gen)
+    s->enter(); // the prelude below enqueues on the simulation's stream
+  s->dt = dt;
+  if (s->accum_dirty) {
+    launch_accumulate(s->vel(), s->vx_acc, s->vy_acc, s->stream);
+    s->accum_dirty = false;
+    s->vy_bc_fresh = false;
+  }
+  s->refresh_vy_bcs();
+  s->ensure_fmask();
+  UBG_TRY(check_launch());
+  const float ih = 1.0f / s->h;
+  auto rec_of = [&](SimRec &r) {
+    const VelView v = s->vel();
+    r = SimRec{};
+    r.vxf = v.vxf;
+    r.vxb = v.vxb;
+    r.vyf = v.vyf;
+    r.vyb = v.vyb;
+    r.fmask = s->fmask;
+    r.flag = s->flag;
+    r.f = s->f;
+    r.p = s->P();
+    r.xcur = s->prm.save_current ? s->vx_cur : nullptr;
+    r.ycur = s->prm.save_current ? s->vy_cur : nullptr;
+    r.W = s->W;
+    r.H = s->H;
+    r.west = s->bc.west;
+    r.east = s->bc.east;
+    r.north = s->bc.north;
+    r.south = s->bc.south;
+    r.a = dt * s->mu * (s->W - 1.0f) / s->pwidth; // as diffuse_fused
+    r.hdt = 0.5f * dt * ih;                       // as advect
+    r.fdt = dt * ih;
+    r.ih = ih;
+  };
+  // the roles of step_body's passes: two diffuse iterations and advect each
+  // flip both velocity buffers; setVBCs + divergence and the correction
+  // tail see the roles after the third flip
+  for (int k = 0; k < 4; k++) {
+    rec_of(rec[k]);
+    if (k < 3) {
+      s->fx = 1 - s->fx;
+      s->fy = 1 - s->fy;
+    }
+  }
+  s->vy_bc_fresh = true; // after the correction tail
+  *has_sinks = !s->sinks.empty();
+  return UBG_OK;
+}
+
+int sim_apply_sinks_on(ubg_sim *s, hipStream_t st) {
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  return s->apply_sinks_on(st);
+}
+
+bool sim_take_own_work(ubg_sim *s) {
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  const bool w = s->own_work;
+  s->own_work = false;
+  return w;
+}
+
+void sim_set_ext_event(ubg_sim *s, const std::shared_ptr<ExtEvent> &ev) {
+  std::lock_guard<std::recursive_mutex> g(s->mu_);
+  s->ext_ev = ev;
+  s->own_work = false;
 }
 
 SimLevel0 sim_level0(ubg_sim *s) {

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -162,8 +163,36 @@ struct IRState; // FP64 iterative refinement buffers (refine.hip)
 // ===========================================================================
 // MG
 // ===========================================================================
+namespace ubg {
+// An event another stream recorded after work on a handle's buffers (a
+// batched step of several simulations).  Shared by the batch that records
+// it and the handles that must wait for it; destroyed with its last holder.
+struct ExtEvent {
+  hipEvent_t ev = nullptr;
+  ~ExtEvent() {
+    if (ev) (void)hipEventDestroy(ev);
+  }
+};
+// Before a handle enqueues work on `stream`: wait for the pending external
+// event, if any, and note that the handle's stream has work of its own.
+inline void enter_stream(hipStream_t stream, std::shared_ptr<ExtEvent> &ext, bool &own_work) {
+  if (ext) {
+    (void)hipStreamWaitEvent(stream, ext->ev, 0);
+    ext.reset();
+  }
+  own_work = true;
+}
+} // namespace ubg
+
 struct ubg_mg {
   int W = 0, H = 0, levels = 0;
+  // a simulation's MG: its owner's external event and work flag (null for a
+  // stand-alone MG)
+  std::shared_ptr<ubg::ExtEvent> *owner_ext = nullptr;
+  bool *owner_work = nullptr;
+  void enter() {
+    if (owner_ext) ubg::enter_stream(stream, *owner_ext, *owner_work);
+  }
   std::vector<Level> lv;
   Pool pool;
   hipStream_t stream = nullptr;
@@ -262,6 +291,18 @@ SimLevel0 sim_level0(ubg_sim *s);
 // stages after them (setPBC + correction + setVBCs + saveCurrent)
 int sim_step_before_solve(ubg_sim *s, float dt);
 int sim_step_after_solve(ubg_sim *s);
+// the whole-batch form: the applyAccumulatedVelocity / vy-boundary /
+// face-factor prelude on the simulation's stream, then the records of the
+// four per-cell stages of its step (two diffuse iterations, advect, setVBCs
+// + divergence and correction tail); the simulation's buffer roles advance
+// as its step would leave them
+int sim_batched_records(ubg_sim *s, float dt, SimRec rec[4], bool *has_sinks);
+int sim_apply_sinks_on(ubg_sim *s, hipStream_t st);
+// whether the simulation's stream has had work since the last batched step
+// (clears the mark)
+bool sim_take_own_work(ubg_sim *s);
+// the event the simulation's next own work waits for
+void sim_set_ext_event(ubg_sim *s, const std::shared_ptr<ExtEvent> &ev);
 
 // Build an MG hierarchy.  If level-0 buffers are given they are borrowed
 // (the simulation's p, f, flag, mask); otherwise the MG owns them.

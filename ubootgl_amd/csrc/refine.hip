@@ -181,6 +181,7 @@ UBG_API void ubg_ir_params_default(ubg_ir_params *out) {
 UBG_API int ubg_mg_ir_set_rhs(ubg_mg *m, const float *f) {
   if (!m || !f) return fail(UBG_E_INVALID, "null argument");
   std::lock_guard<std::recursive_mutex> g(m->mu);
+  m->enter();
   UBG_TRY(ir_ensure(m));
   const size_t n = (size_t)m->W * m->H;
   for (size_t i = 0; i < n; i++)
@@ -193,6 +194,7 @@ UBG_API int ubg_mg_ir_set_rhs(ubg_mg *m, const float *f) {
 UBG_API int ubg_mg_ir_set_guess(ubg_mg *m, const double *x) {
   if (!m) return fail(UBG_E_INVALID, "null mg");
   std::lock_guard<std::recursive_mutex> g(m->mu);
+  m->enter();
   UBG_TRY(ir_ensure(m));
   const size_t n = (size_t)m->W * m->H;
   double *d = m->ir->x[m->ir->cur];
@@ -220,6 +222,7 @@ UBG_API int ubg_mg_ir_solve(ubg_mg *m, float h, const ubg_ir_params *prm, ubg_ir
   if (!(P.ref_norm >= 0.0) || !std::isfinite(P.ref_norm))
     return fail(UBG_E_INVALID, "ref_norm must be finite and >= 0");
   std::lock_guard<std::recursive_mutex> g(m->mu);
+  m->enter();
   UBG_TRY(ir_ensure(m));
   std::memset(res, 0, sizeof(*res));
   const double hd = (double)h;
@@ -273,6 +276,7 @@ UBG_API int ubg_mg_ir_solve(ubg_mg *m, float h, const ubg_ir_params *prm, ubg_ir
 UBG_API int ubg_mg_ir_get_solution(ubg_mg *m, double *x) {
   if (!m || !x) return fail(UBG_E_INVALID, "null argument");
   std::lock_guard<std::recursive_mutex> g(m->mu);
+  m->enter();
   if (!m->ir) return fail(UBG_E_STATE, "no refinement solve has run");
   const size_t n = (size_t)m->W * m->H;
   UBG_HIP(hipMemcpyAsync(x, m->ir->x[m->ir->cur], n * 8, hipMemcpyDeviceToHost, m->stream));
@@ -283,6 +287,7 @@ UBG_API int ubg_mg_ir_get_solution(ubg_mg *m, double *x) {
 UBG_API int ubg_mg_ir_device_ptrs(ubg_mg *m, double **x, float **f) {
   if (!m) return fail(UBG_E_INVALID, "null mg");
   std::lock_guard<std::recursive_mutex> g(m->mu);
+  m->enter();
   UBG_TRY(ir_ensure(m));
   if (x) *x = m->ir->x[m->ir->cur];
   if (f) *f = m->ir->f;

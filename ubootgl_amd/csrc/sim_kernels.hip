@@ -214,13 +214,12 @@ __global__ void k_face_masks(const uint8_t *__restrict__ fl, uint16_t *__restric
 // refreshes it in the setVBCs after the first vx iteration).  The flag
 // products come from the face factors: s * F1 * F2 evaluated left to right
 // equals s * (F1 AND F2) for 0/1 flags, signed zeros included.
-__global__ void k_diffuse_both(const float *__restrict__ sx, float *__restrict__ dx,
-                               const float *__restrict__ sy, float *__restrict__ dy,
-                               const uint16_t *__restrict__ fm, int W, int H, float a,
-                               BCGrid gx, BCGrid gy) {
+__device__ __forceinline__ void diffuse_cell(int x, int y, const float *__restrict__ sx,
+                                             float *__restrict__ dx, const float *__restrict__ sy,
+                                             float *__restrict__ dy,
+                                             const uint16_t *__restrict__ fm, int W, int H,
+                                             float a, const BCGrid &gx, const BCGrid &gy) {
   const int Wx = W - 1, Hy = H - 1;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int y = blockIdx.y + 1;
   if (x >= W - 1) return;
   const unsigned m = fm[(size_t)y * W + x];
   float vxn = 0.0f, vyn = 0.0f;
@@ -256,6 +255,13 @@ __global__ void k_diffuse_both(const float *__restrict__ sx, float *__restrict__
     if (x < Wx - 1) bc_duty(gx, x, y, vxn, nullptr);
     if (y < Hy - 1) bc_duty(gy, x, y, vyn, nullptr);
   }
+}
+__global__ void k_diffuse_both(const float *__restrict__ sx, float *__restrict__ dx,
+                               const float *__restrict__ sy, float *__restrict__ dy,
+                               const uint16_t *__restrict__ fm, int W, int H, float a,
+                               BCGrid gx, BCGrid gy) {
+  diffuse_cell(blockIdx.x * blockDim.x + threadIdx.x + 1, blockIdx.y + 1, sx, dx, sy, dy, fm, W, H,
+               a, gx, gy);
 }
 
 // CubicHermite (interpolators.hpp:78-85), Catmull-Rom, evaluated as
@@ -355,13 +361,12 @@ __device__ __forceinline__ float bicubic(const float *__restrict__ g, int gw, in
 // range (nck is a count of chunks), so one ballot of the lanes' own tests
 // answers it for every chunk of the wave.
 template <bool BUF>
-__device__ __forceinline__ void advect_pair(const float *__restrict__ xf,
+__device__ __forceinline__ void advect_pair(int lane, int y, const float *__restrict__ xf,
                                             const float *__restrict__ yf,
                                             float *__restrict__ xb, float *__restrict__ yb,
                                             const uint8_t *__restrict__ fl, int W, int H,
                                             int nck_x, int nck_y, float hdt, float fdt) {
-  const int lane = blockIdx.x * blockDim.x + threadIdx.x; // 0-based: x = lane + 1
-  const int y = blockIdx.y + 1;
+  // lane: 0-based (x = lane + 1), lane & 63 == threadIdx.x & 63
   const int Wx = W - 1, Hy = H - 1;
   const bool in_x = lane < 8 * nck_x, in_y = lane < 8 * nck_y;
   // every lane of either range has x <= W - 2, so x - 1 .. x + 1 and rows
@@ -417,7 +422,8 @@ __global__ void k_advect(const float *__restrict__ xf, const float *__restrict__
                          float *__restrict__ xb, float *__restrict__ yb,
                          const uint8_t *__restrict__ fl, int W, int H, int nck_x, int nck_y,
                          float hdt, float fdt) {
-  advect_pair<BUF>(xf, yf, xb, yb, fl, W, H, nck_x, nck_y, hdt, fdt);
+  advect_pair<BUF>(blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y + 1, xf, yf, xb, yb, fl, W, H,
+                   nck_x, nck_y, hdt, fdt);
 }
 
 // Collocated velocity for rendering (interp_shader.cs:14-33, dispatched by
@@ -614,12 +620,13 @@ __device__ __forceinline__ void pbc_duty(float *__restrict__ p, int w, int h, in
 // face, so the copy into vx_cur / vy_cur is complete (cur may be null).  A
 // thread issues all its loads before its first store: 12% less time on
 // gfx950 than loading each face's operands beside its own store.
-__global__ void k_correct_tail(float *__restrict__ xf, float *__restrict__ yf,
-                               float *__restrict__ p, const uint16_t *__restrict__ fm, int W,
-                               int H, float ih, BCGrid gx, BCGrid gy, float *__restrict__ xcur,
-                               float *__restrict__ ycur, int bw, int be, int bn, int bs) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int y = blockIdx.y + 1;
+__device__ __forceinline__ void correct_tail_cell(int x, int y, float *__restrict__ xf,
+                                                  float *__restrict__ yf, float *__restrict__ p,
+                                                  const uint16_t *__restrict__ fm, int W, int H,
+                                                  float ih, const BCGrid &gx, const BCGrid &gy,
+                                                  float *__restrict__ xcur,
+                                                  float *__restrict__ ycur, int bw, int be,
+                                                  int bn, int bs) {
   if (x >= W - 1) return;
   const int Wx = W - 1;
   const bool hx = x < W - 2, hy = y < H - 2; // the thread owns a vx / vy face
@@ -653,17 +660,25 @@ __global__ void k_correct_tail(float *__restrict__ xf, float *__restrict__ yf,
     pbc_duty(p, W, H, x, y, bw, be, bn, bs);
   }
 }
+__global__ void k_correct_tail(float *__restrict__ xf, float *__restrict__ yf,
+                               float *__restrict__ p, const uint16_t *__restrict__ fm, int W,
+                               int H, float ih, BCGrid gx, BCGrid gy, float *__restrict__ xcur,
+                               float *__restrict__ ycur, int bw, int be, int bn, int bs) {
+  correct_tail_cell(blockIdx.x * blockDim.x + threadIdx.x + 1, blockIdx.y + 1, xf, yf, p, fm, W,
+                    H, ih, gx, gy, xcur, ycur, bw, be, bn, bs);
+}
 
 // setVBCs after advect (simulation.cpp:369) fused with project's divergence
 // (:166-171).  Thread (x, y) owns the vx face (x, y) and the vy face (x, y):
 // it issues their boundary writes and computes f at cell (x, y) from the
 // face values after setVBCs -- a boundary face it reads is the side / edge
 // value of the rule applied to its interior neighbour (never a corner).
-__global__ void k_vbcs_divergence(const float *__restrict__ xf, const float *__restrict__ yf,
-                                  float *__restrict__ f, int W, int H, float ih, BCGrid gx,
-                                  BCGrid gy) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int y = blockIdx.y + 1;
+__device__ __forceinline__ void vbcs_divergence_cell(int x, int y,
+                                                     const float *__restrict__ xf,
+                                                     const float *__restrict__ yf,
+                                                     float *__restrict__ f, int W, int H,
+                                                     float ih, const BCGrid &gx,
+                                                     const BCGrid &gy) {
   if (x >= W - 1) return;
   const int Wx = W - 1, Hy = H - 1;
   // vx faces x - 1 and x of row y, vy faces y - 1 and y of column x (all
@@ -688,6 +703,12 @@ __global__ void k_vbcs_divergence(const float *__restrict__ xf, const float *__r
     if (x <= Wx - 2) bc_duty(gx, x, y, xe, nullptr);
     if (y <= Hy - 2) bc_duty(gy, x, y, yn, nullptr);
   }
+}
+__global__ void k_vbcs_divergence(const float *__restrict__ xf, const float *__restrict__ yf,
+                                  float *__restrict__ f, int W, int H, float ih, BCGrid gx,
+                                  BCGrid gy) {
+  vbcs_divergence_cell(blockIdx.x * blockDim.x + threadIdx.x + 1, blockIdx.y + 1, xf, yf, f, W, H,
+                       ih, gx, gy);
 }
 
 // scalar bilinearSample (interpolators.hpp:11-27), clamp to [0, n-1.1]
@@ -882,6 +903,54 @@ __global__ void k_gs_diagonal(float *__restrict__ p, const float *__restrict__ f
   p[c] = smooth(p[c], p[c - 1], p[c + 1], p[c - W], p[c + W], f[c], mask[c], h, alpha);
 }
 
+// ---- batched steps of several simulations ---------------------------------
+__device__ __forceinline__ const SimRec &sim_of_row(const SimRec *tab, int n, int row) {
+  int lo = 0, hi = n; // the last simulation whose row0 <= row
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tab[mid].row0 <= row) lo = mid;
+    else hi = mid;
+  }
+  return tab[lo];
+}
+
+// blockIdx.x: interior row in the step's numbering; blockIdx.y: column chunk
+__global__ void kb_sims_diffuse(const SimRec *__restrict__ tab, int n) {
+  const SimRec &S = sim_of_row(tab, n, blockIdx.x);
+  const BCGrid gx{S.vxb, S.vxf, S.W - 1, S.H, S.west, S.east, S.south, S.north, true};
+  const BCGrid gy{S.vyb, S.vyf, S.W, S.H - 1, S.west, S.east, S.south, S.north, false};
+  diffuse_cell(blockIdx.y * blockDim.x + threadIdx.x + 1, blockIdx.x - S.row0 + 1, S.vxf, S.vxb,
+               S.vyf, S.vyb, S.fmask, S.W, S.H, S.a, gx, gy);
+}
+
+template <bool BUF> __global__ void kb_sims_advect(const SimRec *__restrict__ tab, int n) {
+  const SimRec &S = sim_of_row(tab, n, blockIdx.x);
+  const int Wx = S.W - 1;
+  const int nck_x = Wx > 9 ? (Wx - 2) / 8 : 0, nck_y = S.W > 9 ? (S.W - 2) / 8 : 0;
+  const int lanes = 8 * (nck_x > nck_y ? nck_x : nck_y);
+  const int lane = blockIdx.y * blockDim.x + threadIdx.x;
+  if ((int)(blockIdx.y * blockDim.x) >= lanes) return; // whole block past this row's lanes
+  advect_pair<BUF>(lane, blockIdx.x - S.row0 + 1, S.vxf, S.vyf, S.vxb, S.vyb, S.flag, S.W, S.H,
+                   nck_x, nck_y, S.hdt, S.fdt);
+}
+
+__global__ void kb_sims_vbcs_divergence(const SimRec *__restrict__ tab, int n) {
+  const SimRec &S = sim_of_row(tab, n, blockIdx.x);
+  const BCGrid gx{S.vxf, S.vxb, S.W - 1, S.H, S.west, S.east, S.south, S.north, true};
+  const BCGrid gy{S.vyf, S.vyb, S.W, S.H - 1, S.west, S.east, S.south, S.north, false};
+  vbcs_divergence_cell(blockIdx.y * blockDim.x + threadIdx.x + 1, blockIdx.x - S.row0 + 1, S.vxf,
+                       S.vyf, S.f, S.W, S.H, S.ih, gx, gy);
+}
+
+__global__ void kb_sims_correct_tail(const SimRec *__restrict__ tab, int n) {
+  const SimRec &S = sim_of_row(tab, n, blockIdx.x);
+  const BCGrid gx{S.vxf, S.vxb, S.W - 1, S.H, S.west, S.east, S.south, S.north, true};
+  const BCGrid gy{S.vyf, S.vyb, S.W, S.H - 1, S.west, S.east, S.south, S.north, false};
+  correct_tail_cell(blockIdx.y * blockDim.x + threadIdx.x + 1, blockIdx.x - S.row0 + 1, S.vxf,
+                    S.vyf, S.p, S.fmask, S.W, S.H, S.ih, gx, gy, S.xcur, S.ycur, S.west, S.east,
+                    S.north, S.south);
+}
+
 } // namespace
 
 // ---------------------------------------------------------------------------
@@ -1009,6 +1078,28 @@ void launch_gs(float *p, const float *f, const uint8_t *mask, int W, int H, floa
     hipLaunchKernelGGL(k_gs_diagonal, dim3(ceil_div(y1 - y0 + 1, TPB)), dim3(TPB), 0, s, p, f, mask,
                        W, H, h, alpha, d);
   }
+}
+void launch_sims_diffuse(const SimRec *tab, int n, int rows, int max_w, hipStream_t s) {
+  if (n <= 0 || rows <= 0) return;
+  hipLaunchKernelGGL(kb_sims_diffuse, dim3(rows, ceil_div(max_w - 2, TPB)), dim3(TPB), 0, s, tab,
+                     n);
+}
+void launch_sims_advect(const SimRec *tab, int n, int rows, int max_w, bool buf, hipStream_t s) {
+  if (n <= 0 || rows <= 0) return;
+  const int lanes = 8 * ((max_w - 2) / 8); // a bound on every simulation's chunked lanes
+  if (lanes <= 0) return;
+  hipLaunchKernelGGL(buf ? kb_sims_advect<true> : kb_sims_advect<false>,
+                     dim3(rows, ceil_div(lanes, TPB)), dim3(TPB), 0, s, tab, n);
+}
+void launch_sims_vbcs_divergence(const SimRec *tab, int n, int rows, int max_w, hipStream_t s) {
+  if (n <= 0 || rows <= 0) return;
+  hipLaunchKernelGGL(kb_sims_vbcs_divergence, dim3(rows, ceil_div(max_w - 2, TPB)), dim3(TPB), 0,
+                     s, tab, n);
+}
+void launch_sims_correct_tail(const SimRec *tab, int n, int rows, int max_w, hipStream_t s) {
+  if (n <= 0 || rows <= 0) return;
+  hipLaunchKernelGGL(kb_sims_correct_tail, dim3(rows, ceil_div(max_w - 2, TPB)), dim3(TPB), 0, s,
+                     tab, n);
 }
 void launch_tracers(const VelView &v, const uint8_t *flag, float h, float pwidth, float dt,
                     uint32_t seed, float *xy, float *age, const int *perm, int n, hipStream_t s) {

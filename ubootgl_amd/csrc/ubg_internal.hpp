@@ -127,6 +127,30 @@ void launch_correct_tail(const VelView &v, float *p, const uint16_t *fmask, floa
                          const BCs &bc, float *xcur, float *ycur, hipStream_t s);
 void launch_accumulate(const VelView &v, float *vx_acc, float *vy_acc, hipStream_t s);
 void launch_advect(const VelView &v, const uint8_t *flag, float hdt, float fdt, hipStream_t s);
+
+// One simulation in a batched step of several (ubg_sims_step): the buffers
+// and constants its per-cell passes read at one stage, and the first row of
+// its interior in the step's row numbering (row0 = sum of H - 2 over the
+// simulations before it).  A launch covers every simulation's interior
+// rows, one block row per interior row, found by binary search on row0.
+struct SimRec {
+  float *vxf, *vxb, *vyf, *vyb; // front / back roles at this stage
+  const uint16_t *fmask;
+  const uint8_t *flag;
+  float *f, *p, *xcur, *ycur; // xcur / ycur may be null
+  int W, H;
+  int west, east, north, south; // boundary-condition kinds
+  float a;                      // diffuse: dt mu (W - 1) / pwidth
+  float hdt, fdt, ih;           // advect: dt / 2h, dt / h; 1 / h
+  int row0;
+};
+// the stages of Simulation::step after applyAccumulatedVelocity for n
+// simulations: one diffuse iteration, advect, setVBCs + divergence,
+// setPBC + correction + setVBCs + saveCurrent; rows = total interior rows
+void launch_sims_diffuse(const SimRec *tab, int n, int rows, int max_w, hipStream_t s);
+void launch_sims_advect(const SimRec *tab, int n, int rows, int max_w, bool buf, hipStream_t s);
+void launch_sims_vbcs_divergence(const SimRec *tab, int n, int rows, int max_w, hipStream_t s);
+void launch_sims_correct_tail(const SimRec *tab, int n, int rows, int max_w, hipStream_t s);
 void launch_divergence(const VelView &v, float *f, float ih, hipStream_t s);
 // f[cell[i]] = val[i] for n distinct cells
 void launch_stamp_sinks(float *f, const int *cell, const float *val, int n, hipStream_t s);
