@@ -148,3 +148,49 @@ def test_projection_batch_errors(gpu):
     with pytest.raises(L.UbgError) as e:
         B.SimBatch([s1, s3])
     assert e.value.code == L.E_DOMAIN_TOO_SMALL
+
+
+def test_sims_step_interleaved_with_own_work(gpu):
+    """Batched steps mixed with each simulation's own work: a step of its
+    own between batched steps, host writes of a velocity field and of the
+    accumulation field (the next batched step runs applyAccumulatedVelocity
+    on the simulation's stream first), reads right after a batched step,
+    and a batched step after a destroyed-and-rebuilt batch.  Both sets stay
+    bit-identical to the simulations stepped alone."""
+    sims = make_sims(SPECS[:3], None)
+    sb = B.SimBatch([b for _, b, _ in sims])
+    dts = [dt for _, _, dt in sims]
+    g = np.random.default_rng(9)
+
+    def both(fn):
+        for a, b, dt in sims:
+            fn(a, dt)
+            fn(b, dt)
+
+    def check(tag):
+        for k, (a, b, _) in enumerate(sims):
+            for name in FIELDS:
+                np.testing.assert_array_equal(b.get(name), a.get(name),
+                                              err_msg=f"{tag} sim {k} {name}")
+
+    for a, _, dt in sims:
+        a.step(dt)
+    sb.step(dts)
+    check("first")
+    both(lambda s, dt: s.step(dt))  # own steps between batched ones
+    check("own step")
+    for a, b, _ in sims:
+        acc = (g.standard_normal(a.shape("vx_accum")) * 1e-3).astype(np.float32)
+        a.set("vx_accum", acc)
+        b.set("vx_accum", acc)
+    for a, _, dt in sims:
+        a.step(dt)
+    sb.step(dts)
+    check("accumulation")
+    sb.close()
+    sb2 = B.SimBatch([b for _, b, _ in sims])
+    for a, _, dt in sims:
+        a.step(dt)
+    sb2.step(dts)
+    check("new batch")
+    sb2.close()
