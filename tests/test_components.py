@@ -111,6 +111,22 @@ def test_oracle_mean_removal(name, flag):
     assert np.all(out_c[fl] == 0.0)
 
 
+def test_component_entry_points_reject_bad_input():
+    """Argument errors are found on the host, before any device work (so
+    they hold without a GPU too)."""
+    from ubootgl_amd import _lib as L
+    bad = np.ones((8, 8), np.float32)
+    bad[3, 3] = 0.5
+    with pytest.raises(L.UbgError) as e:
+        U.flag_components(bad)
+    assert e.value.code == L.E_FLAG_VALUE
+    f = np.zeros((8, 8), np.float32)
+    f[2, 2] = np.nan
+    with pytest.raises(L.UbgError) as e:
+        U.remove_component_means(np.ones((8, 8), np.float32), f)
+    assert e.value.code == L.E_NONFINITE
+
+
 # ------------------------------------------------------------------- GPU
 
 

@@ -2066,6 +2066,22 @@ UBG_API int ubg_sims_step(ubg_batch *b, ubg_sim *const *sims, int n, const float
     UBG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     b->ev_ready.push_back(e);
   }
+  // a free table slot (allocated before any simulation's state moves)
+  auto &T = b->slot[b->next_slot];
+  b->next_slot ^= 1;
+  const size_t nt = (size_t)4 * n;
+  if (!T.used) UBG_HIP(hipEventCreateWithFlags(&T.used, hipEventDisableTiming));
+  UBG_HIP(hipEventSynchronize(T.used)); // the step two calls ago has read it
+  if (T.cap < nt) {
+    if (T.host) (void)hipHostFree(T.host);
+    if (T.dev) (void)hipFree(T.dev);
+    T.host = nullptr;
+    T.dev = nullptr;
+    T.cap = 0;
+    UBG_HIP(hipHostMalloc((void **)&T.host, nt * sizeof(ubg::SimRec), hipHostMallocDefault));
+    UBG_HIP(hipMalloc((void **)&T.dev, nt * sizeof(ubg::SimRec)));
+    T.cap = nt;
+  }
   // The per-cell stages of every simulation's step as records (their
   // applyAccumulatedVelocity / boundary / face-factor preludes, when due,
   // run on the simulations' own streams); the batch's stream then waits for
@@ -2084,22 +2100,6 @@ UBG_API int ubg_sims_step(ubg_batch *b, ubg_sim *const *sims, int n, const float
     }
   // stage tables: stage k of simulation i at [k n + i], rows numbered
   // through the simulations' interiors
-  auto &T = b->slot[b->next_slot];
-  b->next_slot ^= 1;
-  const size_t nt = (size_t)4 * n;
-  if (T.cap < nt) {
-    if (T.host) (void)hipHostFree(T.host);
-    if (T.dev) (void)hipFree(T.dev);
-    T.host = nullptr;
-    T.dev = nullptr;
-    T.cap = 0;
-    UBG_HIP(hipHostMalloc((void **)&T.host, nt * sizeof(ubg::SimRec), hipHostMallocDefault));
-    UBG_HIP(hipMalloc((void **)&T.dev, nt * sizeof(ubg::SimRec)));
-    T.cap = nt;
-    if (!T.used) UBG_HIP(hipEventCreateWithFlags(&T.used, hipEventDisableTiming));
-  } else {
-    UBG_HIP(hipEventSynchronize(T.used)); // the step two calls ago has read it
-  }
   int rows = 0, maxw = 0;
   bool buf = true;
   for (int i = 0; i < n; i++) {

@@ -250,11 +250,11 @@ int setup(const float *flag, int W, int H, Scratch &sc) {
   if (!flag) return fail(UBG_E_INVALID, "null flag");
   if (W < 1 || H < 1 || (long)W * H >= (1l << 31) / 2)
     return fail(UBG_E_INVALID, "grid size out of range");
+  std::vector<uint8_t> f8;
+  UBG_TRY(validate_flag(flag, (size_t)W * H, f8));
   int nd = 0;
   if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0)
     return fail(UBG_E_NO_DEVICE, "no HIP device visible");
-  std::vector<uint8_t> f8;
-  UBG_TRY(validate_flag(flag, (size_t)W * H, f8));
   UBG_HIP(hipStreamCreateWithFlags(&sc.s, hipStreamNonBlocking));
   UBG_HIP(hipMalloc((void **)&sc.fl, f8.size()));
   UBG_HIP(hipMalloc((void **)&sc.lab, f8.size() * 4));
@@ -283,6 +283,8 @@ UBG_API int ubg_flag_components(const float *flag, int width, int height, int32_
 UBG_API int ubg_remove_component_means(const float *flag, int width, int height, float *f,
                                        int *ncomp) {
   if (!f) return fail(UBG_E_INVALID, "null rhs");
+  if (width < 1 || height < 1 || (long)width * height >= (1l << 31) / 2)
+    return fail(UBG_E_INVALID, "grid size out of range");
   const size_t n = (size_t)width * height;
   for (size_t i = 0; i < n; i++)
     if (!std::isfinite(f[i])) return fail(UBG_E_NONFINITE, "rhs has a non-finite value");
