@@ -238,7 +238,7 @@ struct ubg_mg {
       float hc = h * (L.w - 1.0f) / (C.w - 1.0f); // pressure_solver.cpp:229
       if (!z && tile_pass_ok(prm.nu_pre) && tile_pass_ok(prm.nu_post)) {
         // small level: two LDS-tiled launches instead of 2 + nu_pre + nu_post
-        launch_tile_pre(L, C, prm.nu_pre, h, prm.alpha, stream);
+        launch_tile_pre(L, C, prm.nu_pre, h, prm.alpha, l > 0, stream);
         solve_level(l + 1, hc, false);
         launch_tile_post(L, C, prm.nu_post, h, prm.alpha, stream);
         return;

@@ -15,7 +15,7 @@ __device__ __forceinline__ void tile_origin(int ntx, int &X0, int &Y0, int otx, 
   Y0 = (b / ntx) * oty;
 }
 
-template <int NU, int TPB_M>
+template <int NU, int TPB_M, bool ZP>
 __global__ void __launch_bounds__(TPB_M)
     k_tile_pre(const float *__restrict__ p, float *__restrict__ p_out, const float *__restrict__ f,
                const uint8_t *__restrict__ m, int w, int h, float hh, float alpha, float ihsq,
@@ -24,8 +24,8 @@ __global__ void __launch_bounds__(TPB_M)
   extern __shared__ __align__(16) char smem[];
   int X0, Y0;
   tile_origin(ntx, X0, Y0, otx, oty);
-  tile_pre_body<NU, TPB_M>(smem, X0, Y0, otx, oty, p, p_out, f, m, w, h, hh, alpha, ihsq, fc, ec,
-                           wc, hc);
+  tile_pre_body<NU, TPB_M, ZP>(smem, X0, Y0, otx, oty, p, p_out, f, m, w, h, hh, alpha, ihsq, fc,
+                               ec, wc, hc);
 }
 
 template <int NU, int TPB_M>
@@ -67,12 +67,13 @@ TileGrid tile_grid(int w, int h, int nu, int target) {
 }
 
 template <int NU, int TPB>
-void pre_launch(Level &L, Level &C, float h, float alpha, hipStream_t s) {
+void pre_launch(Level &L, Level &C, float h, float alpha, bool zero_p, hipStream_t s) {
   const TileGrid t = tile_grid(L.w, L.h, NU, tile_target(TPB));
   const size_t shm = tile_pre_lds(t.oty, NU);
   const float ihsq = 1.0f / h / h;
-  hipLaunchKernelGGL((k_tile_pre<NU, TPB>), dim3(t.ntx * t.nty), dim3(TPB), shm, s, L.p, L.p_alt,
-                     L.f, L.mask, L.w, L.h, h, alpha, ihsq, C.f, C.p, C.w, C.h, t.otx, t.oty, t.ntx);
+  hipLaunchKernelGGL((zero_p ? k_tile_pre<NU, TPB, true> : k_tile_pre<NU, TPB, false>),
+                     dim3(t.ntx * t.nty), dim3(TPB), shm, s, L.p, L.p_alt, L.f, L.mask, L.w, L.h, h,
+                     alpha, ihsq, C.f, C.p, C.w, C.h, t.otx, t.oty, t.ntx);
 }
 
 template <int NU, int TPB>
@@ -85,9 +86,9 @@ void post_launch(Level &L, const Level &C, float h, float alpha, hipStream_t s) 
 }
 
 template <int NU>
-void pre_nu(Level &L, Level &C, float h, float alpha, hipStream_t s) {
-  if ((size_t)L.w * L.h >= kTileBigCells) pre_launch<NU, 512>(L, C, h, alpha, s);
-  else pre_launch<NU, 1024>(L, C, h, alpha, s);
+void pre_nu(Level &L, Level &C, float h, float alpha, bool zero_p, hipStream_t s) {
+  if ((size_t)L.w * L.h >= kTileBigCells) pre_launch<NU, 512>(L, C, h, alpha, zero_p, s);
+  else pre_launch<NU, 1024>(L, C, h, alpha, zero_p, s);
 }
 
 template <int NU>
@@ -106,12 +107,13 @@ void swap_p(Level &L) {
 
 bool tile_pass_ok(int nu) { return nu >= 0 && nu <= NU_TILE; }
 
-void launch_tile_pre(Level &L, Level &C, int nu, float h, float alpha, hipStream_t s) {
+void launch_tile_pre(Level &L, Level &C, int nu, float h, float alpha, bool zero_p,
+                     hipStream_t s) {
   switch (nu) {
-  case 0: pre_nu<0>(L, C, h, alpha, s); break;
-  case 1: pre_nu<1>(L, C, h, alpha, s); break;
-  case 2: pre_nu<2>(L, C, h, alpha, s); break;
-  default: pre_nu<3>(L, C, h, alpha, s); break;
+  case 0: pre_nu<0>(L, C, h, alpha, zero_p, s); break;
+  case 1: pre_nu<1>(L, C, h, alpha, zero_p, s); break;
+  case 2: pre_nu<2>(L, C, h, alpha, zero_p, s); break;
+  default: pre_nu<3>(L, C, h, alpha, zero_p, s); break;
   }
   swap_p(L);
 }

@@ -27,11 +27,12 @@ constexpr int KL = 8;      // global loads in flight per thread while staging
 // cells at distance >= j-1 from a halo edge of the region hold the
 // sweep-by-sweep values, so phase j updates those at distance >= j (an edge
 // on the domain boundary is no halo edge: nothing outside it is missing).
-// F holds (f * h) * h.
-template <int WAVES>
+// F holds (f * h) * h, or with RAWF f itself, scaled here by the same two
+// products (hh is h).
+template <int WAVES, bool RAWF = false>
 __device__ __forceinline__ void tile_phase(float *P, const float *F, const uint8_t *M, int LW,
                                            int LH, int gx0, int gy0, int w, int h, int want,
-                                           int j, float alpha) {
+                                           int j, float alpha, float hh = 0.0f) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); // row loop in SALU
   const int xlo = gx0 == 0 ? 1 : j, ylo = gy0 == 0 ? 1 : j;
@@ -45,7 +46,8 @@ __device__ __forceinline__ void tile_phase(float *P, const float *F, const uint8
   if (lx <= xhi) {
     for (int ly = ly0; ly <= yhi; ly += WAVES) {
       const int i = ly * PITCH + lx;
-      P[i] = smooth_fhh(P[i], P[i - 1], P[i + 1], P[i - PITCH], P[i + PITCH], F[i], M[i], alpha);
+      const float fhh = RAWF ? F[i] * hh * hh : F[i];
+      P[i] = smooth_fhh(P[i], P[i - 1], P[i + 1], P[i - PITCH], P[i + PITCH], fhh, M[i], alpha);
     }
   }
   __syncthreads();
@@ -54,7 +56,9 @@ __device__ __forceinline__ void tile_phase(float *P, const float *F, const uint8
 // Stage an LW x LH region of p, (f * hh) * hh and the mask (origin gx0,
 // gy0) into LDS with pitch PITCH.  Each round issues KL loads of each array
 // per thread before the first LDS store, so a round costs one memory latency.
-template <int TPB_M>
+// ZP: p is zero on entry and is not read (the region's p words are set to
+// +0, the value the restriction above stored).  RAWF: F holds f unscaled.
+template <int TPB_M, bool ZP = false, bool RAWF = false>
 __device__ __forceinline__ void stage_fine(float *P, float *F, uint8_t *M, const float *p,
                                            const float *f, const uint8_t *m, int w, int gx0,
                                            int gy0, int LW, int LH, float hh) {
@@ -72,7 +76,7 @@ __device__ __forceinline__ void stage_fine(float *P, float *F, uint8_t *M, const
       li[k] = -1;
       if (ly < LH) {
         const size_t g = (size_t)(gy0 + ly) * w + gx0 + lx;
-        vp[k] = p[g];
+        vp[k] = ZP ? 0.0f : p[g];
         vf[k] = f[g];
         vm[k] = m[g];
         li[k] = ly * PITCH + lx;
@@ -88,7 +92,7 @@ __device__ __forceinline__ void stage_fine(float *P, float *F, uint8_t *M, const
     for (int k = 0; k < KL; k++)
       if (li[k] >= 0) {
         P[li[k]] = vp[k];
-        F[li[k]] = vf[k] * hh * hh;
+        F[li[k]] = RAWF ? vf[k] : vf[k] * hh * hh;
         M[li[k]] = vm[k];
       }
   }
@@ -120,8 +124,10 @@ inline int tile_target(int tpb) { return tpb == 512 ? 1024 : 512; }
 // 213-227): reads p, writes the smoothed level to p_out (every cell of the
 // tile), the coarse rhs fc and a zero coarse solution ec (interior cells).
 // Tile origin (X0, Y0) of an otx x oty partition (otx, oty even); smem
-// holds tile_pre_lds(oty, NU) bytes.
-template <int NU, int TPB_M>
+// holds tile_pre_lds(oty, NU) bytes.  ZP: p is zero on entry (a level below
+// the finest, whose rhs and zero solution the restriction above wrote) and
+// is not read.
+template <int NU, int TPB_M, bool ZP = false>
 __device__ __forceinline__ void
 tile_pre_body(char *smem, int X0, int Y0, int otx, int oty, const float *__restrict__ p,
               float *__restrict__ p_out, const float *__restrict__ f,
@@ -139,25 +145,18 @@ tile_pre_body(char *smem, int X0, int Y0, int otx, int oty, const float *__restr
   const int gx1 = min(X0 + otx + 2 * NU, w - 1), gy1 = min(Y0 + oty + 2 * NU, h - 1);
   const int LW = gx1 - gx0 + 1, LH = gy1 - gy0 + 1;
   constexpr int WAVES = TPB_M / 64;
-  // f where the residual is taken (rows ry0..ry1 <= 33 of them, columns
-  // rx0..rx1 <= 127), loaded now and used after the sweeps
+  // F holds f itself: the sweeps form (f * h) * h where they use it and the
+  // residual reads f from LDS (rows ry0..ry1 <= 33 of them, columns
+  // rx0..rx1 <= 127) instead of loading it a second time
   const int rx0 = max(X0 - 1, 1), rx1 = min(X0 + otx - 1, w - 2);
   const int ry0 = max(Y0 - 1, 1), ry1 = min(Y0 + oty - 1, h - 2);
   constexpr int RR = (33 + WAVES - 1) / WAVES;
-  float fr[RR][2];
-#pragma unroll
-  for (int k = 0; k < RR; k++)
-#pragma unroll
-    for (int c = 0; c < 2; c++) {
-      const int y = ry0 + wave + k * WAVES, x = rx0 + lane + 64 * c;
-      fr[k][c] = (y <= ry1 && x <= rx1) ? f[(size_t)y * w + x] : 0.0f;
-    }
-  stage_fine<TPB_M>(P, F, M, p, f, m, w, gx0, gy0, LW, LH, hh);
+  stage_fine<TPB_M, ZP, true>(P, F, M, p, f, m, w, gx0, gy0, LW, LH, hh);
   __syncthreads();
 #pragma unroll
   for (int s = 0; s < NU; s++) {
-    tile_phase<WAVES>(P, F, M, LW, LH, gx0, gy0, w, h, 1, 2 * s + 1, alpha);
-    tile_phase<WAVES>(P, F, M, LW, LH, gx0, gy0, w, h, 0, 2 * s + 2, alpha);
+    tile_phase<WAVES, true>(P, F, M, LW, LH, gx0, gy0, w, h, 1, 2 * s + 1, alpha, hh);
+    tile_phase<WAVES, true>(P, F, M, LW, LH, gx0, gy0, w, h, 0, 2 * s + 2, alpha, hh);
   }
   // the tile's own cells (boundary cells unchanged) ...
   const int ox1 = min(X0 + otx, w), oy1 = min(Y0 + oty, h);
@@ -172,7 +171,7 @@ tile_pre_body(char *smem, int X0, int Y0, int otx, int oty, const float *__restr
       const int y = ry0 + wave + k * WAVES, x = rx0 + lane + 64 * c;
       if (y <= ry1 && x <= rx1) {
         const int i = (y - gy0) * PITCH + (x - gx0);
-        F[i] = residual_of(P[i], P[i - 1], P[i + 1], P[i - PITCH], P[i + PITCH], fr[k][c], M[i],
+        F[i] = residual_of(P[i], P[i - 1], P[i + 1], P[i - PITCH], P[i + PITCH], F[i], M[i],
                            ihsq);
       }
     }

@@ -96,8 +96,11 @@ void launch_prolong_postsmooth(Level &L, const Level &C, int ns, float h, float 
 // LDS-tiled level passes (mg_tile.hip) for levels below the streaming size:
 // nu sweeps + residual + restriction, and prolongation + correction + nu
 // sweeps, one launch each; both swap L.p / L.p_alt.  No zero-gradient BC.
+// zero_p: L.p is zero on entry (every level below the finest: the
+// restriction above zeroed it), so the pre pass does not read it.
 bool tile_pass_ok(int nu);
-void launch_tile_pre(Level &L, Level &C, int nu, float h, float alpha, hipStream_t s);
+void launch_tile_pre(Level &L, Level &C, int nu, float h, float alpha, bool zero_p,
+                     hipStream_t s);
 void launch_tile_post(Level &L, const Level &C, int nu, float h, float alpha, hipStream_t s);
 void launch_residual(const Level &L, float h, hipStream_t s);
 void launch_residual_restrict(const Level &F, const Level &C, float h, hipStream_t s);
