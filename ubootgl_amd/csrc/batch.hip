@@ -268,8 +268,9 @@ __global__ void __launch_bounds__(TPB) kb_pc(StageArgs a) {
 // domain at one level (each domain's own even column split, oty rows, walked
 // as the other stages): pre reads p[ia], writes p[ia^1], the coarse rhs and
 // a zero coarse p[ib]; post reads p[ia] and the coarse solution p[ib],
-// writes p[ia^1].
-template <int NU, int TPB_T>
+// writes p[ia^1].  ZP (every level below the finest): p[ia] is the zero
+// solution the restriction above wrote, and pre does not read it.
+template <int NU, int TPB_T, bool ZP>
 __global__ void __launch_bounds__(TPB_T) kb_tile_pre(StageArgs a) {
   extern __shared__ __align__(16) char smem[];
   TileWalk tw;
@@ -281,9 +282,9 @@ __global__ void __launch_bounds__(TPB_T) kb_tile_pre(StageArgs a) {
     const BLev &C = a.lev[tw.d * BMAXL + a.level + 1];
     const int s = a.ia[tw.d];
     __syncthreads(); // the previous tile's LDS readers are done
-    tile_pre_body<NU, TPB_T>(smem, tw.tx() * tw.otx, tw.ty() * a.oty, tw.otx, a.oty, F.p[s],
-                             F.p[s ^ 1], F.f, F.mask, F.w, F.h, F.hh, a.alpha, F.ihsq, C.f,
-                             C.p[a.ib[tw.d]], C.w, C.h);
+    tile_pre_body<NU, TPB_T, ZP>(smem, tw.tx() * tw.otx, tw.ty() * a.oty, tw.otx, a.oty,
+                                 F.p[s], F.p[s ^ 1], F.f, F.mask, F.w, F.h, F.hh, a.alpha,
+                                 F.ihsq, C.f, C.p[a.ib[tw.d]], C.w, C.h);
   } while (tw.next(a));
 }
 template <int NU, int TPB_T>
@@ -609,7 +610,10 @@ void launch_tile_pass(bool pre, int nu, int tpb, int blocks, hipStream_t s, cons
   const size_t shm = pre ? tile_pre_lds(a.oty, nu) : tile_post_lds(a.oty, nu);
 #define UBG_TP(N, T)                                                                            \
   if (nu == N && tpb == T) {                                                                    \
-    if (pre) hipLaunchKernelGGL((kb_tile_pre<N, T>), dim3(blocks), dim3(T), shm, s, a);        \
+    if (pre && a.level > 0)                                                                     \
+      hipLaunchKernelGGL((kb_tile_pre<N, T, true>), dim3(blocks), dim3(T), shm, s, a);         \
+    else if (pre)                                                                               \
+      hipLaunchKernelGGL((kb_tile_pre<N, T, false>), dim3(blocks), dim3(T), shm, s, a);        \
     else hipLaunchKernelGGL((kb_tile_post<N, T>), dim3(blocks), dim3(T), shm, s, a);           \
     return;                                                                                     \
   }

@@ -221,6 +221,18 @@ struct ubg_mg {
   // MG::solveLevel (pressure_solver.cpp:201-248).  A level costs two fused
   // passes: (pre-smoothing, residual, restriction) and (prolongation,
   // correction, post-smoothing); setZeroGradientBC happens inside them.
+  // Whether solve_level(l) (l > 0: zero initial guess, no ZG) reads its p
+  // before writing it.  The LDS coarse V-cycle and the tile pre pass do not,
+  // so the restriction above may leave that p as it is instead of zeroing it
+  // (the branches below decide alike).
+  bool reads_zero_p(int l) {
+    const size_t n = (size_t)lv[l].w * lv[l].h;
+    if (l == levels - 2) return true; // sweeps on the coarsest solved level
+    if (n <= kCoarseMaxCells && coarse_fits_lds(lv.data(), l, levels)) return false;
+    if (n < kFuseMinCells && tile_pass_ok(prm.nu_pre) && tile_pass_ok(prm.nu_post)) return false;
+    return true;
+  }
+
   void solve_level(int l, float h, bool zg) {
     const bool z = zg && l == 0;
     if (l == levels - 2) {
@@ -238,7 +250,7 @@ struct ubg_mg {
       float hc = h * (L.w - 1.0f) / (C.w - 1.0f); // pressure_solver.cpp:229
       if (!z && tile_pass_ok(prm.nu_pre) && tile_pass_ok(prm.nu_post)) {
         // small level: two LDS-tiled launches instead of 2 + nu_pre + nu_post
-        launch_tile_pre(L, C, prm.nu_pre, h, prm.alpha, l > 0, stream);
+        launch_tile_pre(L, C, prm.nu_pre, h, prm.alpha, l > 0, reads_zero_p(l + 1), stream);
         solve_level(l + 1, hc, false);
         launch_tile_post(L, C, prm.nu_post, h, prm.alpha, stream);
         return;
@@ -255,7 +267,9 @@ struct ubg_mg {
     // the grid-transfer passes
     const int pre_fused = std::min(prm.nu_pre, 1);
     sweeps(l, prm.nu_pre - pre_fused, h, z);
-    timed(UBG_TIMER_L0_PRE, l, [&] { launch_presmooth_restrict(L, C, pre_fused, h, prm.alpha, z, stream); });
+    timed(UBG_TIMER_L0_PRE, l, [&] {
+      launch_presmooth_restrict(L, C, pre_fused, h, prm.alpha, z, reads_zero_p(l + 1), stream);
+    });
     float hc = h * (L.w - 1.0f) / (C.w - 1.0f); // pressure_solver.cpp:229
     solve_level(l + 1, hc, false);
     const int post_fused = std::min(prm.nu_post, 1);

@@ -65,7 +65,7 @@ void launch_mode(int ns, const StreamArgs &a, bool zg, bool al, int nb, hipStrea
 }
 
 void launch_stream(int mode, int ns, Level &L, const Level &C, float h, float alpha, bool zg,
-                   hipStream_t s) {
+                   bool zero_coarse, hipStream_t s) {
   StreamArgs a{};
   a.src = L.p;
   a.dst = L.p_alt;
@@ -77,7 +77,7 @@ void launch_stream(int mode, int ns, Level &L, const Level &C, float h, float al
   a.alpha = alpha;
   a.ihsq = 1.0f / h / h;
   a.crc = C.f;
-  a.cec = C.p;
+  a.cec = zero_coarse ? C.p : nullptr;
   a.cpin = C.p;
   a.cmask = C.mask;
   a.wc = C.w;
@@ -108,13 +108,13 @@ void launch_stream(int mode, int ns, Level &L, const Level &C, float h, float al
 } // namespace
 
 void launch_presmooth_restrict(Level &L, Level &C, int ns, float h, float alpha, bool zg,
-                               hipStream_t s) {
-  launch_stream(MODE_RR, ns, L, C, h, alpha, zg, s);
+                               bool zero_coarse, hipStream_t s) {
+  launch_stream(MODE_RR, ns, L, C, h, alpha, zg, zero_coarse, s);
 }
 
 void launch_prolong_postsmooth(Level &L, const Level &C, int ns, float h, float alpha, bool zg,
                                hipStream_t s) {
-  launch_stream(MODE_PC, ns, L, C, h, alpha, zg, s);
+  launch_stream(MODE_PC, ns, L, C, h, alpha, zg, false, s);
 }
 
 } // namespace ubg

@@ -67,13 +67,15 @@ TileGrid tile_grid(int w, int h, int nu, int target) {
 }
 
 template <int NU, int TPB>
-void pre_launch(Level &L, Level &C, float h, float alpha, bool zero_p, hipStream_t s) {
+void pre_launch(Level &L, Level &C, float h, float alpha, bool zero_p, bool zero_coarse,
+                hipStream_t s) {
   const TileGrid t = tile_grid(L.w, L.h, NU, tile_target(TPB));
   const size_t shm = tile_pre_lds(t.oty, NU);
   const float ihsq = 1.0f / h / h;
   hipLaunchKernelGGL((zero_p ? k_tile_pre<NU, TPB, true> : k_tile_pre<NU, TPB, false>),
                      dim3(t.ntx * t.nty), dim3(TPB), shm, s, L.p, L.p_alt, L.f, L.mask, L.w, L.h, h,
-                     alpha, ihsq, C.f, C.p, C.w, C.h, t.otx, t.oty, t.ntx);
+                     alpha, ihsq, C.f, zero_coarse ? C.p : nullptr, C.w, C.h, t.otx, t.oty,
+                     t.ntx);
 }
 
 template <int NU, int TPB>
@@ -86,9 +88,11 @@ void post_launch(Level &L, const Level &C, float h, float alpha, hipStream_t s) 
 }
 
 template <int NU>
-void pre_nu(Level &L, Level &C, float h, float alpha, bool zero_p, hipStream_t s) {
-  if ((size_t)L.w * L.h >= kTileBigCells) pre_launch<NU, 512>(L, C, h, alpha, zero_p, s);
-  else pre_launch<NU, 1024>(L, C, h, alpha, zero_p, s);
+void pre_nu(Level &L, Level &C, float h, float alpha, bool zero_p, bool zero_coarse,
+            hipStream_t s) {
+  if ((size_t)L.w * L.h >= kTileBigCells)
+    pre_launch<NU, 512>(L, C, h, alpha, zero_p, zero_coarse, s);
+  else pre_launch<NU, 1024>(L, C, h, alpha, zero_p, zero_coarse, s);
 }
 
 template <int NU>
@@ -108,12 +112,12 @@ void swap_p(Level &L) {
 bool tile_pass_ok(int nu) { return nu >= 0 && nu <= NU_TILE; }
 
 void launch_tile_pre(Level &L, Level &C, int nu, float h, float alpha, bool zero_p,
-                     hipStream_t s) {
+                     bool zero_coarse, hipStream_t s) {
   switch (nu) {
-  case 0: pre_nu<0>(L, C, h, alpha, zero_p, s); break;
-  case 1: pre_nu<1>(L, C, h, alpha, zero_p, s); break;
-  case 2: pre_nu<2>(L, C, h, alpha, zero_p, s); break;
-  default: pre_nu<3>(L, C, h, alpha, zero_p, s); break;
+  case 0: pre_nu<0>(L, C, h, alpha, zero_p, zero_coarse, s); break;
+  case 1: pre_nu<1>(L, C, h, alpha, zero_p, zero_coarse, s); break;
+  case 2: pre_nu<2>(L, C, h, alpha, zero_p, zero_coarse, s); break;
+  default: pre_nu<3>(L, C, h, alpha, zero_p, zero_coarse, s); break;
   }
   swap_p(L);
 }

@@ -49,7 +49,8 @@ struct StreamArgs {
   int w, h;
   float hh, alpha, ihsq;
   float *crc;            // MODE_RR: coarse rhs (rcs[l+1])
-  float *cec;            // MODE_RR: coarse solution, zeroed (ecs[l+1])
+  float *cec;            // MODE_RR: coarse solution, zeroed (ecs[l+1]); null: left
+                         // as it is (the coarse level's next pass does not read it)
   const float *cpin;     // MODE_PC: coarse solution (ecs[l+1])
   const uint8_t *cmask;  // MODE_PC: coarse flag mask
   int wc, hc;
@@ -288,7 +289,7 @@ __device__ __forceinline__ void mg_stream_unit(const StreamArgs &A, int unit) {
               2 * Y <= FA + A.R - 1) {
             size_t ci = (size_t)Y * A.wc + X;
             A.crc[ci] = v * (1.0f / 16.0f);
-            A.cec[ci] = 0.0f;
+            if (A.cec) A.cec[ci] = 0.0f;
           }
         }
         res2 = res1;

@@ -122,7 +122,8 @@ inline int tile_target(int tpb) { return tpb == 512 ? 1024 : 512; }
 
 // Pre-smoothing + calculateResidualField + restrict (pressure_solver.cpp:
 // 213-227): reads p, writes the smoothed level to p_out (every cell of the
-// tile), the coarse rhs fc and a zero coarse solution ec (interior cells).
+// tile), the coarse rhs fc and a zero coarse solution ec (interior cells;
+// ec null: left as it is, for a coarse pass that does not read it).
 // Tile origin (X0, Y0) of an otx x oty partition (otx, oty even); smem
 // holds tile_pre_lds(oty, NU) bytes.  ZP: p is zero on entry (a level below
 // the finest, whose rhs and zero solution the restriction above wrote) and
@@ -190,7 +191,7 @@ tile_pre_body(char *smem, int X0, int Y0, int otx, int oty, const float *__restr
       v += R(-1, 1) * 1.0f + R(0, 1) * 2.0f + R(1, 1) * 1.0f;
       const size_t ci = (size_t)yc * wc + xc;
       fc[ci] = v * (1.0f / 16.0f);
-      ec[ci] = 0.0f;
+      if (ec) ec[ci] = 0.0f;
     }
 }
 

@@ -87,9 +87,10 @@ void launch_rbgs_fused(Level &L, float h, float alpha, bool zero_gradient, hipSt
 void launch_zero_gradient(float *p, int w, int h, hipStream_t s);
 // Row-pipelined level passes (ping-pong: read L.p, write L.p_alt, swap):
 //   ns (0 or 1) sweeps, then residual + restriction into C
-//   (C.f = restricted residual, C.p = 0);
+//   (C.f = restricted residual, C.p = 0 if zero_coarse, else C.p is left as
+//   it is for a coarse pass that does not read it);
 void launch_presmooth_restrict(Level &L, Level &C, int ns, float h, float alpha, bool zg,
-                               hipStream_t s);
+                               bool zero_coarse, hipStream_t s);
 //   prolongation of C.p + correction (+ZG), then ns (0 or 1) sweeps.
 void launch_prolong_postsmooth(Level &L, const Level &C, int ns, float h, float alpha, bool zg,
                                hipStream_t s);
@@ -97,10 +98,12 @@ void launch_prolong_postsmooth(Level &L, const Level &C, int ns, float h, float 
 // nu sweeps + residual + restriction, and prolongation + correction + nu
 // sweeps, one launch each; both swap L.p / L.p_alt.  No zero-gradient BC.
 // zero_p: L.p is zero on entry (every level below the finest: the
-// restriction above zeroed it), so the pre pass does not read it.
+// restriction above zeroed it), so the pre pass does not read it;
+// zero_coarse: the pre pass zeroes C.p (else C.p is left as it is, for a
+// coarse pass that does not read it).
 bool tile_pass_ok(int nu);
 void launch_tile_pre(Level &L, Level &C, int nu, float h, float alpha, bool zero_p,
-                     hipStream_t s);
+                     bool zero_coarse, hipStream_t s);
 void launch_tile_post(Level &L, const Level &C, int nu, float h, float alpha, hipStream_t s);
 void launch_residual(const Level &L, float h, hipStream_t s);
 void launch_residual_restrict(const Level &F, const Level &C, float h, hipStream_t s);
