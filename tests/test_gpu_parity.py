@@ -404,9 +404,10 @@ def test_advect_flat_loads_bit_exact(gpu, monkeypatch):
     compare(sim, ref)
 
 
-@pytest.mark.parametrize("rows", list(range(8, 33, 2)))
-def test_tile_pass_row_heights(gpu, monkeypatch, rows):
-    """Every tile row height the LDS level passes can pick (even, 8..32),
+@pytest.mark.parametrize("rows,nu", [(r, 3) for r in range(8, 37, 2)] + [(38, 2), (40, 2)])
+def test_tile_pass_row_heights(gpu, monkeypatch, rows, nu):
+    """Every tile row height the LDS level passes can pick (even, 8..40; the
+    64 KiB of LDS a tile may use allow 36 rows at nu = 3 and 40 at nu = 2),
     forced on an odd-sized hierarchy whose levels all run as tiles: three
     V-cycles bit-identical to the oracle's."""
     monkeypatch.setenv("UBG_TILE_ROWS", str(rows))
@@ -416,7 +417,8 @@ def test_tile_pass_row_heights(gpu, monkeypatch, rows):
     f = g.standard_normal((Hd, Wd)).astype(np.float32)
     h = float(np.float32(1.0 / Wd))
     mg = U.MG(flag)
-    om = O.OracleMG(Wd, Hd)
+    mg.set_params(U.Params(nu_pre=nu, nu_post=nu, nu_coarse=5))
+    om = O.OracleMG(Wd, Hd, nu_pre=nu, nu_post=nu)
     om.update_fields(flag)
     pg = np.zeros((Hd, Wd), np.float32)
     pr = pg.copy()

@@ -1138,8 +1138,7 @@ int BatchEngine::build_tiles() {
   // fixed by the tuning, or per level the fewest (even, 8 to 32) whose tiles
   // over all member domains fit in one residency wave of the launch's
   // target, with 512-thread workgroups for levels of at least kTileBigCells
-  // cells in total and 1024-thread ones below -- the single-domain rule over
-  // the whole level.
+  // cells in total and 1024-thread ones below.
   auto fused = [&](Span *out, int nu) {
     for (int l = 0; l < BMAXL; l++) {
       size_t cells = 0;

@@ -46,20 +46,21 @@ struct TileGrid {
 };
 
 // Tile shape for a level: columns as wide as the LDS pitch allows, split
-// evenly; rows: the fewest (even, 8 to 32) whose tile count still fits in
-// one residency wave of `target` workgroups (a second, partial wave costs
-// a whole tile lifetime: 2048x1024 with 20 rows, 936 tiles, beats 16 rows,
-// 1152 tiles, by ~6% of the level's time).
-// UBG_TILE_ROWS (an even number 8..32) overrides the rule: a test switch, so
-// every row height the rule can pick is checked on one level.
-TileGrid tile_grid(int w, int h, int nu, int target) {
+// evenly; rows: the fewest (even, 8 to kTileRowsMax) whose tile count still
+// fits in one residency wave of kTileTarget workgroups, within 64 KiB of LDS.
+// UBG_TILE_ROWS (an even number 8..kTileRowsMax whose LDS fits) overrides
+// the rule: a test switch, so every row height the rule can pick is checked
+// on one level.
+TileGrid tile_grid(int w, int h, int nu) {
   TileGrid t;
   tile_cols(w, nu, t.otx, t.ntx);
   int oty = 8;
-  while (oty < 32 && (long)t.ntx * ((h + oty - 1) / oty) > target) oty += 2;
+  while (oty < kTileRowsMax && (long)t.ntx * ((h + oty - 1) / oty) > kTileTarget &&
+         tile_lds_ok(oty + 2, nu))
+    oty += 2;
   if (const char *e = std::getenv("UBG_TILE_ROWS")) {
     const int r = std::atoi(e);
-    if (r >= 8 && r <= 32 && r % 2 == 0) oty = r;
+    if (r >= 8 && r <= kTileRowsMax && r % 2 == 0 && tile_lds_ok(r, nu)) oty = r;
   }
   t.oty = oty;
   t.nty = (h + oty - 1) / oty;
@@ -69,7 +70,7 @@ TileGrid tile_grid(int w, int h, int nu, int target) {
 template <int NU, int TPB>
 void pre_launch(Level &L, Level &C, float h, float alpha, bool zero_p, bool zero_coarse,
                 hipStream_t s) {
-  const TileGrid t = tile_grid(L.w, L.h, NU, tile_target(TPB));
+  const TileGrid t = tile_grid(L.w, L.h, NU);
   const size_t shm = tile_pre_lds(t.oty, NU);
   const float ihsq = 1.0f / h / h;
   hipLaunchKernelGGL((zero_p ? k_tile_pre<NU, TPB, true> : k_tile_pre<NU, TPB, false>),
@@ -80,7 +81,7 @@ void pre_launch(Level &L, Level &C, float h, float alpha, bool zero_p, bool zero
 
 template <int NU, int TPB>
 void post_launch(Level &L, const Level &C, float h, float alpha, hipStream_t s) {
-  const TileGrid t = tile_grid(L.w, L.h, NU, tile_target(TPB));
+  const TileGrid t = tile_grid(L.w, L.h, NU);
   const size_t shm = tile_post_lds(t.oty, NU);
   hipLaunchKernelGGL((k_tile_post<NU, TPB>), dim3(t.ntx * t.nty), dim3(TPB), shm, s, L.p,
                      L.p_alt, L.f, L.mask, L.w, L.h, h, alpha, C.p, C.mask, C.w, C.h, t.otx,
@@ -90,15 +91,12 @@ void post_launch(Level &L, const Level &C, float h, float alpha, hipStream_t s) 
 template <int NU>
 void pre_nu(Level &L, Level &C, float h, float alpha, bool zero_p, bool zero_coarse,
             hipStream_t s) {
-  if ((size_t)L.w * L.h >= kTileBigCells)
-    pre_launch<NU, 512>(L, C, h, alpha, zero_p, zero_coarse, s);
-  else pre_launch<NU, 1024>(L, C, h, alpha, zero_p, zero_coarse, s);
+  pre_launch<NU, TILE_TPB>(L, C, h, alpha, zero_p, zero_coarse, s);
 }
 
 template <int NU>
 void post_nu(Level &L, const Level &C, float h, float alpha, hipStream_t s) {
-  if ((size_t)L.w * L.h >= kTileBigCells) post_launch<NU, 512>(L, C, h, alpha, s);
-  else post_launch<NU, 1024>(L, C, h, alpha, s);
+  post_launch<NU, TILE_TPB>(L, C, h, alpha, s);
 }
 
 void swap_p(Level &L) {

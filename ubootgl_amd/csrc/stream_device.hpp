@@ -87,7 +87,12 @@ __device__ __forceinline__ void mg_stream_unit(const StreamArgs &A, int unit) {
   const int FA = A.R * r, FB = min(FA + A.R - 1, h - 1); // owned fine rows
   const int LNS = FA - (MODE == MODE_RR ? 2 : 0);
   const int UNS = FB + (MODE == MODE_RR ? 1 : 0);
-  const int L0 = LNS - 2 * NS;     // first input row
+  // first input row: a sweep's final row j depends on rows j-2..j+2 of its
+  // input.  Under ZG, row h-1 is a copy of row h-2 (of the last sweep's
+  // output, or of the corrected input), so a block that owns only row h-1
+  // (odd h) needs row h-2 as well: two more rows (L0 stays even)
+  const int L0 =
+      LNS - 2 * NS - ((ZG && (NS > 0 || MODE == MODE_PC) && FA == h - 1) ? 2 : 0);
   const int TEND = UNS + 3 * NS + 1; // last iteration
 
   // clamped (always valid) load addresses; clamped values only ever feed

@@ -114,10 +114,26 @@ inline void tile_cols(int w, int nu, int &otx, int &ntx) {
   otx = ((w + ntx - 1) / ntx + 1) & ~1;
   ntx = (w + otx - 1) / otx;
 }
-// levels with at least this many cells run 512-thread tile workgroups (more
-// of them fit a CU), smaller ones 1024-thread workgroups (more waves per tile)
+// Tile workgroups have 1024 threads: 16 waves share a tile's colour phases,
+// so a phase walks few rows per wave.  Two fit a CU, so a launch aims for
+// kTileTarget tiles, one residency wave (a second, partial wave costs a
+// whole tile lifetime).  Rows per tile: the fewest (even, 8 to kTileRowsMax)
+// whose tiles fit that target, within 64 KiB of LDS.  Measured on gfx950,
+// 2048x1024 with 38 rows per 1024-thread tile (486 tiles) against 20 rows
+// per 512-thread tile (936 tiles): pre 16.0 instead of 17.8 us, post 16.0
+// instead of 17.2 us (fewer halo rows, fewer rows per wave per phase).
+constexpr int TILE_TPB = 1024;
+constexpr int kTileTarget = 512;
+constexpr int kTileRowsMax = 48;
+inline bool tile_lds_ok(int oty, int nu) {
+  return tile_pre_lds(oty, nu) <= 65536 && tile_post_lds(oty, nu) <= 65536;
+}
+// The batched solver's tiles (many domains' tiles walked by each workgroup)
+// keep 512-thread workgroups for levels of at least kTileBigCells cells in
+// all and 1024-thread ones below, of 8 to 32 rows, aiming for
+// tile_target(tpb) workgroups: measured on gfx950, the single-domain rule
+// above made the 256-domain batch 12% slower and the served mix 9%.
 constexpr size_t kTileBigCells = size_t(1) << 18;
-// workgroups a tile launch aims for: about two per CU
 inline int tile_target(int tpb) { return tpb == 512 ? 1024 : 512; }
 
 // Pre-smoothing + calculateResidualField + restrict (pressure_solver.cpp:
@@ -147,11 +163,10 @@ tile_pre_body(char *smem, int X0, int Y0, int otx, int oty, const float *__restr
   const int LW = gx1 - gx0 + 1, LH = gy1 - gy0 + 1;
   constexpr int WAVES = TPB_M / 64;
   // F holds f itself: the sweeps form (f * h) * h where they use it and the
-  // residual reads f from LDS (rows ry0..ry1 <= 33 of them, columns
-  // rx0..rx1 <= 127) instead of loading it a second time
+  // residual reads f from LDS (rows ry0..ry1, columns rx0..rx1 <= 127)
+  // instead of loading it a second time
   const int rx0 = max(X0 - 1, 1), rx1 = min(X0 + otx - 1, w - 2);
   const int ry0 = max(Y0 - 1, 1), ry1 = min(Y0 + oty - 1, h - 2);
-  constexpr int RR = (33 + WAVES - 1) / WAVES;
   stage_fine<TPB_M, ZP, true>(P, F, M, p, f, m, w, gx0, gy0, LW, LH, hh);
   __syncthreads();
 #pragma unroll
@@ -165,12 +180,11 @@ tile_pre_body(char *smem, int X0, int Y0, int otx, int oty, const float *__restr
     for (int x = X0 + lane; x < ox1; x += 64)
       p_out[(size_t)y * w + x] = P[(y - gy0) * PITCH + (x - gx0)];
   // ... and the residual in place of F where the restriction reads it
-#pragma unroll
-  for (int k = 0; k < RR; k++)
+  for (int y = ry0 + wave; y <= ry1; y += WAVES)
 #pragma unroll
     for (int c = 0; c < 2; c++) {
-      const int y = ry0 + wave + k * WAVES, x = rx0 + lane + 64 * c;
-      if (y <= ry1 && x <= rx1) {
+      const int x = rx0 + lane + 64 * c;
+      if (x <= rx1) {
         const int i = (y - gy0) * PITCH + (x - gx0);
         F[i] = residual_of(P[i], P[i - 1], P[i + 1], P[i - PITCH], P[i + PITCH], F[i], M[i],
                            ihsq);
