@@ -218,13 +218,10 @@ struct ubg_mg {
       timed(UBG_TIMER_SMOOTH_L0, l, [&] { launch_rbgs_fused(L, h, prm.alpha, zg, stream); });
   }
 
-  // MG::solveLevel (pressure_solver.cpp:201-248).  A level costs two fused
-  // passes: (pre-smoothing, residual, restriction) and (prolongation,
-  // correction, post-smoothing); setZeroGradientBC happens inside them.
   // Whether solve_level(l) (l > 0: zero initial guess, no ZG) reads its p
   // before writing it.  The LDS coarse V-cycle and the tile pre pass do not,
   // so the restriction above may leave that p as it is instead of zeroing it
-  // (the branches below decide alike).
+  // (the branches of solve_level decide alike).
   bool reads_zero_p(int l) {
     const size_t n = (size_t)lv[l].w * lv[l].h;
     if (l == levels - 2) return true; // sweeps on the coarsest solved level
@@ -233,6 +230,9 @@ struct ubg_mg {
     return true;
   }
 
+  // MG::solveLevel (pressure_solver.cpp:201-248).  A level costs two fused
+  // passes: (pre-smoothing, residual, restriction) and (prolongation,
+  // correction, post-smoothing); setZeroGradientBC happens inside them.
   void solve_level(int l, float h, bool zg) {
     const bool z = zg && l == 0;
     if (l == levels - 2) {

@@ -88,8 +88,9 @@ void launch_stream(int mode, int ns, Level &L, const Level &C, float h, float al
   // to [16, 64], even.  Each row block runs up to 5NS+4 extra pipeline rows
   // (halo rows, re-read from L2), so shorter blocks cost VALU work; longer
   // ones leave too few waves in flight to cover the serial row loop's load
-  // latency.  Measured on gfx950 at 4096x2048: R=16 beats R=18 (~1.5%), R=20
-  // (~3%), R=2 (2.2x slower) and R=32 (1.5x slower).
+  // latency.  Measured on gfx950 at 4096x2048: R=16 beats R=12 and R=14
+  // (~0.5% of the step), R=18 (~1.5%), R=20 (~3%), R=2 (2.2x slower) and
+  // R=32 (1.5x slower).
   int R = (int)((long)a.nstrips * L.h / 4608);
   R = R < 16 ? 16 : (R > 64 ? 64 : R);
   R &= ~1;
