@@ -299,14 +299,19 @@ UBG_API int ubg_batch_get_pressure(ubg_batch *b, int d, float *p);
  * as MG::solve's argument */
 UBG_API int ubg_batch_vcycles(ubg_batch *b, int ncycles, int zero_gradient_bc);
 /* A batch whose domain i is simulation i's pressure solve (size, flag, h;
- * the simulations must share their multigrid parameters) ... */
+ * the simulations must share their multigrid parameters; each appears once)
+ * ... */
 UBG_API int ubg_batch_create_for_sims(ubg_sim *const *sims, int n, ubg_batch **out);
 /* ... and one Simulation::step (simulation.cpp:356-374) of each of the n
- * simulations with timestep dts[i], their pressure solves batched: each
- * simulation runs up to its solve on its own stream, the V-cycles of all run
- * on the batch's stream, then each finishes its step.  Equal, bit for bit, to
- * ubg_sim_step on each simulation.  A simulation whose flags changed since the
- * last call (or another simulation at slot i) gets its domain rebuilt. */
+ * simulations with timestep dts[i], every stage batched: each per-cell stage
+ * (two diffuse iterations, advect, setVBCs + divergence, the correction tail)
+ * is one launch over all n on the batch's stream, and so is each stage of
+ * their V-cycles.  A simulation's own pending work (accumulated velocity,
+ * boundary refresh) runs first on its own stream, and its next own work
+ * waits for the batched step.  Equal, bit for bit, to ubg_sim_step on each
+ * simulation.  A simulation whose flags changed since the last call (or
+ * another simulation at slot i) gets its domain rebuilt; a simulation listed
+ * twice is UBG_E_INVALID. */
 UBG_API int ubg_sims_step(ubg_batch *b, ubg_sim *const *sims, int n, const float *dts);
 
 /* ---- Served solves ----------------------------------------------------------

@@ -8,6 +8,8 @@ Parity contract: bit-identical to the single-domain solve (MG.solve) and to
 Simulation.step on each simulation alone, which the parity tests hold
 bit-identical to the oracle.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -148,6 +150,20 @@ def test_projection_batch_errors(gpu):
     with pytest.raises(L.UbgError) as e:
         B.SimBatch([s1, s3])
     assert e.value.code == L.E_DOMAIN_TOO_SMALL
+    # a simulation listed twice would be advanced twice in the same launches
+    with pytest.raises(L.UbgError) as e:
+        B.SimBatch([s1, s1])
+    assert e.value.code == L.E_INVALID
+    s4 = U.Simulation(d.flag, d.pwidth, d.mu)
+    sb = B.SimBatch([s1, s4])
+    before = s1.get("vx")
+    hs = (C.c_void_p * 2)(s1._h.value, s1._h.value)
+    dts = np.full(2, d.dt, np.float32)
+    with pytest.raises(L.UbgError) as e:
+        L.check(L.lib().ubg_sims_step(sb._h, hs, 2, L.fptr(dts)))
+    assert e.value.code == L.E_INVALID
+    np.testing.assert_array_equal(s1.get("vx"), before)  # rejected before any work
+    sb.close()
 
 
 def test_sims_step_interleaved_with_own_work(gpu):

@@ -17,6 +17,7 @@
 #include "stream_device.hpp"
 #include "tile_device.hpp"
 
+#include <algorithm>
 #include <array>
 #include <cstdio>
 #include <fstream>
@@ -1993,6 +1994,14 @@ bool same_solver(const ubg_params &a, const ubg_params &b) {
          a.vcycles_per_step == b.vcycles_per_step && a.alpha == b.alpha;
 }
 
+// whether a simulation appears more than once in the list (one batched
+// step would then advance its fields twice in the same launches)
+bool repeats(ubg_sim *const *sims, int n) {
+  std::vector<ubg_sim *> v(sims, sims + n);
+  std::sort(v.begin(), v.end());
+  return std::adjacent_find(v.begin(), v.end()) != v.end();
+}
+
 // (re)build domain d from simulation s's level-0 flag and h
 int domain_from_sim(ubg_batch *b, int d, ubg_sim *s) {
   const ubg::SimLevel0 q = ubg::sim_level0(s);
@@ -2024,6 +2033,7 @@ UBG_API int ubg_batch_create_for_sims(ubg_sim *const *sims, int n, ubg_batch **o
     ws[i] = q.W;
     hs[i] = q.H;
   }
+  if (repeats(sims, n)) return fail(UBG_E_INVALID, "a simulation appears twice in the list");
   ubg_batch *b = nullptr;
   UBG_TRY(ubg_batch_create(n, ws.data(), hs.data(), &prm, &b));
   b->src.assign(n, nullptr);
@@ -2053,6 +2063,7 @@ UBG_API int ubg_sims_step(ubg_batch *b, ubg_sim *const *sims, int n, const float
       return fail(UBG_E_INVALID, "simulation " + std::to_string(i) + " does not match its domain");
     if (!std::isfinite(dts[i])) return fail(UBG_E_INVALID, "dt is not finite");
   }
+  if (repeats(sims, n)) return fail(UBG_E_INVALID, "a simulation appears twice in the list");
   // geometry of a domain whose simulation changed its flags (or is another one)
   for (int i = 0; i < n; i++)
     if (b->src[i] != sims[i] || b->src_gen[i] != q[i].flag_gen) {
