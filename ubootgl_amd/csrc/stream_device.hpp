@@ -171,10 +171,30 @@ __device__ __forceinline__ void mg_stream_unit(const StreamArgs &A, int unit) {
   P2 px[2] = {ld2(A.src, L0), ld2(A.src, L0 + 1)};
   P2 pf[2] = {ld2(A.f, L0), ld2(A.f, L0 + 1)};
   M2 pm[2] = {ldm2(L0), ldm2(L0 + 1)};
+  // MODE_PC: the coarse solution and coarse flags under this lane's pair
+  // (columns X, X1) of one coarse row.  A trip's two fine rows t2, t2+1
+  // prolongate from coarse rows Y = t2/2 and Y+1 (clamped into the level;
+  // rows outside it only feed cells that are not corrected), and the next
+  // trip shares row Y+1, so a trip loads one coarse row, for the next trip,
+  // and the rows rotate through cr0, cr1.
+  struct CRow {
+    float e, e1;
+    int g, g1;
+  };
+  const int cX = max(0, min(x0 >> 1, A.wc - 1)), cX1 = min(cX + 1, A.wc - 1);
+  auto ldc = [&](int Y) -> CRow {
+    if (MODE != MODE_PC) return CRow{0.0f, 0.0f, 0, 0};
+    const int Yc = min(max(Y, 0), A.hc - 1);
+    const float *e = A.cpin + (size_t)Yc * A.wc;
+    const uint8_t *g = A.cmask + (size_t)Yc * A.wc;
+    return CRow{e[cX], e[cX1], g[cX] & 1, g[cX1] & 1};
+  };
+  CRow cr0 = ldc(L0 >> 1), cr1 = ldc((L0 >> 1) + 1);
   for (int t2 = L0; t2 <= TEND; t2 += 2) {
     const P2 lx0 = ld2(A.src, t2 + 2), lx1 = ld2(A.src, t2 + 3);
     const P2 lf0 = ld2(A.f, t2 + 2), lf1 = ld2(A.f, t2 + 3);
     const M2 lm0 = ldm2(t2 + 2), lm1 = ldm2(t2 + 3);
+    const CRow crn = ldc((t2 >> 1) + 2);
 #pragma unroll
     for (int par = 0; par < 2; par++) {
       const int t = t2 + par;
@@ -193,22 +213,16 @@ __device__ __forceinline__ void mg_stream_unit(const StreamArgs &A, int unit) {
       if (MODE == MODE_PC) {
         // prolongate + correct (pressure_solver.cpp:134-181) on interior cells
         if (t >= 1 && t <= h - 2) {
-          const int wc = A.wc, hc = A.hc;
-          const int X = max(0, min(x0 >> 1, wc - 1)), X1 = min(X + 1, wc - 1);
-          const int Y = min(t >> 1, hc - 1), Y1 = min(Y + 1, hc - 1);
-          const float *e0 = A.cpin + (size_t)Y * wc, *e1 = A.cpin + (size_t)Y1 * wc;
-          const uint8_t *g0 = A.cmask + (size_t)Y * wc, *g1 = A.cmask + (size_t)Y1 * wc;
+          // coarse row Y = t/2 is cr0, row min(Y + 1, hc - 1) is cr1
           const float Fa = (MQa[0] & MK_SELF) ? 1.0f : 0.0f, Fb = (MQb[0] & MK_SELF) ? 1.0f : 0.0f;
           float ea = 0.0f, eb = 0.0f;
           if (par == 0) { // t even
-            ea = e0[X] * Fa;
-            if (x1 <= w - 3)
-              eb = Fb * (e0[X] + e0[X1]) / eps_count((g0[X] & 1) + (g0[X1] & 1));
+            ea = cr0.e * Fa;
+            if (x1 <= w - 3) eb = Fb * (cr0.e + cr0.e1) / eps_count(cr0.g + cr0.g1);
           } else if (t <= h - 3) {
-            ea = Fa * (e0[X] + e1[X]) / eps_count((g0[X] & 1) + (g1[X] & 1));
+            ea = Fa * (cr0.e + cr1.e) / eps_count(cr0.g + cr1.g);
             if (x1 <= w - 3)
-              eb = Fb * (e0[X] + e1[X1] + e0[X1] + e1[X]) /
-                   eps_count((g0[X] & 1) + (g1[X1] & 1) + (g0[X1] & 1) + (g1[X] & 1));
+              eb = Fb * (cr0.e + cr1.e1 + cr0.e1 + cr1.e) / eps_count(cr0.g + cr1.g1 + cr0.g1 + cr1.g);
           }
           x.a = int0 ? x.a + ea : x.a;
           x.b = int1 ? x.b + eb : x.b;
@@ -309,6 +323,8 @@ __device__ __forceinline__ void mg_stream_unit(const StreamArgs &A, int unit) {
     pf[1] = lf1;
     pm[0] = lm0;
     pm[1] = lm1;
+    cr0 = cr1;
+    cr1 = crn;
   }
 }
 
