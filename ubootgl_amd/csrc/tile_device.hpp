@@ -236,16 +236,19 @@ tile_post_body(char *smem, int X0, int Y0, int otx, int oty, const float *__rest
   const int ex0 = gx0 >> 1, ey0 = gy0 >> 1;
   const int ex1 = min((gx1 >> 1) + 1, wc - 1), ey1 = min((gy1 >> 1) + 1, hc - 1);
   {
-    // coarse values first (one round), then the fine region
+    // coarse values, two per thread per round: the first round's loads are
+    // issued before the fine region's and stored after it, so the two
+    // memory latencies overlap (further rounds, for regions of more than
+    // 2 TPB_M coarse cells, follow one after the other)
     const int CW = ex1 - ex0 + 1, CH = ey1 - ey0 + 1, nc = CW * CH;
     // (row, column) of element threadIdx.x + TPB_M * k, stepped as in stage_fine
     const int CWs = CW > 0 ? CW : 1; // CW < 1: nc < 1, nothing is staged
     const int dq = TPB_M / CWs, dr = TPB_M - dq * CWs;
     int cy = (int)threadIdx.x / CWs, cx = (int)threadIdx.x - cy * CWs;
-    for (int base = 0; base < nc; base += 2 * TPB_M) {
-      float ve[2];
-      uint8_t vc[2];
-      int li[2];
+    float ve[2];
+    uint8_t vc[2];
+    int li[2];
+    auto load_round = [&] {
 #pragma unroll
       for (int k = 0; k < 2; k++) {
         li[k] = -1;
@@ -262,15 +265,23 @@ tile_post_body(char *smem, int X0, int Y0, int otx, int oty, const float *__rest
           cy++;
         }
       }
+    };
+    auto store_round = [&] {
 #pragma unroll
       for (int k = 0; k < 2; k++)
         if (li[k] >= 0) {
           E[li[k]] = ve[k];
           FC[li[k]] = vc[k];
         }
+    };
+    load_round();
+    stage_fine<TPB_M>(P, F, M, p, f, m, w, gx0, gy0, LW, LH, hh);
+    store_round();
+    for (int base = 2 * TPB_M; base < nc; base += 2 * TPB_M) {
+      load_round();
+      store_round();
     }
   }
-  stage_fine<TPB_M>(P, F, M, p, f, m, w, gx0, gy0, LW, LH, hh);
   __syncthreads();
   // e for every interior cell of the region (no neighbour dependence), by
   // 2x2 blocks of fine cells over one coarse cell (a, b): even-even, odd-even,
