@@ -83,15 +83,18 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseArgs &a, const fl
   const int tid = threadIdx.x;
   // load: f of the first level, masks of all levels; p = 0 everywhere.
   // Each round issues KC loads per thread before its LDS stores, so a round
-  // costs one memory latency instead of one per element.
-  constexpr int KC = 8;
-  for (int l = 0; l < a.n; l++) {
+  // costs one memory latency instead of one per element; the first
+  // TPB_C-element round of levels 1..KR rides in level 0's first round, so
+  // the small levels' masks cost no latency of their own.
+  constexpr int KC = 8, KR = 4;
+  // elements [from, n) of level l, KC per thread per round
+  auto stage = [&](int l, int from) {
     float *P = (float *)(smem + a.op[l]);
     float *Fv = (float *)(smem + a.of[l]);
     uint8_t *M = (uint8_t *)(smem + a.om[l]);
     const uint8_t *mg = a.mask[l];
     const int n = a.w[l] * a.h[l];
-    for (int base = 0; base < n; base += KC * TPB_C) {
+    for (int base = from; base < n; base += KC * TPB_C) {
       float vf[KC];
       uint8_t vm[KC];
 #pragma unroll
@@ -112,7 +115,23 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseArgs &a, const fl
         }
       }
     }
+  };
+  {
+    uint8_t rm[KR];
+#pragma unroll
+    for (int r = 0; r < KR; r++)
+      if (1 + r < a.n && tid < a.w[1 + r] * a.h[1 + r]) rm[r] = a.mask[1 + r][tid];
+    stage(0, 0);
+#pragma unroll
+    for (int r = 0; r < KR; r++)
+      if (1 + r < a.n && tid < a.w[1 + r] * a.h[1 + r]) {
+        const int l = 1 + r;
+        ((float *)(smem + a.op[l]))[tid] = 0.0f;
+        ((float *)(smem + a.of[l]))[tid] = 0.0f;
+        ((uint8_t *)(smem + a.om[l]))[tid] = rm[r];
+      }
   }
+  for (int l = 1; l < a.n; l++) stage(l, l <= KR ? TPB_C : 0);
   __syncthreads();
 
   // descend
