@@ -29,12 +29,13 @@ struct CoarseArgs {
 // 1024 threads as 16 waves x 64 lanes.  A level row with at most `count`
 // items (cells of a colour, cells, blocks) takes L lanes, L the smallest
 // power of two >= count (at most 64), so a wave holds G = 64 / L rows at
-// once: lane group g of wave ty starts at row offset ty + 16 g and steps by
-// 16 G.  On the narrow coarse levels this packs 2-8 rows into one wave
-// instead of walking them one after another.
+// once: lane group g of wave ty starts at row offset G ty + g and steps by
+// 16 G.  On the narrow coarse levels this packs 2-8 consecutive rows into
+// one wave, so a level of few rows keeps few waves busy (and few per SIMD)
+// instead of one partly filled wave per row.
 struct RowGroups {
   int lr;   // lane within the row group
-  int r0;   // first row offset (wave + 16 * group)
+  int r0;   // first row offset (G * wave + group)
   int step; // row step, 16 * G
   int L;    // lanes per row
 };
@@ -46,27 +47,90 @@ __device__ __forceinline__ RowGroups row_groups(int count) {
   RowGroups r;
   r.L = 1 << lg;
   r.lr = tx & (r.L - 1);
-  r.r0 = ty + WAVES_C * (tx >> lg);
+  r.r0 = (ty << (6 - lg)) + (tx >> lg);
   r.step = WAVES_C << (6 - lg);
   return r;
 }
 
-// One red+black sweep of a w x h level in LDS.  A row's cells of a colour
+// A cell's smoothing constants, decoded once from its mask byte and f for
+// sweeps that update the same cell again and again.  smooth_rule performs
+// the operations of smooth() in the same order (the division by the count
+// as div_count does it), so the results are identical.
+struct CellRule {
+  int sw, se, ss, sn; // 0 / -1: the neighbour's value is kept, or +0 added
+  float inv, ns;      // RN(1 / s) and -s for the neighbour count s
+  float F, fhh;       // own flag (0 / 1) and (f * h) * h
+  bool empty;         // s == 0: the quotient is +0
+};
+__device__ __forceinline__ CellRule cell_rule(unsigned mk, float fv, float h) {
+  CellRule c;
+  c.sw = -(int)((mk >> 1) & 1u);
+  c.se = -(int)((mk >> 2) & 1u);
+  c.ss = -(int)((mk >> 3) & 1u);
+  c.sn = -(int)((mk >> 4) & 1u);
+  const unsigned s = mk >> 5;
+  const float pow2 = __int_as_float((int)((127u - (s >> 1)) << 23));
+  c.inv = (s == 3u) ? (1.0f / 3.0f) : pow2;
+  c.ns = -(float)s;
+  c.empty = s == 0u;
+  c.F = (float)(mk & MK_SELF);
+  c.fhh = fv * h * h;
+  return c;
+}
+__device__ __forceinline__ float smooth_rule(float pc, float pw, float pe, float ps, float pn,
+                                             const CellRule &c, float alpha) {
+  float val = 0.0f;
+  val += __int_as_float(__float_as_int(pw) & c.sw);
+  val += __int_as_float(__float_as_int(pe) & c.se);
+  val += __int_as_float(__float_as_int(ps) & c.ss);
+  val += __int_as_float(__float_as_int(pn) & c.sn);
+  val += c.fhh;
+  const float q = val * c.inv;
+  const float r = fmaf(c.ns, q, val);
+  const float q2 = fmaf(r, c.inv, q);
+  val = c.empty ? 0.0f : q2;
+  return c.F * (alpha * val + (1.0f - alpha) * pc);
+}
+
+// nu red+black sweeps of a w x h level in LDS.  A row's cells of a colour
 // are at most (w - 1) / 2; rows of one lane group share the colour parity
-// (their offsets differ by multiples of 16).
-__device__ __forceinline__ void lds_sweep(float *P, const float *Fv, const uint8_t *M, int w,
-                                          int h, float hh, float alpha) {
+// (their offsets differ by multiples of 16).  When the lane groups cover
+// every interior row and every cell of a colour at once (levels of at most
+// 130 x 18, 66 x 34 or 34 x 66 cells), each thread owns at most one cell of
+// each colour for the whole smoothing: its cells' rules
+// are decoded once, and a colour phase is the five reads of p, the update
+// and one write.
+__device__ __forceinline__ void lds_sweeps(float *P, const float *Fv, const uint8_t *M, int w,
+                                           int h, float hh, float alpha, int nu) {
   const RowGroups q = row_groups((w - 1) >> 1);
-  for (int color = 0; color < 2; color++) {
-    const int x0 = 1 + ((1 + q.r0 + color) & 1) + 2 * q.lr;
-    for (int y = 1 + q.r0; y < h - 1; y += q.step) {
-      for (int x = x0; x < w - 1; x += 2 * q.L) {
-        int i = y * w + x;
-        P[i] = smooth(P[i], P[i - 1], P[i + 1], P[i - w], P[i + w], Fv[i], M[i], hh, alpha);
-      }
+  if (q.step >= h - 2 && 2 * q.L >= w - 2) {
+    const int y = 1 + q.r0;
+    const int xa = 1 + ((1 + q.r0) & 1) + 2 * q.lr; // colour 0
+    const int xb = 1 + ((2 + q.r0) & 1) + 2 * q.lr; // colour 1
+    const bool oka = y < h - 1 && xa < w - 1, okb = y < h - 1 && xb < w - 1;
+    const int ia = y * w + xa, ib = y * w + xb;
+    CellRule ra{}, rb{};
+    if (oka) ra = cell_rule(M[ia], Fv[ia], hh);
+    if (okb) rb = cell_rule(M[ib], Fv[ib], hh);
+    for (int s = 0; s < nu; s++) {
+      if (oka) P[ia] = smooth_rule(P[ia], P[ia - 1], P[ia + 1], P[ia - w], P[ia + w], ra, alpha);
+      __syncthreads();
+      if (okb) P[ib] = smooth_rule(P[ib], P[ib - 1], P[ib + 1], P[ib - w], P[ib + w], rb, alpha);
+      __syncthreads();
     }
-    __syncthreads();
+    return;
   }
+  for (int s = 0; s < nu; s++)
+    for (int color = 0; color < 2; color++) {
+      const int x0 = 1 + ((1 + q.r0 + color) & 1) + 2 * q.lr;
+      for (int y = 1 + q.r0; y < h - 1; y += q.step) {
+        for (int x = x0; x < w - 1; x += 2 * q.L) {
+          int i = y * w + x;
+          P[i] = smooth(P[i], P[i - 1], P[i + 1], P[i - w], P[i + w], Fv[i], M[i], hh, alpha);
+        }
+      }
+      __syncthreads();
+    }
 }
 
 __device__ __forceinline__ float lds_residual(const float *P, const float *Fv, const uint8_t *M,
@@ -140,7 +204,7 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseArgs &a, const fl
     float *Fv = (float *)(smem + a.of[l]);
     const uint8_t *M = (const uint8_t *)(smem + a.om[l]);
     const int w = a.w[l], h = a.h[l];
-    for (int s = 0; s < a.nu_pre; s++) lds_sweep(P, Fv, M, w, h, a.hh[l], a.alpha);
+    lds_sweeps(P, Fv, M, w, h, a.hh[l], a.alpha, a.nu_pre);
     // residual (pressure_solver.cpp:91-116) of every interior cell once,
     // then restrict (:118-132) into level l+1, whose p starts at 0
     float *Rr = (float *)(smem + a.orr);
@@ -173,7 +237,7 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseArgs &a, const fl
     float *P = (float *)(smem + a.op[l]);
     float *Fv = (float *)(smem + a.of[l]);
     const uint8_t *M = (const uint8_t *)(smem + a.om[l]);
-    for (int s = 0; s < a.nu_coarse; s++) lds_sweep(P, Fv, M, a.w[l], a.h[l], a.hh[l], a.alpha);
+    lds_sweeps(P, Fv, M, a.w[l], a.h[l], a.hh[l], a.alpha, a.nu_coarse);
   }
   // ascend
   for (int l = a.n - 2; l >= 0; l--) {
@@ -217,7 +281,7 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseArgs &a, const fl
         }
     }
     __syncthreads();
-    for (int s = 0; s < a.nu_post; s++) lds_sweep(P, Fv, M, w, h, a.hh[l], a.alpha);
+    lds_sweeps(P, Fv, M, w, h, a.hh[l], a.alpha, a.nu_post);
   }
   // write the first level's solution
   {
