@@ -35,6 +35,15 @@ __device__ __forceinline__ float keep_if(unsigned mk, float v) {
   return __int_as_float(__float_as_int(v) & sel);
 }
 
+// v if bit K of the mask is set, else c (a bitwise select on the bit
+// sign-extended to 0 / -1)
+template <int K>
+__device__ __forceinline__ float pick_if(unsigned mk, float v, float c) {
+  int sel;
+  asm("v_bfe_i32 %0, %1, %2, 1" : "=v"(sel) : "v"(mk), "i"(K));
+  return __int_as_float((__float_as_int(v) & sel) | (__float_as_int(c) & ~sel));
+}
+
 // smoothingKernel (pressure_solver.cpp:10-24) from neighbour values and mask;
 // the neighbour-flag count is stored in mask bits 5-7.
 // ... with the source term fhh = (f * h) * h already formed
@@ -58,16 +67,19 @@ __device__ __forceinline__ float smooth(float pc, float pw, float pe, float ps, 
 // calculateResidualField for one cell (pressure_solver.cpp:100-108):
 // r = F*(f + (sum_n(F_n p_n + (1-F_n) p) - 4p)/h^2), neighbours in the order
 // x-1, x+1, y-1, y+1.
+// The selects are bitwise on the mask bit sign-extended to 0 / -1 (two
+// instructions each instead of an and / compare / select triple); the
+// values are those of the conditional expressions.
 __device__ __forceinline__ float residual_of(float pc, float pw, float pe, float ps, float pn,
                                              float fv, unsigned mk, float ihsq) {
   float val = 0.0f;
-  val += (mk & MK_W) ? pw : pc;
-  val += (mk & MK_E) ? pe : pc;
-  val += (mk & MK_S) ? ps : pc;
-  val += (mk & MK_N) ? pn : pc;
+  val += pick_if<1>(mk, pw, pc); // (mk & MK_W) ? pw : pc
+  val += pick_if<2>(mk, pe, pc);
+  val += pick_if<3>(mk, ps, pc);
+  val += pick_if<4>(mk, pn, pc);
   val += -4.0f * pc;
   val *= ihsq;
-  return (mk & MK_SELF) ? (fv + val) : 0.0f;
+  return keep_if<0>(mk, fv + val); // (mk & MK_SELF) ? fv + val : +0
 }
 
 // Whole-wave lane shifts by one (DPP wave_shr:1 / wave_shl:1, a VALU move):

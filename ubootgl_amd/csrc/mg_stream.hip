@@ -38,13 +38,20 @@ namespace {
 
 constexpr int TPB = 256;
 
-// one wave per unit (strip x row block) of the level; body in stream_device.hpp
+// one wave per unit (strip x row block) of the level; body in stream_device.hpp.
+// The smoothing updates take the mask's constants from an LDS table (8 KiB
+// per workgroup) instead of decoding the byte: these passes are bound by
+// VALU issue, and the decode is about 15 of the ~38 instructions of an
+// update.
 template <int NS, int MODE, bool ZG, bool AL>
 __global__ void __launch_bounds__(TPB) k_mg_stream(const StreamArgs A) {
+  __shared__ uint4 tab[kRuleEntries];
+  fill_rule_table(tab);
+  __syncthreads();
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int unit = xcd_tile(blockIdx.x, gridDim.x) * (TPB / 64) + wid;
   if (unit >= A.nunits) return; // whole wave leaves together
-  mg_stream_unit<NS, MODE, ZG, AL>(A, unit);
+  mg_stream_unit<NS, MODE, ZG, AL, true>(A, unit, tab);
 }
 
 template <int NS, int MODE>

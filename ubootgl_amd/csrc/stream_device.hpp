@@ -64,8 +64,48 @@ struct M2 {
   unsigned a, b;
 };
 
-template <int NS, int MODE, bool ZG, bool AL>
-__device__ __forceinline__ void mg_stream_unit(const StreamArgs &A, int unit) {
+// The smoothing constants of every mask byte, for passes that look them up
+// in LDS instead of decoding the byte at each update: entry 2 mk holds the
+// four neighbour selectors (0 / -1 for the bits W, E, S, N), entry 2 mk + 1
+// RN(1/s) and -s for the neighbour count s, the quotient's select (0 when
+// s == 0, else -1) and F, as bits.  Filled by the whole workgroup.
+constexpr int kRuleEntries = 512;
+__device__ __forceinline__ void fill_rule_table(uint4 *tab) {
+  for (unsigned mk = threadIdx.x; mk < 256u; mk += blockDim.x) {
+    const unsigned s = mk >> 5;
+    const float pow2 = __int_as_float((int)((127u - (s >> 1)) << 23));
+    const float inv = (s == 3u) ? (1.0f / 3.0f) : pow2;
+    tab[2 * mk] = make_uint4(0u - ((mk >> 1) & 1u), 0u - ((mk >> 2) & 1u), 0u - ((mk >> 3) & 1u),
+                             0u - ((mk >> 4) & 1u));
+    tab[2 * mk + 1] = make_uint4(__float_as_uint(inv), __float_as_uint(-(float)s),
+                                 s == 0u ? 0u : ~0u, __float_as_uint((float)(mk & MK_SELF)));
+  }
+}
+// smooth_fhh with the mask's constants from the table: the same operations
+// in the same order (keep_if, div_count, the zero-count case, F), so the
+// same value.
+__device__ __forceinline__ float smooth_tab(float pc, float pw, float pe, float ps, float pn,
+                                            float fhh, const uint4 *tab, unsigned mk,
+                                            float alpha) {
+  const uint4 nb = tab[2 * mk], dc = tab[2 * mk + 1];
+  float val = 0.0f;
+  val += __uint_as_float(__float_as_uint(pw) & nb.x);
+  val += __uint_as_float(__float_as_uint(pe) & nb.y);
+  val += __uint_as_float(__float_as_uint(ps) & nb.z);
+  val += __uint_as_float(__float_as_uint(pn) & nb.w);
+  val += fhh;
+  const float inv = __uint_as_float(dc.x);
+  const float q = val * inv;
+  const float r = fmaf(__uint_as_float(dc.y), q, val);
+  const float q2 = fmaf(r, inv, q);
+  val = __uint_as_float(__float_as_uint(q2) & dc.z);
+  return __uint_as_float(dc.w) * (alpha * val + (1.0f - alpha) * pc);
+}
+
+// TAB: the smoothing constants come from `tab` (fill_rule_table, in LDS).
+template <int NS, int MODE, bool ZG, bool AL, bool TAB = false>
+__device__ __forceinline__ void mg_stream_unit(const StreamArgs &A, int unit,
+                                               const uint4 *tab = nullptr) {
   constexpr int HALO = NS + 2;
   constexpr int OWN = 64 - 2 * HALO;
   // stage k (0-based) consumes row t-3k at iteration t (stage k-1's output
@@ -117,11 +157,19 @@ __device__ __forceinline__ void mg_stream_unit(const StreamArgs &A, int unit) {
     return M2{row[c0], row[c1]};
   };
   auto relax_a = [&](P2 v, float dn, float up, float fv, unsigned mk) {
-    float na = smooth(v.a, from_left(v.b), v.b, dn, up, fv, mk, A.hh, A.alpha);
+    float na;
+    if constexpr (TAB)
+      na = smooth_tab(v.a, from_left(v.b), v.b, dn, up, fv * A.hh * A.hh, tab, mk, A.alpha);
+    else
+      na = smooth(v.a, from_left(v.b), v.b, dn, up, fv, mk, A.hh, A.alpha);
     return int0 ? na : v.a;
   };
   auto relax_b = [&](P2 v, float dn, float up, float fv, unsigned mk) {
-    float nb = smooth(v.b, v.a, from_right(v.a), dn, up, fv, mk, A.hh, A.alpha);
+    float nb;
+    if constexpr (TAB)
+      nb = smooth_tab(v.b, v.a, from_right(v.a), dn, up, fv * A.hh * A.hh, tab, mk, A.alpha);
+    else
+      nb = smooth(v.b, v.a, from_right(v.a), dn, up, fv, mk, A.hh, A.alpha);
     return int1 ? nb : v.b;
   };
   // side-column setZeroGradientBC of an interior row
